@@ -1,0 +1,173 @@
+/* windtunnel.h -- C ABI of the MI355X-native D3Q19 lattice-Boltzmann and
+ * voxelizer library (libwindtunnel.so, HIP for gfx950).
+ *
+ * The ABI is the boundary a host binding (the Node N-API addon in
+ * windtunnel_amd/addon, the Python ctypes binding in windtunnel_amd/lbm.py)
+ * calls to replace the reference's solver object and voxelizer:
+ *
+ *   js/lbm.js:51-546     class LBM            -> wt_create ... wt_destroy
+ *   js/lbmWorker.js:35-80 worker protocol     -> wt_step_slabs, wt_set_parity
+ *   js/pool.js:97-125    Pool.step totals     -> wt_step_slabs (per-slab forces)
+ *   js/voxel.js:32-102   Voxel.bounds/build   -> wt_voxel_bounds, wt_voxel_build
+ *   js/shapes.js:369-388 grid-backed makeTest -> wt_voxelize_grid
+ *
+ * Conventions: every function returns 0 on success and a negative WT_E*
+ * code on failure; wt_last_error() then names the failure (thread-local
+ * text, valid until the next failing call on the same thread). Handles are
+ * not thread-safe: one host thread drives one lattice. Arrays are plain
+ * host pointers; the library copies to and from its own device memory.
+ */
+#ifndef WINDTUNNEL_H
+#define WINDTUNNEL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WT_ABI_VERSION 1
+
+enum {
+    WT_OK = 0,
+    WT_E_ARG = -1,      /* bad argument (dimension, size, enum, null pointer) */
+    WT_E_DEVICE = -2,   /* HIP runtime error, or no usable gfx950 device */
+    WT_E_NOMEM = -3,    /* device or host allocation failed */
+    WT_E_STATE = -4,    /* call not valid in the handle's current state */
+    WT_E_FAULT = -5,    /* injected fault (WT_FAULT environment variable) */
+    WT_E_VOX_EMPTY = -11,      /* "Mesh has no triangles." (js/voxel.js:51) */
+    WT_E_VOX_DEGENERATE = -12, /* "Mesh is degenerate (zero size)." (js/voxel.js:56) */
+    WT_E_VOX_TOO_LARGE = -13,  /* "Voxel grid too large; lower the resolution." (js/voxel.js:71) */
+    WT_E_VOX_NONFINITE = -14   /* a vertex coordinate is NaN or infinite */
+};
+
+/* Population storage. F32 is the reference's Float32Array of deviations
+ * f_q - w_q (js/lbm.js:74-75,145-162); F16 stores the same deviations as
+ * scaled binary16; F64 stores binary64 (the FP64 oracle run). */
+enum { WT_STORE_F32 = 0, WT_STORE_F16 = 1, WT_STORE_F64 = 2 };
+
+/* Collision arithmetic. F64 evaluates the reference's binary64 expression
+ * tree with contraction off (with F32 storage: the reference's own numbers);
+ * F32 is the fast path. */
+enum { WT_ARITH_F32 = 0, WT_ARITH_F64 = 1 };
+
+/* Population layout in HBM. AB = two buffers swapped by parity, the
+ * reference's scheme (js/lbm.js:71-80,130-132); ESOTERIC = one buffer
+ * streamed in place (esoteric pull). */
+enum { WT_LAYOUT_AB = 0, WT_LAYOUT_ESOTERIC = 1 };
+
+typedef struct wt_lattice wt_lattice;
+
+/* Solver knobs of js/lbm.js:106-111 (settable properties of LBM). */
+typedef struct {
+    double tau;      /* BGK / symmetric relaxation time tau0 */
+    double csm;      /* Smagorinsky constant */
+    double lambda;   /* TRT magic parameter */
+    double u0;       /* inflow speed, lattice units */
+    int les;         /* Smagorinsky eddy viscosity on/off */
+    int trt;         /* two-relaxation-time collision on/off (off = BGK) */
+    int movingWalls; /* apply the moving-wall bounce-back correction */
+} wt_params;
+
+/* Results of LBM.voxelize (js/lbm.js:247-265). */
+typedef struct {
+    int64_t frontalArea, planformArea, solidCells, rideHeight;
+    int32_t movingWalls, ground;
+} wt_body_stats;
+
+/* Observable scalar state of an LBM object (js/lbm.js:80,113,117). */
+typedef struct {
+    int64_t parity, steps;
+    double force[3];
+} wt_state_t;
+
+/* Voxel grid geometry and statistics: the object Voxel.build returns
+ * (js/voxel.js:89-101), minus the grid itself. */
+typedef struct {
+    int32_t nx, ny, nz, ntri;
+    double cell, inv, min[3], ext[3], centre[3], bbox[6];
+    int64_t solid, disagree;
+    double openness;
+} wt_voxmeta;
+
+const char *wt_last_error(void);
+int wt_abi_version(void);
+
+/* new LBM(nx,ny,nz) including its reset() (js/lbm.js:58-128). */
+int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lattice **out);
+int wt_destroy(wt_lattice *h);
+
+/* Property setters/getters of js/lbm.js:106-111 (and the worker 'params'
+ * message, js/lbmWorker.js:53-56). */
+int wt_set_params(wt_lattice *h, const wt_params *p);
+int wt_get_params(const wt_lattice *h, wt_params *p);
+
+/* reset(): equilibrium at u0 in fluid, at rest in solid, both buffers
+ * (js/lbm.js:164-174). */
+int wt_reset(wt_lattice *h);
+
+/* voxelize(inside, box, ground, wallVel, roadVel) (js/lbm.js:191-266), with
+ * the host closures already evaluated: box = the clamped box
+ * {x0,y0,z0,x1,y1,z1} (inclusive, within [1,n-2] per axis), mask one byte
+ * per box cell in x-fastest order (0 outside, 1 inside, 2 inside with a wall
+ * velocity), wv three floats per mask==2 cell in the same order. */
+int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const float *wv, int ground,
+                double roadVel, wt_body_stats *out);
+
+/* Grid-backed body (js/shapes.js:369-388): the voxel lookup is evaluated on
+ * the GPU for every cell centre of the clamped box, then voxelize proceeds
+ * as above without wall velocities. grid is nx*ny*nz bytes of a voxel grid
+ * (wt_voxel_build), M the 3x3 body->world rotation (row-major), pos the body
+ * origin in lattice cells, scale cells per body unit. */
+int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, const double M[9],
+                     const double pos[3], double scale, const int box[6], int ground,
+                     double roadVel, wt_body_stats *out);
+
+/* step() nsteps times (js/lbm.js:285-291). forces (may be NULL) receives the
+ * 3 force components after each step, FP64, nsteps*3 doubles. */
+int wt_step(wt_lattice *h, int nsteps, double *forces);
+
+/* Worker-pool form (js/pool.js:97-125, js/lbmWorker.js:43-79): the
+ * interior is split into nslabs z-slabs as the worker 'init' handler does
+ * and each step runs slab by slab; forces (may be NULL) receives
+ * nsteps*nslabs*3 doubles, [step][slab][xyz], the per-worker totals the
+ * pool sums in worker order. */
+int wt_step_slabs(wt_lattice *h, int nsteps, int nslabs, double *forces);
+
+/* The worker 'step' message carries the page's parity (js/lbmWorker.js:60). */
+int wt_set_parity(wt_lattice *h, int parity);
+int wt_get_state(const wt_lattice *h, wt_state_t *out);
+
+/* Macroscopic fields rho, ux, uy, uz (js/lbm.js:85-88,385): element type is
+ * float, or double for WT_STORE_F64; any pointer may be NULL. The values are
+ * those of the last completed step (they are written on the last step of
+ * every wt_step / wt_step_slabs call). */
+int wt_get_fields(wt_lattice *h, void *rho, void *ux, void *uy, void *uz);
+
+/* Populations f (which=0, current) or g (which=1) in the reference's AB
+ * order f[q*n + cell] as deviations f_q - w_q: float, or double for
+ * WT_STORE_F64 (F16 storage is decoded to float). With the esoteric layout
+ * only which=0 is available; solid cells read as 0 there. */
+int wt_get_pops(wt_lattice *h, int which, void *out);
+
+/* solid (0 fluid, 1 body, 2 road) and near-wall flags (js/lbm.js:82-83). */
+int wt_get_solid(wt_lattice *h, uint8_t *solid, uint8_t *near);
+
+/* Device time of nsteps steps, measured with HIP events on the lattice's
+ * stream (no host sync inside the timed region). */
+int wt_time_steps(wt_lattice *h, int nsteps, double *ms);
+
+/* Voxel.bounds (js/voxel.js:32-42) over len floats. */
+int wt_voxel_bounds(const float *P, int64_t len, double out[6]);
+
+/* Voxel.build(P,res) geometry only (js/voxel.js:48-71): fills vm. */
+int wt_voxel_plan(const float *P, int64_t len, int res, wt_voxmeta *vm);
+
+/* Voxel.build(P,res) on the GPU (js/voxel.js:48-102): grid receives
+ * nx*ny*nz bytes (size them with wt_voxel_plan), vm the geometry and stats. */
+int wt_voxel_build(const float *P, int64_t len, int res, wt_voxmeta *vm, uint8_t *grid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

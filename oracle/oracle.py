@@ -1,0 +1,207 @@
+"""ctypes binding of the CPU oracle (libwt_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke`` and
+the CPU-baseline leg of ``bench.py``, never by the product package.
+
+The oracle restates js/lbm.js, js/lbmWorker.js, js/pool.js and js/voxel.js
+in C with binary64 arithmetic in the reference's operation order; see
+oracle/wt_oracle.c for the per-function citations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwt_oracle.so")
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P, I, D, V = C.c_void_p, C.c_int, C.c_double, None
+        sig = {
+            "wto_create": (P, [I, I, I, I]),
+            "wto_destroy": (V, [P]),
+            "wto_set_params": (V, [P, D, I, D, I, D, D, I]),
+            "wto_reset": (V, [P]),
+            "wto_voxelize": (V, [P, P, P, P, I, D, P]),
+            "wto_step": (V, [P, I, P]),
+            "wto_step_pool": (I, [P, I, I, I, P]),
+            "wto_step_slab": (V, [P, I, I, P]),
+            "wto_boundaries": (V, [P, I, I, I, I, I]),
+            "wto_flip": (V, [P]),
+            "wto_array": (P, [P, I]),
+            "wto_state": (V, [P, P, P]),
+            "wto_sample": (V, [P, D, D, D, P]),
+            "wto_is_solid": (I, [P, D, D, D]),
+            "wto_voxel_bounds": (V, [P, C.c_long, P]),
+            "wto_voxel_plan": (I, [P, C.c_long, I, P]),
+            "wto_voxel_build": (I, [P, C.c_long, I, P, P]),
+            "wto_grid_mask": (V, [P, P, P, D, P, P, D, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+ARRAYS = {"f": 0, "g": 1, "solid": 2, "near": 3, "ux": 4, "uy": 5, "uz": 6, "rho": 7,
+          "uwx": 8, "uwy": 9, "uwz": 10, "bufA": 11, "bufB": 12}
+
+
+class OracleLBM:
+    """The reference ``LBM`` object restated on the CPU (js/lbm.js:51-546)."""
+
+    def __init__(self, nx: int, ny: int, nz: int, prec: int = 4):
+        self.nx, self.ny, self.nz, self.prec = nx, ny, nz, prec
+        self.n = nx * ny * nz
+        self._h = lib().wto_create(nx, ny, nz, prec)
+        if not self._h:
+            raise ValueError("bad lattice dimensions")
+        self._params = dict(tau=0.5035, les=True, csm=0.14, trt=True, **{"lambda": 0.25},
+                            u0=0.06, movingWalls=False)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().wto_destroy(self._h)
+            self._h = None
+
+    # -- parameters -----------------------------------------------------
+    def set_params(self, **kw):
+        p = self._params
+        for k, v in kw.items():
+            if k not in p:
+                raise KeyError(k)
+            p[k] = v
+        lib().wto_set_params(self._h, float(p["tau"]), int(bool(p["les"])), float(p["csm"]),
+                             int(bool(p["trt"])), float(p["lambda"]), float(p["u0"]),
+                             int(bool(p["movingWalls"])))
+
+    def params(self) -> dict:
+        return dict(self._params)
+
+    # -- state ----------------------------------------------------------
+    def array(self, name: str) -> np.ndarray:
+        which = ARRAYS[name]
+        p = lib().wto_array(self._h, which)
+        if name in ("solid", "near"):
+            dt, cnt = np.uint8, self.n
+        elif name.startswith("uw"):
+            dt, cnt = np.float32, self.n
+        elif name in ("f", "g", "bufA", "bufB"):
+            dt, cnt = (np.float32 if self.prec == 4 else np.float64), 19 * self.n
+        else:
+            dt, cnt = (np.float32 if self.prec == 4 else np.float64), self.n
+        buf = (C.c_char * (cnt * np.dtype(dt).itemsize)).from_address(p)
+        return np.frombuffer(buf, dtype=dt, count=cnt)
+
+    def state(self) -> dict:
+        s = np.zeros(8, dtype=np.int64)
+        f = np.zeros(3)
+        lib().wto_state(self._h, _ptr(s), _ptr(f))
+        keys = ["parity", "steps", "frontalArea", "planformArea", "solidCells", "rideHeight",
+                "movingWalls", "ground"]
+        d = {k: int(v) for k, v in zip(keys, s)}
+        d["force"] = f
+        return d
+
+    # -- operations ------------------------------------------------------
+    def reset(self):
+        lib().wto_reset(self._h)
+
+    def voxelize_mask(self, box, mask, wv=None, ground=False, roadVel=0.0) -> dict:
+        box = np.ascontiguousarray(box, dtype=np.int32)
+        mask = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
+        wv = np.ascontiguousarray(wv if wv is not None else np.zeros(0), dtype=np.float32)
+        stats = np.zeros(5, dtype=np.int64)
+        lib().wto_voxelize(self._h, _ptr(box), _ptr(mask), _ptr(wv), int(bool(ground)),
+                           float(roadVel), _ptr(stats))
+        self._params["movingWalls"] = bool(stats[4])
+        return dict(frontalArea=int(stats[0]), planformArea=int(stats[1]),
+                    solidCells=int(stats[2]), rideHeight=int(stats[3]), movingWalls=bool(stats[4]))
+
+    def step(self, n: int = 1) -> np.ndarray:
+        f = np.zeros((n, 3))
+        lib().wto_step(self._h, n, _ptr(f))
+        return f
+
+    def step_pool(self, n: int, nworkers: int, threaded: bool = False) -> np.ndarray:
+        f = np.zeros((n, 3))
+        rc = lib().wto_step_pool(self._h, n, nworkers, int(threaded), _ptr(f))
+        if rc:
+            raise ValueError("bad worker count")
+        return f
+
+    def sample(self, x, y, z):
+        out = np.zeros(3)
+        lib().wto_sample(self._h, float(x), float(y), float(z), _ptr(out))
+        return out
+
+    def is_solid(self, x, y, z) -> bool:
+        return bool(lib().wto_is_solid(self._h, float(x), float(y), float(z)))
+
+
+class VoxMeta(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ntri", C.c_int),
+                ("cell", C.c_double), ("inv", C.c_double), ("min", C.c_double * 3),
+                ("ext", C.c_double * 3), ("centre", C.c_double * 3), ("bbox", C.c_double * 6),
+                ("solid", C.c_long), ("disagree", C.c_long), ("openness", C.c_double)]
+
+
+VOXEL_ERRORS = {-1: "Mesh has no triangles.", -2: "Mesh is degenerate (zero size).",
+                -3: "Voxel grid too large; lower the resolution."}
+
+
+def voxel_bounds(P: np.ndarray) -> list:
+    P = np.ascontiguousarray(P, dtype=np.float32).reshape(-1)
+    b = np.zeros(6)
+    lib().wto_voxel_bounds(_ptr(P), P.size, _ptr(b))
+    return list(b)
+
+
+def voxel_build(P: np.ndarray, res: int = 96) -> dict:
+    """``Voxel.build(P,res)`` restated (js/voxel.js:48-102)."""
+    P = np.ascontiguousarray(P, dtype=np.float32).reshape(-1)
+    m = VoxMeta()
+    rc = lib().wto_voxel_plan(_ptr(P), P.size, int(res), C.byref(m))
+    if rc:
+        raise ValueError(VOXEL_ERRORS[rc])
+    grid = np.zeros(m.nx * m.ny * m.nz, dtype=np.uint8)
+    lib().wto_voxel_build(_ptr(P), P.size, int(res), C.byref(m), _ptr(grid))
+    return dict(grid=grid, nx=m.nx, ny=m.ny, nz=m.nz, cell=m.cell, min=list(m.min), inv=m.inv,
+                ext=list(m.ext), centre=list(m.centre), bbox=list(m.bbox),
+                stats=dict(tris=m.ntri, solid=m.solid, openness=m.openness), disagree=m.disagree)
+
+
+def grid_mask(grid, gdim, gmin, ginv, M, pos, scale, box) -> np.ndarray:
+    """Grid-backed ``Body.makeTest`` evaluated over a clamped lattice box."""
+    grid = np.ascontiguousarray(grid, dtype=np.uint8)
+    gdim = np.ascontiguousarray(gdim, dtype=np.int32)
+    gmin = np.ascontiguousarray(gmin, dtype=np.float64)
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    box = np.ascontiguousarray(box, dtype=np.int32)
+    n = (box[3] - box[0] + 1) * (box[4] - box[1] + 1) * (box[5] - box[2] + 1)
+    mask = np.zeros(max(int(n), 0), dtype=np.uint8)
+    lib().wto_grid_mask(_ptr(grid), _ptr(gdim), _ptr(gmin), float(ginv), _ptr(M), _ptr(pos),
+                        float(scale), _ptr(box), _ptr(mask))
+    return mask

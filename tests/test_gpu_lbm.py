@@ -1,0 +1,224 @@
+"""GPU parity of the fused D3Q19 step (libwindtunnel on gfx950).
+
+* binary64 collision with float storage must reproduce the reference's own
+  Float32Array contents bit for bit: population buffers, macroscopic fields,
+  solid/near maps against the golden hashes written from js/lbm.js;
+* the float32 fast path must stay within relative L2 1e-4 of the oracle on
+  density, velocity and C_p, and of the force-coefficient series, at
+  laminar Reynolds numbers;
+* FP64 storage must equal the oracle's FP64-storage run bit for bit.
+"""
+import numpy as np
+import pytest
+
+from _util import arr, load, rel_l2, replay_voxelize, set_params, sha, unbits
+from oracle.oracle import OracleLBM
+from windtunnel_amd.lbm import LBM
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "ux", "uy", "uz")
+
+
+def make(fx, **kw):
+    L = LBM(*fx["dims"], **kw)
+    set_params(L, fx["params"])
+    L.reset()
+    return L
+
+
+def oracle_for(fx, prec=4):
+    O = OracleLBM(*fx["dims"], prec=prec)
+    set_params(O, fx["params"])
+    O.reset()
+    replay_voxelize(O, fx["voxelize"])
+    return O
+
+
+def check_hashes(L, want, keys=("f", "g", "rho", "ux", "uy", "uz")):
+    got = {"f": sha(L.f), "g": sha(L.g)}
+    F = L.fields()
+    for k in FIELDS:
+        got[k] = sha(F[k])
+    bad = [k for k in keys if got[k] != want[k]]
+    assert not bad, f"arrays differ from the reference: {bad}"
+    assert L.parity == want["parity"] and L.steps == want["steps"]
+
+
+@pytest.mark.parametrize("name", ["lbm_bgk_sphere", "lbm_trt_les", "lbm_moving", "lbm_outlet_body", "lbm_revox"])
+def test_exact_mode_bitwise(name):
+    fx = load(name)
+    L = make(fx, arith="f64")
+    print(f"[exact] {name} dims={fx['dims']}", flush=True)
+    replay_voxelize(L, fx["voxelize"])
+    s, nr = L.solid_near()
+    assert sha(s) == fx["voxelize"]["after"]["solid"]
+    assert sha(nr) == fx["voxelize"]["after"]["near"]
+    want_f = arr(fx["forces"])
+    done = 0
+    got = []
+    for stop in sorted(int(k) for k in fx["snaps"] if k != "raw"):
+        got.append(L.step(stop - done))
+        done = stop
+        check_hashes(L, fx["snaps"][str(stop)])
+    got = np.concatenate(got)
+    # the device sums the force in a tree order; the reference sums cell by cell
+    np.testing.assert_allclose(got, want_f[:done], rtol=1e-11, atol=1e-14)
+    if "revox" in fx:
+        replay_voxelize(L, fx["revox"])
+        f2 = L.step(30)
+        check_hashes(L, fx["snaps2"]["60"])
+        np.testing.assert_allclose(f2, arr(fx["forces2"]), rtol=1e-11, atol=1e-14)
+
+
+def test_exact_mode_init_and_voxelize():
+    fx = load("lbm_init")
+    L = LBM(*fx["dims"], arith="f64")
+    L.u0 = fx["u0"]
+    L.reset()
+    check_hashes(L, fx["reset"])
+    for rec in fx["voxelize"]:
+        replay_voxelize(L, rec)
+        check_hashes(L, rec["after"])
+        assert L.frontalArea == rec["result"]["frontalArea"]
+        assert L.rideHeight == rec["result"]["rideHeight"]
+        assert L.movingWalls == rec["result"]["movingWalls"]
+
+
+def test_exact_mode_c1_2000_steps():
+    """96x48x48 BGK sphere, the CPU-runnable workload, 2000 steps bitwise."""
+    fx = load("lbm_c1")
+    L = make(fx, arith="f64")
+    replay_voxelize(L, fx["voxelize"])
+    want = arr(fx["forces"])
+    done = 0
+    for stop in (1, 100, 500, 2000):
+        f = L.step(stop - done)
+        np.testing.assert_allclose(f, want[done:stop], rtol=1e-10, atol=1e-13)
+        done = stop
+        check_hashes(L, fx["snaps"][str(stop)])
+
+
+@pytest.mark.parametrize("name", ["lbm_bgk_sphere", "lbm_trt_les", "lbm_moving"])
+def test_fp32_fields_within_1e4(name):
+    fx = load(name)
+    L = make(fx)
+    replay_voxelize(L, fx["voxelize"])
+    O = oracle_for(fx)
+    fg = L.step(fx["nsteps"])
+    fo = O.step(fx["nsteps"])
+    F = L.fields()
+    for k in FIELDS:
+        e = rel_l2(F[k], O.array(k))
+        assert e < 1e-4, f"{k}: rel-L2 {e:.3e}"
+    u0 = fx["params"]["u0"]
+    cp_g = (F["rho"].astype(np.float64) - 1) / 3 / (0.5 * u0 * u0)
+    cp_o = (O.array("rho").astype(np.float64) - 1) / 3 / (0.5 * u0 * u0)
+    assert rel_l2(cp_g, cp_o) < 1e-4
+    assert rel_l2(fg[:, 0], fo[:, 0]) < 1e-4
+
+
+def test_fp32_c1_force_and_fields():
+    """C1 (laminar, Re 40): 2000 fp32 steps against the oracle."""
+    fx = load("lbm_c1")
+    L = make(fx)
+    replay_voxelize(L, fx["voxelize"])
+    O = oracle_for(fx)
+    fg = L.step(2000)
+    fo = O.step(2000)
+    F = L.fields()
+    for k in FIELDS:
+        e = rel_l2(F[k], O.array(k))
+        assert e < 1e-4, f"{k}: rel-L2 {e:.3e}"
+    A = L.frontalArea
+    q = 0.5 * 0.05 ** 2 * A
+    assert rel_l2(fg[:, 0] / q, fo[:, 0] / q) < 1e-4
+    cd_g, cd_o = fg[-500:, 0].mean() / q, fo[-500:, 0].mean() / q
+    assert abs(cd_g - cd_o) / abs(cd_o) < 1e-4
+
+
+def test_uniform_flow_is_a_fixed_point():
+    """No body: the equilibrium at u0 is an exact fixed point of the step."""
+    L = LBM(40, 20, 18, arith="f64")
+    L.set_params(tau=0.6, les=True, trt=True, u0=0.07)
+    L.reset()
+    f0 = L.f.copy()
+    L.step(25)
+    assert np.array_equal(L.f, f0)
+    assert np.all(L.ux == np.float32(0.07)) and np.all(L.rho == 1)
+    L32 = LBM(40, 20, 18)
+    L32.set_params(tau=0.6, les=True, trt=True, u0=0.07)
+    L32.reset()
+    L32.step(25)
+    assert np.max(np.abs(L32.ux - np.float32(0.07))) < 1e-6
+    assert np.max(np.abs(L32.rho - 1)) < 1e-6
+
+
+def test_fp64_storage_matches_fp64_oracle():
+    fx = load("lbm_trt_les")
+    L = make(fx, storage="f64", arith="f64")
+    replay_voxelize(L, fx["voxelize"])
+    O = oracle_for(fx, prec=8)
+    L.step(50)
+    O.step(50)
+    assert np.array_equal(L.f, O.array("f"))
+    for k in FIELDS:
+        assert np.array_equal(L.fields()[k], O.array(k))
+
+
+def test_pool_slabs_exact():
+    """Worker-pool form: 3 slabs, per-slab forces, fields equal to the fixture."""
+    fx = load("lbm_pool3")
+    L = LBM(*fx["dims"], arith="f64")
+    set_params(L, fx["worker_params"])
+    L.reset()
+    replay_voxelize(L, fx["voxelize"])
+    per = L.step_slabs(fx["nsteps"], fx["nWorkers"])
+    check_hashes(L, fx["pooled"])
+    tot = np.zeros((fx["nsteps"], 3))
+    for w in range(fx["nWorkers"]):
+        tot = tot + per[:, w, :]
+    np.testing.assert_allclose(tot, arr(fx["forces"]), rtol=1e-11, atol=1e-14)
+
+
+def test_fp16_storage_tracks_fp64():
+    """binary16 deviations with float32 collision against the FP64 oracle."""
+    fx = load("lbm_trt_les")
+    L = make(fx, storage="f16")
+    replay_voxelize(L, fx["voxelize"])
+    O = oracle_for(fx, prec=8)
+    L.step(fx["nsteps"])
+    O.step(fx["nsteps"])
+    F = L.fields()
+    u = np.sqrt(F["ux"].astype(np.float64) ** 2 + F["uy"].astype(np.float64) ** 2)
+    uo = np.sqrt(O.array("ux") ** 2 + O.array("uy") ** 2)
+    assert rel_l2(u, uo) < 2e-3
+    assert rel_l2(F["rho"], O.array("rho")) < 1e-4
+
+
+def test_determinism_fp32():
+    fx = load("lbm_trt_les")
+    out = []
+    for _ in range(2):
+        L = make(fx)
+        replay_voxelize(L, fx["voxelize"])
+        f = L.step(40)
+        out.append((f, L.f.copy()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_grid_lookup_on_gpu():
+    """Grid-backed body lookup (K5) against the reference's own mask."""
+    fx = load("vox_meshcube_lookup")
+    vx = fx["vox"]
+    grid = unbits(vx["grid_bits"], vx["nx"] * vx["ny"] * vx["nz"])
+    rec = fx["meshcube"]
+    b = rec["body"]
+    L = LBM(80, 36, 36)
+    L.voxelize_grid(dict(grid=grid, nx=vx["nx"], ny=vx["ny"], nz=vx["nz"], min=vx["min"], inv=vx["inv"],
+                         cell=vx["cell"]), b["M"], b["pos"], b["scale"], rec["box"])
+    assert sha(L.solid) == rec["after"]["solid"]
+    assert sha(L.near) == rec["after"]["near"]
+    assert L.frontalArea == rec["result"]["frontalArea"]
+    assert L.solidCells == rec["result"]["solidCells"]

@@ -1,0 +1,446 @@
+// extern "C" entry points of libwindtunnel (declared in include/windtunnel.h).
+//
+// Host-side bookkeeping that the reference does on its typed arrays (the
+// solid map, wall velocities, body statistics of voxelize) runs here on host
+// mirrors; everything per cell and per step runs in the kernels of lbm.hip
+// and voxel.hip on the lattice's own HIP stream.
+
+#include "wt_internal.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+
+static thread_local char g_err[512] = "";
+
+int wt_fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define WT_CHECK_H(h)                                                   \
+    do {                                                                \
+        if (!(h)) return wt_fail(WT_E_ARG, "null lattice handle");      \
+    } while (0)
+
+namespace {
+
+// Host-side fault injection for failure-handling tests:
+// WT_FAULT=step:N makes the call that would run step N fail with
+// WT_E_FAULT before launching anything.
+int64_t fault_step()
+{
+    static int64_t v = -2;
+    if (v == -2) {
+        v = -1;
+        const char *e = std::getenv("WT_FAULT");
+        if (e && std::strncmp(e, "step:", 5) == 0) v = std::atoll(e + 5);
+    }
+    return v;
+}
+
+int check_device()
+{
+    int dev = 0, count = 0;
+    WT_HIP(hipGetDeviceCount(&count));
+    if (count < 1) return wt_fail(WT_E_DEVICE, "no HIP device");
+    WT_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    WT_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return wt_fail(WT_E_DEVICE, "device %d is %s; this library is built for gfx950 only", dev, prop.gcnArchName);
+    return 0;
+}
+
+int alloc(void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return wt_fail(WT_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return 0;
+}
+
+void release(wt_lattice *h)
+{
+    for (void *p : {(void *)h->d_buf[0], (void *)h->d_buf[1], (void *)h->d_solid, (void *)h->d_sbits,
+                    (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
+                    (void *)h->d_forces})
+        if (p) (void)hipFree(p);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+int upload_body(wt_lattice *h)
+{
+    WT_HIP(hipMemcpyAsync(h->d_solid, h->h_solid.data(), (size_t)h->n, hipMemcpyHostToDevice, h->stream));
+    WT_HIP(hipMemcpyAsync(h->d_uw, h->h_uw.data(), sizeof(float) * 3 * (size_t)h->n, hipMemcpyHostToDevice,
+                          h->stream));
+    return lbm_rebuild_masks(h);
+}
+
+// Worker slab split of js/lbmWorker.js:43-48.
+void slab_range(int nz, int id, int nw, int *bz0, int *bz1)
+{
+    const int lo = 1, hi = nz - 1, span = hi - lo;
+    const int z0 = lo + (int)(((int64_t)span * id) / nw);
+    const int z1 = lo + (int)(((int64_t)span * (id + 1)) / nw);
+    *bz0 = id == 0 ? 0 : z0;
+    *bz1 = id == nw - 1 ? nz : z1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *wt_last_error(void) { return g_err; }
+int wt_abi_version(void) { return WT_ABI_VERSION; }
+
+int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lattice **out)
+{
+    if (!out) return wt_fail(WT_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (nx < 3 || ny < 3 || nz < 3) return wt_fail(WT_E_ARG, "lattice must be at least 3x3x3, got %dx%dx%d", nx, ny, nz);
+    if ((int64_t)nx * ny * nz > ((int64_t)1 << 31) / 19)
+        return wt_fail(WT_E_ARG, "lattice of %lld cells exceeds the 32-bit population index range",
+                       (long long)nx * ny * nz);
+    if (ny > 65535 || nz > 65535) return wt_fail(WT_E_ARG, "ny and nz must be below 65536");
+    const bool ok = (storage == WT_STORE_F32 && (arith == WT_ARITH_F32 || arith == WT_ARITH_F64)) ||
+                    (storage == WT_STORE_F16 && arith == WT_ARITH_F32) ||
+                    (storage == WT_STORE_F64 && arith == WT_ARITH_F64);
+    if (!ok) return wt_fail(WT_E_ARG, "unsupported storage/arith combination %d/%d", storage, arith);
+    if (layout != WT_LAYOUT_AB) return wt_fail(WT_E_ARG, "layout %d not available", layout);
+    if (int rc = check_device()) return rc;
+
+    wt_lattice *h = new wt_lattice();
+    h->nx = nx; h->ny = ny; h->nz = nz; h->n = (int64_t)nx * ny * nz;
+    h->storage = storage; h->arith = arith; h->layout = layout;
+    h->pop_elem = storage == WT_STORE_F16 ? 2 : (storage == WT_STORE_F64 ? 8 : 4);
+    h->mac_elem = storage == WT_STORE_F64 ? 8 : 4;
+    h->nwx = (nx + 31) / 32;
+    const size_t n = (size_t)h->n;
+    const int bx = wt::row_block(nx);
+    const int64_t blocks = (int64_t)((nx + bx - 1) / bx) * ny * nz;
+    int rc = 0;
+    if (!rc) rc = alloc(&h->d_buf[0], 19 * n * h->pop_elem);
+    if (!rc) rc = alloc(&h->d_buf[1], 19 * n * h->pop_elem);
+    if (!rc) rc = alloc((void **)&h->d_solid, n);
+    if (!rc) rc = alloc((void **)&h->d_sbits, sizeof(uint32_t) * h->nwx * (size_t)ny * nz);
+    if (!rc) rc = alloc((void **)&h->d_nbits, sizeof(uint32_t) * h->nwx * (size_t)ny * nz);
+    if (!rc) rc = alloc((void **)&h->d_uw, sizeof(float) * 3 * n);
+    if (!rc) rc = alloc(&h->d_mac, 4 * n * h->mac_elem);
+    h->partial_cap = blocks;
+    if (!rc) rc = alloc((void **)&h->d_partial, sizeof(double) * 3 * (size_t)blocks);
+    h->forces_cap = 4096;
+    if (!rc) rc = alloc((void **)&h->d_forces, sizeof(double) * 3 * (size_t)h->forces_cap);
+    if (!rc && hipStreamCreate(&h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "hipStreamCreate failed");
+    if (!rc && (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess))
+        rc = wt_fail(WT_E_DEVICE, "hipEventCreate failed");
+    if (rc) { release(h); delete h; return rc; }
+    h->h_solid.assign(n, 0);
+    h->h_uw.assign(3 * n, 0.0f);
+    rc = upload_body(h);
+    if (!rc) rc = lbm_reset(h);
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "initial reset failed");
+    if (rc) { release(h); delete h; return rc; }
+    *out = h;
+    return 0;
+}
+
+int wt_destroy(wt_lattice *h)
+{
+    if (!h) return 0;
+    (void)hipStreamSynchronize(h->stream);
+    release(h);
+    delete h;
+    return 0;
+}
+
+int wt_set_params(wt_lattice *h, const wt_params *p)
+{
+    WT_CHECK_H(h);
+    if (!p) return wt_fail(WT_E_ARG, "null params");
+    h->p = *p;
+    return 0;
+}
+
+int wt_get_params(const wt_lattice *h, wt_params *p)
+{
+    WT_CHECK_H(h);
+    if (!p) return wt_fail(WT_E_ARG, "null params");
+    *p = h->p;
+    return 0;
+}
+
+int wt_reset(wt_lattice *h)
+{
+    WT_CHECK_H(h);
+    if (int rc = lbm_reset(h)) return rc;
+    WT_HIP(hipStreamSynchronize(h->stream));
+    h->steps = 0;
+    h->force[0] = h->force[1] = h->force[2] = 0;
+    return 0;
+}
+
+int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const float *wv, int ground,
+                double roadVel, wt_body_stats *out)
+{
+    WT_CHECK_H(h);
+    if (!box) return wt_fail(WT_E_ARG, "null box");
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    const int x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
+    const bool empty = x1 < x0 || y1 < y0 || z1 < z0;
+    if (!empty && (x0 < 1 || y0 < 1 || z0 < 1 || x1 > nx - 2 || y1 > ny - 2 || z1 > nz - 2))
+        return wt_fail(WT_E_ARG, "box [%d,%d,%d]-[%d,%d,%d] is not within the interior", x0, y0, z0, x1, y1, z1);
+    if (!empty && !mask) return wt_fail(WT_E_ARG, "null mask for a non-empty box");
+    const int64_t n = h->n;
+    std::vector<uint8_t> ops((size_t)n, 0);
+    uint8_t *solid = h->h_solid.data();
+    float *uwx = h->h_uw.data(), *uwy = uwx + n, *uwz = uwx + 2 * n;
+    // js/lbm.js:194-202: the old body becomes fluid at u0
+    h->ground = !!ground;
+    int mov = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (solid[i] == wt::SOLID_OBJ) {
+            solid[i] = wt::FLUID; ops[i] |= 1;
+            uwx[i] = uwy[i] = uwz[i] = 0;
+        }
+    }
+    // js/lbm.js:204-214: the road row
+    const int gy = ground ? wt::SOLID_WALL : wt::FLUID;
+    const double rv = (roadVel != roadVel || roadVel == 0) ? 0 : roadVel;
+    if (rv != 0) mov = 1;
+    for (int z = 0; z < nz; z++) {
+        for (int x = 0; x < nx; x++) {
+            const int64_t i = x + (int64_t)nx * ny * z;
+            if (solid[i] != gy) { solid[i] = (uint8_t)gy; if (gy == wt::FLUID) ops[i] |= 1; }
+            uwx[i] = gy == wt::SOLID_WALL ? (float)rv : 0.0f;
+            uwy[i] = uwz[i] = 0;
+        }
+    }
+    // js/lbm.js:216-246: the new body, its projections and lowest cells
+    std::vector<uint8_t> col((size_t)ny * nz, 0), plan((size_t)nx * nz, 0);
+    std::vector<int16_t> low((size_t)nx * nz, 32767);
+    int64_t cells = 0, k = 0, kw = 0;
+    int zlo = nz, zhi = -1;
+    if (!empty) {
+        for (int z = z0; z <= z1; z++)
+            for (int y = y0; y <= y1; y++) {
+                int64_t i = x0 + (int64_t)nx * (y + (int64_t)ny * z);
+                for (int x = x0; x <= x1; x++, i++, k++) {
+                    if (!mask[k]) continue;
+                    solid[i] = wt::SOLID_OBJ;
+                    ops[i] |= 2;
+                    uwx[i] = uwy[i] = uwz[i] = 0;
+                    if (mask[k] == 2) {
+                        if (!wv) return wt_fail(WT_E_ARG, "mask holds wall-velocity cells but wv is null");
+                        uwx[i] = wv[3 * kw]; uwy[i] = wv[3 * kw + 1]; uwz[i] = wv[3 * kw + 2];
+                        kw++;
+                        mov = 1;
+                    }
+                    col[y + (size_t)ny * z] = 1;
+                    const size_t pk = x + (size_t)nx * z;
+                    plan[pk] = 1;
+                    if (y < low[pk]) low[pk] = (int16_t)y;
+                    cells++;
+                    zlo = std::min(zlo, z); zhi = std::max(zhi, z);
+                }
+            }
+    }
+    int64_t a = 0, p = 0;
+    for (uint8_t c : col) a += c;
+    for (uint8_t c : plan) p += c;
+    h->body.frontalArea = std::max<int64_t>(1, a);
+    h->body.planformArea = std::max<int64_t>(1, p);
+    h->body.solidCells = cells;
+    std::vector<int> gaps;
+    for (size_t c = 0; c < plan.size(); c++)
+        if (plan[c] && low[c] < 32767) { const int g = low[c] - 1; if (g > 0) gaps.push_back(g); }
+    std::sort(gaps.begin(), gaps.end());
+    h->body.rideHeight = gaps.empty() ? 0 : gaps[gaps.size() >> 1];
+    h->p.movingWalls = mov;
+    // z rows whose fluid cells can carry body links
+    if (zhi >= 0) { h->fz0 = std::max(1, zlo - 1); h->fz1 = std::min(nz - 2, zhi + 1); }
+    else { h->fz0 = h->fz1 = -1; }
+
+    uint8_t *d_ops = nullptr;
+    if (int rc = alloc((void **)&d_ops, (size_t)n)) return rc;
+    int rc = 0;
+    if (hipMemcpyAsync(d_ops, ops.data(), (size_t)n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = wt_fail(WT_E_DEVICE, "voxelize upload failed");
+    if (!rc) rc = lbm_apply_voxelize(h, d_ops);
+    if (!rc) rc = upload_body(h);
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "voxelize failed");
+    (void)hipFree(d_ops);
+    if (rc) return rc;
+    if (out) {
+        out->frontalArea = h->body.frontalArea; out->planformArea = h->body.planformArea;
+        out->solidCells = h->body.solidCells; out->rideHeight = h->body.rideHeight;
+        out->movingWalls = mov; out->ground = h->ground;
+    }
+    return 0;
+}
+
+int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, const double M[9],
+                     const double pos[3], double scale, const int box[6], int ground, double roadVel,
+                     wt_body_stats *out)
+{
+    WT_CHECK_H(h);
+    if (!grid || !vm || !M || !pos || !box) return wt_fail(WT_E_ARG, "null argument");
+    if (vm->nx < 1 || vm->ny < 1 || vm->nz < 1) return wt_fail(WT_E_ARG, "empty voxel grid");
+    const int64_t cnt = std::max(0, box[3] - box[0] + 1) * (int64_t)std::max(0, box[4] - box[1] + 1) *
+                        std::max(0, box[5] - box[2] + 1);
+    std::vector<uint8_t> mask((size_t)cnt);
+    if (cnt > 0) {
+        const size_t gbytes = (size_t)vm->nx * vm->ny * vm->nz;
+        uint8_t *d_grid = nullptr, *d_mask = nullptr;
+        if (int rc = alloc((void **)&d_grid, gbytes)) return rc;
+        if (int rc = alloc((void **)&d_mask, (size_t)cnt)) { (void)hipFree(d_grid); return rc; }
+        int rc = 0;
+        if (hipMemcpyAsync(d_grid, grid, gbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+            rc = wt_fail(WT_E_DEVICE, "grid upload failed");
+        if (!rc) rc = lbm_grid_mask(h, d_grid, vm, M, pos, scale, box, d_mask);
+        if (!rc && hipMemcpyAsync(mask.data(), d_mask, (size_t)cnt, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+            rc = wt_fail(WT_E_DEVICE, "mask download failed");
+        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "grid lookup failed");
+        (void)hipFree(d_grid); (void)hipFree(d_mask);
+        if (rc) return rc;
+    }
+    return wt_voxelize(h, box, mask.data(), nullptr, ground, roadVel, out);
+}
+
+static int run_steps(wt_lattice *h, int nsteps, int nslabs, double *forces)
+{
+    WT_CHECK_H(h);
+    if (nsteps < 0) return wt_fail(WT_E_ARG, "negative step count");
+    if (nslabs < 1 || nslabs > h->nz - 2) return wt_fail(WT_E_ARG, "slab count %d out of range", nslabs);
+    const int64_t fs = fault_step();
+    if (fs >= 0 && h->steps <= fs && fs < h->steps + nsteps)
+        return wt_fail(WT_E_FAULT, "injected fault at step %lld", (long long)fs);
+    const int per = 3 * nslabs;
+    const int chunk = std::max(1, h->forces_cap / nslabs);
+    for (int s0 = 0; s0 < nsteps; s0 += chunk) {
+        const int cnt = std::min(chunk, nsteps - s0);
+        for (int s = 0; s < cnt; s++) {
+            const bool macro = s0 + s == nsteps - 1;
+            for (int w = 0; w < nslabs; w++) {
+                int bz0 = 0, bz1 = h->nz;
+                if (nslabs > 1) slab_range(h->nz, w, nslabs, &bz0, &bz1);
+                if (int rc = lbm_launch_step(h, bz0, bz1, macro, h->d_forces + (int64_t)s * per + 3 * w)) return rc;
+            }
+            h->parity ^= 1;
+            h->steps++;
+        }
+        std::vector<double> tmp((size_t)cnt * per);
+        WT_HIP(hipMemcpyAsync(tmp.data(), h->d_forces, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, h->stream));
+        WT_HIP(hipStreamSynchronize(h->stream));
+        if (forces) std::memcpy(forces + (int64_t)s0 * per, tmp.data(), sizeof(double) * tmp.size());
+        for (int k = 0; k < 3; k++) {
+            double t = 0;
+            for (int w = 0; w < nslabs; w++) t += tmp[(size_t)(cnt - 1) * per + 3 * w + k];
+            h->force[k] = t;
+        }
+    }
+    return 0;
+}
+
+int wt_step(wt_lattice *h, int nsteps, double *forces) { return run_steps(h, nsteps, 1, forces); }
+
+int wt_step_slabs(wt_lattice *h, int nsteps, int nslabs, double *forces)
+{
+    return run_steps(h, nsteps, nslabs, forces);
+}
+
+int wt_set_parity(wt_lattice *h, int parity)
+{
+    WT_CHECK_H(h);
+    if (parity != 0 && parity != 1) return wt_fail(WT_E_ARG, "parity must be 0 or 1");
+    h->parity = parity;
+    return 0;
+}
+
+int wt_get_state(const wt_lattice *h, wt_state_t *out)
+{
+    WT_CHECK_H(h);
+    if (!out) return wt_fail(WT_E_ARG, "null output");
+    out->parity = h->parity; out->steps = h->steps;
+    for (int k = 0; k < 3; k++) out->force[k] = h->force[k];
+    return 0;
+}
+
+int wt_get_fields(wt_lattice *h, void *rho, void *ux, void *uy, void *uz)
+{
+    WT_CHECK_H(h);
+    void *dst[4] = {rho, ux, uy, uz};
+    const size_t bytes = (size_t)h->n * h->mac_elem;
+    for (int k = 0; k < 4; k++)
+        if (dst[k])
+            WT_HIP(hipMemcpyAsync(dst[k], (char *)h->d_mac + k * bytes, bytes, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int wt_get_pops(wt_lattice *h, int which, void *out)
+{
+    WT_CHECK_H(h);
+    if (!out || (which != 0 && which != 1)) return wt_fail(WT_E_ARG, "bad population request");
+    const size_t cnt = 19 * (size_t)h->n;
+    const void *src = h->d_buf[h->parity ^ which];
+    if (h->storage != WT_STORE_F16) {
+        WT_HIP(hipMemcpyAsync(out, src, cnt * h->pop_elem, hipMemcpyDeviceToHost, h->stream));
+        WT_HIP(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    std::vector<__half> tmp(cnt);
+    WT_HIP(hipMemcpyAsync(tmp.data(), src, cnt * 2, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipStreamSynchronize(h->stream));
+    float *o = (float *)out;
+    for (size_t k = 0; k < cnt; k++) o[k] = __half2float(tmp[k]) * wt::F16_INV_SCALE;
+    return 0;
+}
+
+int wt_get_solid(wt_lattice *h, uint8_t *solid, uint8_t *near)
+{
+    WT_CHECK_H(h);
+    if (solid) std::memcpy(solid, h->h_solid.data(), (size_t)h->n);
+    if (near) {
+        const size_t words = (size_t)h->nwx * h->ny * h->nz;
+        std::vector<uint32_t> nb(words);
+        WT_HIP(hipMemcpyAsync(nb.data(), h->d_nbits, words * 4, hipMemcpyDeviceToHost, h->stream));
+        WT_HIP(hipStreamSynchronize(h->stream));
+        for (int64_t i = 0; i < h->n; i++) {
+            const int64_t x = i % h->nx, row = i / h->nx;
+            near[i] = (nb[(x >> 5) + h->nwx * row] >> (x & 31)) & 1u;
+        }
+    }
+    return 0;
+}
+
+int wt_time_steps(wt_lattice *h, int nsteps, double *ms)
+{
+    WT_CHECK_H(h);
+    if (nsteps < 1 || !ms) return wt_fail(WT_E_ARG, "bad timing request");
+    WT_HIP(hipEventRecord(h->ev0, h->stream));
+    for (int s = 0; s < nsteps; s++) {
+        if (int rc = lbm_launch_step(h, 0, h->nz, s == nsteps - 1, h->d_forces + 3 * (s % h->forces_cap))) return rc;
+        h->parity ^= 1;
+        h->steps++;
+    }
+    WT_HIP(hipEventRecord(h->ev1, h->stream));
+    WT_HIP(hipEventSynchronize(h->ev1));
+    float t = 0;
+    WT_HIP(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    *ms = t;
+    const int last = (nsteps - 1) % h->forces_cap;
+    WT_HIP(hipMemcpy(h->force, h->d_forces + 3 * last, sizeof(double) * 3, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"

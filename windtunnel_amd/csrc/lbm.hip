@@ -1,0 +1,537 @@
+// D3Q19 lattice kernels for gfx950: fused stream/collide step, reset,
+// voxelize updates, mask builds, grid-backed body lookup.
+//
+// The step restates js/lbm.js:303-512 as ONE kernel per time step: every
+// thread owns one cell of the full domain (x fastest, one block per lattice
+// row), pulls its 19 populations, applies halfway bounce-back, the
+// moving-wall correction and momentum exchange on near-wall cells, collides
+// (BGK/TRT, optional Smagorinsky LES) and writes its 19 post-collision
+// populations. Domain-boundary cells are handled by the same launch: the
+// equilibrium planes write their equilibrium, and the x = nx-2 threads also
+// write the zero-gradient outlet copy into x = nx-1 (js/lbm.js:504-511), so
+// no second pass and no barrier are needed.
+//
+// Arithmetic type T is float (fast path) or double. With T = double and
+// float storage the expression tree is the reference's, evaluated with
+// contraction off, so the stored values are the reference's own.
+
+#include "wt_internal.h"
+
+#include <cmath>
+
+namespace wt {
+namespace {
+
+// ---------------------------------------------------------------- storage
+
+template <class S> struct Pop;
+template <> struct Pop<float> {
+    template <class T> __device__ static T ld(const float *p, int64_t k) { return (T)p[k]; }
+    template <class T> __device__ static void st(float *p, int64_t k, T v) { p[k] = (float)v; }
+};
+template <> struct Pop<double> {
+    template <class T> __device__ static T ld(const double *p, int64_t k) { return (T)p[k]; }
+    template <class T> __device__ static void st(double *p, int64_t k, T v) { p[k] = (double)v; }
+};
+template <> struct Pop<__half> {
+    template <class T> __device__ static T ld(const __half *p, int64_t k)
+    {
+        return (T)(__half2float(p[k]) * F16_INV_SCALE);
+    }
+    template <class T> __device__ static void st(__half *p, int64_t k, T v)
+    {
+        p[k] = __float2half_rn((float)v * F16_SCALE);
+    }
+};
+
+template <class S, class MAC> struct StepArgs {
+    const S *F;
+    S *G;
+    const uint8_t *solid;
+    const uint32_t *sbits, *nbits;
+    const float *uwx, *uwy, *uwz;
+    MAC *rho, *ux, *uy, *uz;
+    double *partial;
+    int64_t n, bmin, bmax;
+    int nx, ny, nz, nwx, z0, mov;
+    double tau, sq18cs2, lambda, u0;
+    double eqv[Q];  // boundary equilibrium deviation at (rho=1, u0, 0, 0)
+};
+
+__device__ __forceinline__ bool bit(const uint32_t *bits, int nwx, int ny, int x, int y, int z)
+{
+    return (bits[(x >> 5) + (int64_t)nwx * (y + (int64_t)ny * z)] >> (x & 31)) & 1u;
+}
+
+// Moments, deviation equilibrium, LES and TRT/BGK relaxation of one cell
+// (js/lbm.js:377-460) in the reference's expression order.
+template <class T, bool LES, bool TRT>
+__device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq18cs2, T lam, T (&g)[Q],
+                                        T &r, T &vx, T &vy, T &vz)
+{
+    const T f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4], f5 = f[5], f6 = f[6],
+            f7 = f[7], f8 = f[8], f9 = f[9], f10 = f[10], f11 = f[11], f12 = f[12], f13 = f[13],
+            f14 = f[14], f15 = f[15], f16 = f[16], f17 = f[17], f18 = f[18];
+    const T dr = f0 + f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9 + f10 + f11 + f12 + f13 + f14 +
+                 f15 + f16 + f17 + f18;
+    r = T(1) + dr;
+    const T inv = T(1) / r;
+    vx = (f1 - f2 + f7 - f8 + f9 - f10 + f11 - f12 + f13 - f14) * inv;
+    vy = (f3 - f4 + f7 - f8 - f9 + f10 + f15 - f16 + f17 - f18) * inv;
+    vz = (f5 - f6 + f11 - f12 - f13 + f14 + f15 - f16 - f17 + f18) * inv;
+
+    const T W0 = T(1.0 / 3), WF = T(1.0 / 18), WE = T(1.0 / 36);
+    const T A = dr - r * T(1.5) * (vx * vx + vy * vy + vz * vz);
+    const T r3 = T(3) * r, r45 = T(4.5) * r;
+    const T bxy = vx + vy, bxY = vx - vy, bxz = vx + vz, bxZ = vx - vz, byz = vy + vz, byZ = vy - vz;
+    const T e0 = W0 * A;
+    const T hx = r45 * vx * vx, hy = r45 * vy * vy, hz = r45 * vz * vz;
+    const T e1 = WF * (A + r3 * vx + hx), e2 = WF * (A - r3 * vx + hx);
+    const T e3 = WF * (A + r3 * vy + hy), e4 = WF * (A - r3 * vy + hy);
+    const T e5 = WF * (A + r3 * vz + hz), e6 = WF * (A - r3 * vz + hz);
+    const T h7 = r45 * bxy * bxy, h9 = r45 * bxY * bxY, h11 = r45 * bxz * bxz;
+    const T h13 = r45 * bxZ * bxZ, h15 = r45 * byz * byz, h17 = r45 * byZ * byZ;
+    const T e7 = WE * (A + r3 * bxy + h7), e8 = WE * (A - r3 * bxy + h7);
+    const T e9 = WE * (A + r3 * bxY + h9), e10 = WE * (A - r3 * bxY + h9);
+    const T e11 = WE * (A + r3 * bxz + h11), e12 = WE * (A - r3 * bxz + h11);
+    const T e13 = WE * (A + r3 * bxZ + h13), e14 = WE * (A - r3 * bxZ + h13);
+    const T e15 = WE * (A + r3 * byz + h15), e16 = WE * (A - r3 * byz + h15);
+    const T e17 = WE * (A + r3 * byZ + h17), e18 = WE * (A - r3 * byZ + h17);
+    const T n1 = f1 - e1, n2 = f2 - e2, n3 = f3 - e3, n4 = f4 - e4, n5 = f5 - e5, n6 = f6 - e6;
+    const T n7 = f7 - e7, n8 = f8 - e8, n9 = f9 - e9, n10 = f10 - e10, n11 = f11 - e11;
+    const T n12 = f12 - e12, n13 = f13 - e13, n14 = f14 - e14, n15 = f15 - e15, n16 = f16 - e16;
+    const T n17 = f17 - e17, n18 = f18 - e18;
+
+    T om = omega0, tauEff = tau0;
+    if (LES) {
+        const T pxx = n1 + n2 + n7 + n8 + n9 + n10 + n11 + n12 + n13 + n14;
+        const T pyy = n3 + n4 + n7 + n8 + n9 + n10 + n15 + n16 + n17 + n18;
+        const T pzz = n5 + n6 + n11 + n12 + n13 + n14 + n15 + n16 + n17 + n18;
+        const T pxy = n7 + n8 - n9 - n10;
+        const T pxz = n11 + n12 - n13 - n14;
+        const T pyz = n15 + n16 - n17 - n18;
+        const T mag = sqrt(T(2) * (pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz)));
+        tauEff = T(0.5) * (tau0 + sqrt(tau0 * tau0 + sq18cs2 * mag * inv));
+        om = T(1) / tauEff;
+    }
+    const T omm = TRT ? T(1) / (T(0.5) + lam / (tauEff - T(0.5))) : om;
+
+    g[0] = f0 - om * (f0 - e0);
+    T sp, sm;
+#define WT_PAIR(a, b)                                \
+    sp = T(0.5) * (n##a + n##b);                     \
+    sm = T(0.5) * (n##a - n##b);                     \
+    g[a] = f##a - om * sp - omm * sm;                \
+    g[b] = f##b - om * sp + omm * sm;
+    WT_PAIR(1, 2) WT_PAIR(3, 4) WT_PAIR(5, 6) WT_PAIR(7, 8) WT_PAIR(9, 10)
+    WT_PAIR(11, 12) WT_PAIR(13, 14) WT_PAIR(15, 16) WT_PAIR(17, 18)
+#undef WT_PAIR
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT>
+__global__ void __launch_bounds__(1024) k_step_ab(StepArgs<S, MAC> a)
+{
+    using P = Pop<S>;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z + a.z0;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t n = a.n;
+    double fx = 0, fy = 0, fz = 0;
+
+    if (x < nx - 1) {
+        const int64_t i = x + (int64_t)nx * (y + (int64_t)ny * z);
+        const bool out = x == nx - 2;  // also writes the outlet copy at x = nx-1
+        const bool bnd = x == 0 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1;
+        const bool sol = bit(a.sbits, a.nwx, ny, x, y, z);
+        if (bnd || sol) {
+            // Equilibrium planes (js/lbm.js:488-503): all of them write except
+            // the floor where it is solid and no other plane covers the cell.
+            const bool eq = bnd && !(sol && y == 0 && x != 0 && z != 0 && z != nz - 1 && y != ny - 1);
+            if (eq) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    P::template st<double>(a.G, q * n + i, a.eqv[q]);
+                    if (out) P::template st<double>(a.G, q * n + i + 1, a.eqv[q]);
+                }
+                if (MACRO) {
+                    a.rho[i] = (MAC)1; a.ux[i] = (MAC)a.u0; a.uy[i] = 0; a.uz[i] = 0;
+                    if (out) { a.rho[i + 1] = (MAC)1; a.ux[i + 1] = (MAC)a.u0; a.uy[i + 1] = 0; a.uz[i + 1] = 0; }
+                }
+            } else if (out) {
+                // the outlet copies whatever a solid x = nx-2 cell holds
+#pragma unroll
+                for (int q = 0; q < Q; q++) a.G[q * n + i + 1] = a.G[q * n + i];
+                if (MACRO) {
+                    a.rho[i + 1] = a.rho[i]; a.ux[i + 1] = a.ux[i];
+                    a.uy[i + 1] = a.uy[i]; a.uz[i + 1] = a.uz[i];
+                }
+            }
+        } else {
+            T f[Q];
+            const bool nr = bit(a.nbits, a.nwx, ny, x, y, z);
+            if (nr) {
+                // halfway bounce-back, moving wall, momentum exchange (js/lbm.js:334-368)
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const int64_t off = EX[q] + (int64_t)nx * (EY[q] + (int64_t)ny * EZ[q]);
+                    const int64_t j = i - off;
+                    const int sj = a.solid[j];
+                    if (sj) {
+                        const T fd = P::template ld<T>(a.F, OPP[q] * n + i);
+                        T corr = 0;
+                        if (a.mov) {
+                            const T cu = (T)EX[q] * (T)a.uwx[j] + (T)EY[q] * (T)a.uwy[j] + (T)EZ[q] * (T)a.uwz[j];
+                            if (cu != T(0)) {
+                                const T wq = q == 0 ? T(1.0 / 3) : (q <= 6 ? T(1.0 / 18) : T(1.0 / 36));
+                                corr = T(6) * wq * (T)a.rho[i] * cu;
+                            }
+                        }
+                        f[q] = fd + corr;
+                        if (sj == SOLID_OBJ) {
+                            const double d = (double)(T(2) * fd + corr);
+                            fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
+                        }
+                    } else {
+                        f[q] = P::template ld<T>(a.F, q * n + j);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const int64_t off = EX[q] + (int64_t)nx * (EY[q] + (int64_t)ny * EZ[q]);
+                    f[q] = P::template ld<T>(a.F, q * n + i - off);
+                }
+            }
+            T g[Q], r, vx, vy, vz;
+            collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                P::template st<T>(a.G, q * n + i, g[q]);
+                if (out) P::template st<T>(a.G, q * n + i + 1, g[q]);
+            }
+            if (MACRO) {
+                a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
+                if (out) { a.rho[i + 1] = (MAC)r; a.ux[i + 1] = (MAC)vx; a.uy[i + 1] = (MAC)vy; a.uz[i + 1] = (MAC)vz; }
+            } else if (nr) {
+                a.rho[i] = (MAC)r;  // next step's moving-wall correction reads it
+            }
+        }
+    }
+
+    // Per-block FP64 force partial for blocks that can hold body links:
+    // wave shuffles, then the waves' sums in wave order.
+    const int64_t bid = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+    if (bid < a.bmin || bid > a.bmax) return;
+    __shared__ double red[16][3];
+    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { red[wv][0] = fx; red[wv][1] = fy; red[wv][2] = fz; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sx = 0, sy = 0, sz = 0;
+        const int nw = blockDim.x >> 6;
+        for (int w = 0; w < nw; w++) { sx += red[w][0]; sy += red[w][1]; sz += red[w][2]; }
+        double *o = a.partial + (bid - a.bmin) * 3;
+        o[0] = sx; o[1] = sy; o[2] = sz;
+    }
+}
+
+// Fixed-order sum of the block partials into one step's force.
+__global__ void __launch_bounds__(256) k_force_sum(const double *partial, int64_t count, double *out)
+{
+    double s[3] = {0, 0, 0};
+    for (int64_t b = threadIdx.x; b < count; b += 256) {
+        s[0] += partial[3 * b]; s[1] += partial[3 * b + 1]; s[2] += partial[3 * b + 2];
+    }
+    __shared__ double red[4][3];
+    for (int k = 0; k < 3; k++) s[k] = wave_sum(s[k]);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) for (int k = 0; k < 3; k++) red[wv][k] = s[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) out[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+}
+
+// ------------------------------------------------------- reset / voxelize
+
+template <class S, class MAC> struct InitArgs {
+    S *f, *g;
+    MAC *rho, *ux, *uy, *uz;
+    const uint8_t *solid, *ops;
+    int64_t n;
+    double u0;
+    double eq_u0[Q], eq_rest[Q];
+};
+
+// reset(): equilibrium at u0 (fluid) or at rest (solid) in both buffers
+// (js/lbm.js:164-174).
+template <class S, class MAC>
+__global__ void __launch_bounds__(256) k_reset(InitArgs<S, MAC> a)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const bool s = a.solid[i] != 0;
+    for (int q = 0; q < Q; q++) {
+        const double v = s ? a.eq_rest[q] : a.eq_u0[q];
+        Pop<S>::template st<double>(a.f, q * a.n + i, v);
+        Pop<S>::template st<double>(a.g, q * a.n + i, v);
+    }
+    a.rho[i] = 1; a.ux[i] = s ? (MAC)0 : (MAC)a.u0; a.uy[i] = 0; a.uz[i] = 0;
+}
+
+// voxelize updates (js/lbm.js:197-233): bit 0 = _seedCell (u0 equilibrium in
+// f and g), bit 1 = body cell (rest equilibrium in f, fields at rest);
+// applied in that order.
+template <class S, class MAC>
+__global__ void __launch_bounds__(256) k_apply_ops(InitArgs<S, MAC> a)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int op = a.ops[i];
+    if (!op) return;
+    if (op & 1) {
+        for (int q = 0; q < Q; q++) {
+            Pop<S>::template st<double>(a.f, q * a.n + i, a.eq_u0[q]);
+            Pop<S>::template st<double>(a.g, q * a.n + i, a.eq_u0[q]);
+        }
+        a.rho[i] = 1; a.ux[i] = (MAC)a.u0; a.uy[i] = 0; a.uz[i] = 0;
+    }
+    if (op & 2) {
+        for (int q = 0; q < Q; q++) Pop<S>::template st<double>(a.f, q * a.n + i, a.eq_rest[q]);
+        a.rho[i] = 1; a.ux[i] = 0; a.uy[i] = 0; a.uz[i] = 0;
+    }
+}
+
+// Bit-packed solid mask and near-wall mask (js/lbm.js:268-282): near = an
+// interior fluid cell with a solid cell among its 18 upstream neighbours.
+__global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, uint32_t *sbits, uint32_t *nbits,
+                                               int nx, int ny, int nz, int nwx)
+{
+    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nwords = (int64_t)nwx * ny * nz;
+    if (w >= nwords) return;
+    const int wx = (int)(w % nwx);
+    const int64_t row = w / nwx;
+    const int y = (int)(row % ny), z = (int)(row / ny);
+    uint32_t sb = 0, nb = 0;
+    for (int b = 0; b < 32; b++) {
+        const int x = wx * 32 + b;
+        if (x >= nx) break;
+        const int64_t i = x + (int64_t)nx * (y + (int64_t)ny * z);
+        if (solid[i]) { sb |= 1u << b; continue; }
+        if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2 || z < 1 || z > nz - 2) continue;
+        for (int q = 1; q < Q; q++) {
+            const int64_t off = EX[q] + (int64_t)nx * (EY[q] + (int64_t)ny * EZ[q]);
+            if (solid[i - off]) { nb |= 1u << b; break; }
+        }
+    }
+    sbits[w] = sb;
+    nbits[w] = nb;
+}
+
+// ECMAScript ToInt32 of a double (the `|0` operator).
+__device__ __forceinline__ int js_toint32(double d)
+{
+    if (!(d == d) || isinf(d)) return 0;
+    const double t = trunc(d);
+    if (t > -2147483649.0 && t < 2147483648.0) return (int)t;
+    double m = fmod(t, 4294967296.0);
+    if (m < 0) m += 4294967296.0;
+    return (int)(uint32_t)m;
+}
+
+struct GridArgs {
+    const uint8_t *grid;
+    uint8_t *mask;
+    int gnx, gny, gnz;
+    double gmin[3], ginv, M[9], pos[3], inv;
+    int x0, y0, z0, bx, by, bz;
+};
+
+// Grid-backed Body.makeTest at every cell centre of the box
+// (js/shapes.js:376-388): world -> body by M^T / scale, truncating lookup.
+__global__ void __launch_bounds__(256) k_grid_mask(GridArgs a)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t cnt = (int64_t)a.bx * a.by * a.bz;
+    if (k >= cnt) return;
+    const int x = a.x0 + (int)(k % a.bx);
+    const int y = a.y0 + (int)((k / a.bx) % a.by);
+    const int z = a.z0 + (int)(k / ((int64_t)a.bx * a.by));
+    const double X = x + 0.5, Y = y + 0.5, Z = z + 0.5;
+    const double dx = X - a.pos[0], dy = Y - a.pos[1], dz = Z - a.pos[2];
+    const double lx = (a.M[0] * dx + a.M[3] * dy + a.M[6] * dz) * a.inv;
+    const double ly = (a.M[1] * dx + a.M[4] * dy + a.M[7] * dz) * a.inv;
+    const double lz = (a.M[2] * dx + a.M[5] * dy + a.M[8] * dz) * a.inv;
+    uint8_t m = 0;
+    const int ia = js_toint32((lx - a.gmin[0]) * a.ginv);
+    if (ia >= 0 && ia < a.gnx) {
+        const int ib = js_toint32((ly - a.gmin[1]) * a.ginv);
+        if (ib >= 0 && ib < a.gny) {
+            const int ic = js_toint32((lz - a.gmin[2]) * a.ginv);
+            if (ic >= 0 && ic < a.gnz) m = a.grid[ia + (int64_t)a.gnx * (ib + (int64_t)a.gny * ic)] != 0;
+        }
+    }
+    a.mask[k] = m;
+}
+
+template <class S, class MAC> InitArgs<S, MAC> init_args(wt_lattice *h, const uint8_t *ops)
+{
+    InitArgs<S, MAC> a{};
+    a.f = (S *)h->d_buf[h->parity];
+    a.g = (S *)h->d_buf[h->parity ^ 1];
+    MAC *m = (MAC *)h->d_mac;
+    a.rho = m; a.ux = m + h->n; a.uy = m + 2 * h->n; a.uz = m + 3 * h->n;
+    a.solid = h->d_solid; a.ops = ops; a.n = h->n; a.u0 = h->p.u0;
+    eq_deviation(1, h->p.u0, 0, 0, a.eq_u0);
+    eq_deviation(1, 0, 0, 0, a.eq_rest);
+    return a;
+}
+
+template <class S, class MAC> int reset_t(wt_lattice *h)
+{
+    auto a = init_args<S, MAC>(h, nullptr);
+    const int64_t nb = (h->n + 255) / 256;
+    hipLaunchKernelGGL((k_reset<S, MAC>), dim3((unsigned)nb), dim3(256), 0, h->stream, a);
+    WT_HIP(hipGetLastError());
+    return 0;
+}
+
+template <class S, class MAC> int ops_t(wt_lattice *h, const uint8_t *d_ops)
+{
+    auto a = init_args<S, MAC>(h, d_ops);
+    const int64_t nb = (h->n + 255) / 256;
+    hipLaunchKernelGGL((k_apply_ops<S, MAC>), dim3((unsigned)nb), dim3(256), 0, h->stream, a);
+    WT_HIP(hipGetLastError());
+    return 0;
+}
+
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT>
+int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
+{
+    StepArgs<S, MAC> a{};
+    a.F = (const S *)h->d_buf[h->parity];
+    a.G = (S *)h->d_buf[h->parity ^ 1];
+    a.solid = h->d_solid; a.sbits = h->d_sbits; a.nbits = h->d_nbits;
+    a.uwx = h->d_uw; a.uwy = h->d_uw + h->n; a.uwz = h->d_uw + 2 * h->n;
+    MAC *m = (MAC *)h->d_mac;
+    a.rho = m; a.ux = m + h->n; a.uy = m + 2 * h->n; a.uz = m + 3 * h->n;
+    a.partial = h->d_partial;
+    a.n = h->n; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx; a.z0 = bz0;
+    a.mov = h->p.movingWalls;
+    a.tau = h->p.tau; a.sq18cs2 = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
+    a.u0 = h->p.u0;
+    eq_deviation(1, h->p.u0, 0, 0, a.eqv);
+    const int bx = row_block(h->nx);
+    const int gx = (h->nx + bx - 1) / bx;
+    // blocks of the z rows that can hold body links write partials
+    const int lo = h->fz0 > bz0 ? h->fz0 : bz0, hi = h->fz1 < bz1 - 1 ? h->fz1 : bz1 - 1;
+    int64_t count = 0;
+    if (h->fz0 >= 0 && lo <= hi) {
+        a.bmin = (int64_t)(lo - bz0) * h->ny * gx;
+        a.bmax = (int64_t)(hi - bz0 + 1) * h->ny * gx - 1;
+        count = a.bmax - a.bmin + 1;
+    } else {
+        a.bmin = 1; a.bmax = 0;
+    }
+    hipLaunchKernelGGL((k_step_ab<S, T, MAC, MACRO, LES, TRT>), dim3(gx, h->ny, bz1 - bz0), dim3(bx), 0,
+                       h->stream, a);
+    WT_HIP(hipGetLastError());
+    if (d_force) {
+        hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(256), 0, h->stream, h->d_partial, count, d_force);
+        WT_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+template <class S, class T, class MAC> int step_dispatch(wt_lattice *h, int bz0, int bz1, bool macro, double *f)
+{
+    const bool les = h->p.les, trt = h->p.trt;
+#define WT_CASE(M, L, R) \
+    if (macro == M && les == L && trt == R) return step_t<S, T, MAC, M, L, R>(h, bz0, bz1, f);
+    WT_CASE(true, true, true) WT_CASE(true, true, false) WT_CASE(true, false, true) WT_CASE(true, false, false)
+    WT_CASE(false, true, true) WT_CASE(false, true, false) WT_CASE(false, false, true) WT_CASE(false, false, false)
+#undef WT_CASE
+    return wt_fail(WT_E_STATE, "unreachable step configuration");
+}
+
+}  // namespace
+}  // namespace wt
+
+using namespace wt;
+
+// _writeEq's formula for one cell (js/lbm.js:155-162), binary64 in the
+// reference's order; the kernels round these values to the storage type.
+void eq_deviation(double r, double vx, double vy, double vz, double out[19])
+{
+    const double W[3] = {1.0 / 3, 1.0 / 18, 1.0 / 36};
+    const double usq = 1.5 * (vx * vx + vy * vy + vz * vz);
+    for (int q = 0; q < Q; q++) {
+        const double cu = 3 * ((double)EX[q] * vx + (double)EY[q] * vy + (double)EZ[q] * vz);
+        const double w = q == 0 ? W[0] : (q <= 6 ? W[1] : W[2]);
+        out[q] = w * ((r - 1) + r * (cu + 0.5 * cu * cu - usq));
+    }
+}
+
+int lbm_reset(wt_lattice *h)
+{
+    switch (h->storage) {
+    case WT_STORE_F32: return reset_t<float, float>(h);
+    case WT_STORE_F16: return reset_t<__half, float>(h);
+    default: return reset_t<double, double>(h);
+    }
+}
+
+int lbm_apply_voxelize(wt_lattice *h, const uint8_t *d_ops)
+{
+    switch (h->storage) {
+    case WT_STORE_F32: return ops_t<float, float>(h, d_ops);
+    case WT_STORE_F16: return ops_t<__half, float>(h, d_ops);
+    default: return ops_t<double, double>(h, d_ops);
+    }
+}
+
+int lbm_rebuild_masks(wt_lattice *h)
+{
+    const int64_t nwords = (int64_t)h->nwx * h->ny * h->nz;
+    hipLaunchKernelGGL(k_masks, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, h->stream, h->d_solid,
+                       h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx);
+    WT_HIP(hipGetLastError());
+    return 0;
+}
+
+int lbm_launch_step(wt_lattice *h, int bz0, int bz1, bool macro, double *d_force)
+{
+    if (h->storage == WT_STORE_F32 && h->arith == WT_ARITH_F32)
+        return step_dispatch<float, float, float>(h, bz0, bz1, macro, d_force);
+    if (h->storage == WT_STORE_F32 && h->arith == WT_ARITH_F64)
+        return step_dispatch<float, double, float>(h, bz0, bz1, macro, d_force);
+    if (h->storage == WT_STORE_F16)
+        return step_dispatch<__half, float, float>(h, bz0, bz1, macro, d_force);
+    return step_dispatch<double, double, double>(h, bz0, bz1, macro, d_force);
+}
+
+int lbm_grid_mask(wt_lattice *h, const uint8_t *d_grid, const wt_voxmeta *vm, const double M[9],
+                  const double pos[3], double scale, const int box[6], uint8_t *d_mask)
+{
+    GridArgs a{};
+    a.grid = d_grid; a.mask = d_mask;
+    a.gnx = vm->nx; a.gny = vm->ny; a.gnz = vm->nz;
+    for (int k = 0; k < 3; k++) { a.gmin[k] = vm->min[k]; a.pos[k] = pos[k]; }
+    for (int k = 0; k < 9; k++) a.M[k] = M[k];
+    a.ginv = vm->inv; a.inv = 1 / scale;
+    a.x0 = box[0]; a.y0 = box[1]; a.z0 = box[2];
+    a.bx = box[3] - box[0] + 1; a.by = box[4] - box[1] + 1; a.bz = box[5] - box[2] + 1;
+    const int64_t cnt = (int64_t)a.bx * a.by * a.bz;
+    if (cnt <= 0) return 0;
+    hipLaunchKernelGGL(k_grid_mask, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, a);
+    WT_HIP(hipGetLastError());
+    return 0;
+}

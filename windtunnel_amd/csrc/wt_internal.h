@@ -1,0 +1,94 @@
+// Internal declarations shared by the HIP translation units of libwindtunnel.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/windtunnel.h"
+
+namespace wt {
+
+// D3Q19 velocity set in the reference's slot order (js/lbm.js:28-37). The
+// slot index q is part of the storage format: f[q*n + cell].
+constexpr int Q = 19;
+constexpr int EX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int EY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int EZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+enum { FLUID = 0, SOLID_OBJ = 1, SOLID_WALL = 2 };
+
+// binary16 storage keeps (f_q - w_q) * F16_SCALE: a power of two, so the
+// scaling itself is exact, sized so |f_q - w_q| up to 1.99 stays below the
+// binary16 maximum while resolvable deviations stay far above its subnormals.
+constexpr float F16_SCALE = 32768.0f;
+constexpr float F16_INV_SCALE = 1.0f / 32768.0f;
+
+// Threads per block along x: one block spans a whole lattice row (up to
+// 1024 cells), so the +-1 x-shifted pull loads of neighbouring waves hit
+// the same CU's cache instead of re-fetching the straddled line from HBM.
+inline int row_block(int nx) { int b = (nx + 63) / 64 * 64; return b > 1024 ? 1024 : b; }
+
+struct Body {
+    int64_t frontalArea = 1, planformArea = 1, solidCells = 0, rideHeight = 0;
+};
+
+}  // namespace wt
+
+struct wt_lattice {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t n = 0;
+    int storage = WT_STORE_F32, arith = WT_ARITH_F32, layout = WT_LAYOUT_AB;
+    int pop_elem = 4;   // bytes per stored population
+    int mac_elem = 4;   // bytes per macroscopic field value
+    void *d_buf[2] = {nullptr, nullptr};
+    uint8_t *d_solid = nullptr;   // 0 fluid, 1 body, 2 road
+    uint32_t *d_sbits = nullptr;  // bit-packed solid mask, 32 cells along x per word
+    uint32_t *d_nbits = nullptr;  // bit-packed near-wall mask
+    int nwx = 0;                  // words per row of a bit mask
+    float *d_uw = nullptr;        // wall velocity, 3 planes of n floats
+    void *d_mac = nullptr;        // rho, ux, uy, uz planes of n values
+    double *d_partial = nullptr;  // per-block force partials
+    int64_t partial_cap = 0;
+    double *d_forces = nullptr;   // per-step force totals of the current batch
+    int forces_cap = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    wt_params p{0.5035, 0.14, 0.25, 0.06, 1, 1, 0};
+    int parity = 0;
+    int64_t steps = 0;
+    double force[3] = {0, 0, 0};
+    int ground = 0;
+    wt::Body body;
+    // inclusive range of z rows holding force-carrying (near-body) cells; -1 = none
+    int fz0 = -1, fz1 = -1;
+
+    std::vector<uint8_t> h_solid;  // host mirror of the solid array
+    std::vector<float> h_uw;       // host mirror of the wall velocities (3n)
+};
+
+// error reporting (capi.hip)
+int wt_fail(int code, const char *fmt, ...);
+#define WT_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return wt_fail(WT_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// lattice kernels (lbm.hip)
+int lbm_reset(wt_lattice *h);
+int lbm_apply_voxelize(wt_lattice *h, const uint8_t *ops);
+int lbm_rebuild_masks(wt_lattice *h);
+int lbm_launch_step(wt_lattice *h, int bz0, int bz1, bool macro, double *force_out);
+int lbm_get_pops(wt_lattice *h, int which, void *out);
+int lbm_grid_mask(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, const double M[9],
+                  const double pos[3], double scale, const int box[6], uint8_t *mask);
+
+// host reference formulas shared by host and kernels
+void eq_deviation(double r, double vx, double vy, double vz, double out[19]);
