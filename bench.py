@@ -1,0 +1,170 @@
+"""Throughput of the D3Q19 step on one MI355X: MLUPS and fraction of the HBM roofline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2] [--no-cpu]
+
+Workload (default): D3Q19 TRT (Lambda = 3/16) with Smagorinsky LES (Cs = 0.1),
+tau = 0.505, inflow u0 = 0.08, on a 512x256x256 lattice (33.5 M cells) around
+the synthetic bluff body scaled 2x (ellipsoid 80x28x24 semi-axes unioned
+with a 60x32x20 box). One step is one fused stream/collide/bounce-back/force
+launch over the whole lattice plus the per-step force reduction; the populations
+are resident in HBM before timing starts. The headline value is the float32
+storage path; the binary16-offset storage path runs on the same lattice and is
+reported beside it.
+
+Roofline: HBM bound. Algorithmic bytes per cell update = 19 populations read +
+19 written = 152 B (float32) or 76 B (binary16); MLUPS counts all nx*ny*nz
+cells as the reference's benchmark does (js/main.js:621).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "D3Q19 MLUPS & % HBM roofline (fp32 vs fp16-store); adjoint/forward cost"
+
+WORKLOADS = {
+    # TRT+LES around the synthetic body; the 512^2x256 form is where the
+    # throughput targets are quoted, 256x128x128 is its half-scale sibling.
+    "c3": dict(dims=(512, 256, 256), scale=2, tau=0.505, u0=0.08, lam=3 / 16, csm=0.1),
+    "c2": dict(dims=(256, 128, 128), scale=1, tau=0.505, u0=0.08, lam=3 / 16, csm=0.1),
+}
+
+
+def body_mask(dims, scale):
+    """Ellipsoid (40,14,12)*s semi-axes unioned with a (30,16,10)*s box centred
+    (22,-6,0)*s from the ellipsoid centre; centre at (0.3 nx, ny/2, nz/2).
+    Point test at cell centres over the clamped box, x fastest."""
+    nx, ny, nz = dims
+    s = scale
+    c = np.array([0.3 * nx, ny / 2, nz / 2])
+    ex, ey, ez = 40 * s, 14 * s, 12 * s
+    bc = c + np.array([22 * s, -6 * s, 0])
+    bh = np.array([15 * s, 8 * s, 5 * s])
+    lo = np.floor(np.minimum(c - [ex, ey, ez], bc - bh)).astype(int) - 1
+    hi = np.ceil(np.maximum(c + [ex, ey, ez], bc + bh)).astype(int) + 1
+    lo = np.maximum(lo, 1)
+    hi = np.minimum(hi, np.array(dims) - 2)
+    z, y, x = np.meshgrid(np.arange(lo[2], hi[2] + 1), np.arange(lo[1], hi[1] + 1),
+                          np.arange(lo[0], hi[0] + 1), indexing="ij")
+    X, Y, Z = x + 0.5, y + 0.5, z + 0.5
+    a, b, cc = (X - c[0]) / ex, (Y - c[1]) / ey, (Z - c[2]) / ez
+    ins = a * a + b * b + cc * cc <= 1
+    ins |= (np.abs(X - bc[0]) <= bh[0]) & (np.abs(Y - bc[1]) <= bh[1]) & (np.abs(Z - bc[2]) <= bh[2])
+    box = [int(lo[0]), int(lo[1]), int(lo[2]), int(hi[0]), int(hi[1]), int(hi[2])]
+    return box, ins.astype(np.uint8).reshape(-1)
+
+
+def run_gpu(wl, storage, steps, warmup):
+    from windtunnel_amd.lbm import LBM
+    dims = wl["dims"]
+    L = LBM(*dims, storage=storage)
+    L.set_params(tau=wl["tau"], les=True, csm=wl["csm"], trt=True, u0=wl["u0"], **{"lambda": wl["lam"]})
+    L.reset()
+    box, mask = body_mask(dims, wl["scale"])
+    L.voxelize_mask(box, mask)
+    L.step(warmup)
+    L.set_timing(True)
+    t0 = time.perf_counter()
+    forces = L.step(steps)          # returns after the stream has drained
+    wall = time.perf_counter() - t0
+    kms, launches = L.timing()
+    L.set_timing(False)
+    n = L.n
+    bpc = 76 if storage == "f16" else 152
+    kavg = kms / launches
+    res = dict(
+        mlups=n * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps, kernel_ms=kavg,
+        achieved_gbs=n * bpc / (kavg * 1e-3) / 1e9, bytes_per_cell=bpc,
+        solid_cells=L.solidCells, frontal_area=L.frontalArea,
+        finite=bool(np.all(np.isfinite(forces))),
+        cd_mean=float(np.mean(forces[:, 0]) / (0.5 * wl["u0"] ** 2 * L.frontalArea)),
+    )
+    L.close()
+    return res
+
+
+def run_cpu(wl, threads):
+    """The oracle (C restatement of js/lbm.js with the worker-pool slab split
+    of js/lbmWorker.js) on the host cores: 2 warm-up steps, then best of 3 x 2
+    steps on the full lattice, the reference benchmark's best-of protocol
+    (js/main.js:614-620) shortened to bound the run time."""
+    from oracle.oracle import OracleLBM
+    dims = wl["dims"]
+    O = OracleLBM(*dims)
+    O.set_params(tau=wl["tau"], les=True, csm=wl["csm"], trt=True, u0=wl["u0"], **{"lambda": wl["lam"]})
+    O.reset()
+    box, mask = body_mask(dims, wl["scale"])
+    O.voxelize_mask(box, mask)
+    O.step_pool(2, threads, threaded=True)
+    best = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.step_pool(2, threads, threaded=True)
+        best = min(best, (time.perf_counter() - t0) / 2)
+    n = O.n
+    del O
+    return n / best / 1e6
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f16", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only; the multi-GPU path (z-slab halo exchange) is not built",
+              file=sys.stderr)
+        sys.exit(2)
+    wl = WORKLOADS[a.workload]
+    nx, ny, nz = wl["dims"]
+    print(f"[bench] {a.workload} {nx}x{ny}x{nz} f32 warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+    r32 = run_gpu(wl, "f32", a.steps, a.warmup)
+    print(f"[bench] f32 {r32['mlups']:.0f} MLUPS kernel {r32['kernel_ms']:.3f} ms", file=sys.stderr, flush=True)
+    out = {
+        "metric": METRIC, "value": round(r32["mlups"], 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(r32["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"D3Q19 TRT(Lambda=3/16)+LES(Cs=0.1) {nx}x{ny}x{nz}, tau=0.505, u0=0.08, "
+                               f"synthetic ellipsoid+box body x{wl['scale']} (analytic point test)",
+                   "nx": nx, "ny": ny, "nz": nz, "storage": "f32", "arith": "f32", "layout": "ab",
+                   "solid_cells": r32["solid_cells"]},
+        "roofline": {"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": round(r32["kernel_ms"], 5), "bytes_per_cell": 152},
+    }
+    if not a.no_f16:
+        print("[bench] f16", file=sys.stderr, flush=True)
+        r16 = run_gpu(wl, "f16", a.steps, a.warmup)
+        out["fp16"] = {"value": round(r16["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r16["ms_per_step"], 4),
+                       "speedup_vs_fp32": round(r16["mlups"] / r32["mlups"], 3),
+                       "roofline": {"bound": "hbm", "achieved": round(r16["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                                    "unit": "GB/s", "frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                                    "kernel_ms": round(r16["kernel_ms"], 5), "bytes_per_cell": 76}}
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        print(f"[bench] cpu baseline, {threads} threads", file=sys.stderr, flush=True)
+        v = run_cpu(wl, threads)
+        out["cpu_baseline"] = {"value": round(v, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+                               "sample": f"oracle (C restatement of js/lbm.js, binary64 arithmetic) on the full "
+                                         f"{nx}x{ny}x{nz} lattice, {threads} z-slab threads, best of 3x2 steps "
+                                         f"after 2 warm-up steps; host {platform.processor() or platform.machine()}"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -157,6 +157,13 @@ int wt_get_solid(wt_lattice *h, uint8_t *solid, uint8_t *near);
  * stream (no host sync inside the timed region). */
 int wt_time_steps(wt_lattice *h, int nsteps, double *ms);
 
+/* Per-launch timing of the step kernel: while on, every step kernel is
+ * bracketed by HIP events on the lattice's stream and its duration added to
+ * a running total (the force-sum kernel is excluded). Turning it on clears
+ * the totals. */
+int wt_set_timing(wt_lattice *h, int on);
+int wt_get_timing(const wt_lattice *h, double *kernel_ms, int64_t *launches);
+
 /* Voxel.bounds (js/voxel.js:32-42) over len floats. */
 int wt_voxel_bounds(const float *P, int64_t len, double out[6]);
 

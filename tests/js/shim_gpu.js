@@ -1,0 +1,111 @@
+/* Node drop-in checks on a GPU: the reference's own usage patterns
+ * (new LBM / voxelize / step / force, the worker pool, Voxel.build) run
+ * through windtunnel_amd/js against golden vectors from the reference. */
+'use strict';
+const path = require('path');
+const { load, arr, sha, assert, close, applyParams, replay } = require('./fixtures.js');
+const JS = path.join(__dirname, '..', '..', 'windtunnel_amd', 'js');
+const { LBM } = require(path.join(JS, 'lbm.js'));
+const { Pool } = require(path.join(JS, 'pool.js'));
+const Voxel = require(path.join(JS, 'voxel.js'));
+
+const checks = [];
+function check(name, fn) { checks.push([name, fn]); }
+
+check('exact LBM object reproduces lbm_bgk_sphere', () => {
+  const fx = load('lbm_bgk_sphere');
+  const L = new LBM(...fx.dims, { arith: 'f64' });
+  applyParams(L, fx.params);
+  L.reset();
+  replay(L, fx.voxelize);
+  assert(L.frontalArea === fx.voxelize.result.frontalArea, 'frontal area');
+  assert(sha(L.solid) === fx.voxelize.after.solid && sha(L.near) === fx.voxelize.after.near, 'solid/near');
+  const want = arr(fx.forces);
+  for (let s = 0; s < fx.nsteps; s++) {
+    L.step();
+    close(L.force, want.subarray(3 * s, 3 * s + 3), 1e-11, 1e-14, 'force step ' + s);
+  }
+  const snap = fx.snaps[String(fx.nsteps)];
+  for (const k of ['rho', 'ux', 'uy', 'uz']) assert(sha(L[k]) === snap[k], k);
+  assert(sha(L.f) === snap.f && sha(L.g) === snap.g, 'populations');
+  assert(L.steps === fx.nsteps && L.parity === snap.parity, 'steps/parity');
+  for (const s of fx.samples) {
+    const o = [0, 0, 0];
+    L.sample(s.p[0], s.p[1], s.p[2], o);
+    assert(o[0] === s.u[0] && o[1] === s.u[1] && o[2] === s.u[2], 'sample ' + s.p);
+    assert(L.isSolid(s.p[0], s.p[1], s.p[2]) === s.solid, 'isSolid ' + s.p);
+  }
+});
+
+check('closure voxelize equals the recorded mask (lbm_trt_les)', () => {
+  const fx = load('lbm_trt_les');
+  const L = new LBM(...fx.dims, { arith: 'f64' });
+  applyParams(L, fx.params);
+  L.reset();
+  const ell = (x, y, z) => {
+    const a = (x - 10) / 5, b = (y - 8) / 3, c = (z - 8) / 2.5;
+    if (a * a + b * b + c * c <= 1) return true;
+    return Math.abs(x - 14) <= 3 && Math.abs(y - 7) <= 2 && Math.abs(z - 8) <= 1.5;
+  };
+  L.voxelize(ell, { x0: 4, y0: 4, z0: 4, x1: 18, y1: 12, z1: 12 }, false);
+  assert(sha(L.solid) === fx.voxelize.after.solid, 'solid');
+  const f = L.stepMany(fx.nsteps);
+  close(f, arr(fx.forces), 1e-11, 1e-14, 'forces');
+  for (const k of ['rho', 'ux', 'uy', 'uz']) assert(sha(L[k]) === fx.snaps[String(fx.nsteps)][k], k);
+});
+
+check('worker pool protocol reproduces lbm_pool3', async () => {
+  const fx = load('lbm_pool3');
+  const L = new LBM(...fx.dims, { arith: 'f64' });
+  applyParams(L, fx.main_params);
+  L.reset();
+  replay(L, fx.voxelize);
+  const pool = await new Pool(L, fx.nWorkers).start();
+  assert(pool.slabs.map(s => [s.z0, s.z1]).join() === fx.slabs.map(s => [s[0], s[1]]).join(), 'slab split');
+  pool.syncParams();                       // tau/les/u0/movingWalls only, as the reference sends
+  const want = arr(fx.forces);
+  let done = 0;
+  while (done < fx.nsteps) {
+    const n = Math.min(16, fx.nsteps - done);
+    const sf = await pool.step(n);
+    for (let s = 0; s < n; s++) close(sf[s], want.subarray(3 * (done + s), 3 * (done + s) + 3), 1e-11, 1e-14, 'pool force');
+    done += n;
+  }
+  pool.dispose();
+  for (const k of ['rho', 'ux', 'uy', 'uz']) assert(sha(L[k]) === fx.pooled[k], k);
+  assert(L.parity === fx.pooled.parity && L.steps === fx.pooled.steps, 'parity/steps');
+});
+
+check('Voxel.build is bit-exact (vox_box, vox_hitcap, vox_sphere)', () => {
+  for (const name of ['vox_box', 'vox_hitcap', 'vox_sphere']) {
+    const fx = load(name);
+    const v = Voxel.build(arr(fx.mesh), fx.res);
+    assert(sha(v.grid) === fx.grid_sha256, name + ' grid');
+    assert(v.stats.solid === fx.stats.solid && v.stats.openness === fx.stats.openness, name + ' stats');
+    assert(v.nx === fx.nx && v.cell === fx.cell && v.min.join() === fx.min.join(), name + ' geometry');
+  }
+  let msg = null;
+  try { Voxel.build(new Float32Array(0)); } catch (e) { msg = e.message; }
+  assert(msg === load('vox_errors').empty, 'error message');
+});
+
+check('injected device fault surfaces as an Error', () => {
+  // WT_FAULT=step:3 is set by the runner for this process
+  const L = new LBM(16, 8, 8);
+  L.stepMany(3);
+  let err = null;
+  try { L.step(); } catch (e) { err = e; }
+  assert(err && err.code === 'WT-5', 'fault code');
+});
+
+(async () => {
+  let failed = 0;
+  // the injected-fault check needs WT_FAULT in its own process
+  const only = process.argv[2] === 'fault';
+  for (const [name, fn] of checks) {
+    if (only !== name.startsWith('injected')) continue;
+    try { await fn(); console.log('ok   ' + name); } catch (e) { failed++; console.log('FAIL ' + name + ': ' + e.message); }
+  }
+  console.log(failed ? `${failed} failed` : 'all passed');
+  process.exit(failed ? 1 : 0);
+})();

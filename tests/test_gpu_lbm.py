@@ -222,3 +222,82 @@ def test_grid_lookup_on_gpu():
     assert sha(L.near) == rec["after"]["near"]
     assert L.frontalArea == rec["result"]["frontalArea"]
     assert L.solidCells == rec["result"]["solidCells"]
+
+
+# ------------------------------------------------------------ esoteric pull
+
+def stored_mask(dims, solid):
+    """Entries of f[q*n+cell] an in-place lattice stores: non-solid cells,
+    and for odd q only where the neighbour along c_q is inside the lattice."""
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y, z = x.reshape(-1), y.reshape(-1), z.reshape(-1)
+    ok = []
+    for q in range(19):
+        m = solid == 0
+        if q % 2 == 1:
+            X, Y, Zc = x + LBM.EX[q], y + LBM.EY[q], z + LBM.EZ[q]
+            m = m & (X >= 0) & (X < nx) & (Y >= 0) & (Y < ny) & (Zc >= 0) & (Zc < nz)
+        ok.append(m)
+    return np.concatenate(ok)
+
+
+@pytest.mark.parametrize("name", ["lbm_bgk_sphere", "lbm_trt_les", "lbm_moving", "lbm_revox"])
+def test_esoteric_exact_mode(name):
+    """In-place streaming with binary64 collision: the macroscopic fields are
+    the reference's bit for bit, and every stored population equals the
+    oracle's f."""
+    fx = load(name)
+    L = make(fx, arith="f64", layout="esoteric")
+    print(f"[esoteric exact] {name}", flush=True)
+    replay_voxelize(L, fx["voxelize"])
+    O = oracle_for(fx)
+    want_f = arr(fx["forces"])
+    done = 0
+    for stop in sorted(int(k) for k in fx["snaps"] if k != "raw"):
+        f = L.step(stop - done)
+        O.step(stop - done)
+        np.testing.assert_allclose(f, want_f[done:stop], rtol=1e-11, atol=1e-14)
+        done = stop
+        F = L.fields()
+        for k in FIELDS:
+            assert sha(F[k]) == fx["snaps"][str(stop)][k], (stop, k)
+        m = stored_mask(fx["dims"], O.array("solid"))
+        assert np.array_equal(L.f[m], O.array("f")[m]), f"populations differ at step {stop}"
+    if "revox" in fx:
+        replay_voxelize(L, fx["revox"])
+        L.step(30)
+        for k in FIELDS:
+            assert sha(L.fields()[k]) == fx["snaps2"]["60"][k]
+
+
+def test_esoteric_fp32_equals_ab_fp32():
+    fx = load("lbm_moving")
+    out = {}
+    for lay in ("ab", "esoteric"):
+        L = make(fx, layout=lay)
+        replay_voxelize(L, fx["voxelize"])
+        f = L.step(fx["nsteps"])
+        out[lay] = (f, {k: v.copy() for k, v in L.fields().items()})
+    for k in FIELDS:
+        assert np.array_equal(out["ab"][1][k], out["esoteric"][1][k]), k
+    np.testing.assert_allclose(out["ab"][0], out["esoteric"][0], rtol=1e-12, atol=1e-15)
+
+
+def test_esoteric_pool_slabs_and_f16():
+    fx = load("lbm_pool3")
+    L = LBM(*fx["dims"], arith="f64", layout="esoteric")
+    set_params(L, fx["worker_params"])
+    L.reset()
+    replay_voxelize(L, fx["voxelize"])
+    L.step_slabs(fx["nsteps"], fx["nWorkers"])
+    for k in FIELDS:
+        assert sha(L.fields()[k]) == fx["pooled"][k], k
+    fx = load("lbm_trt_les")
+    a = make(fx, storage="f16")
+    b = make(fx, storage="f16", layout="esoteric")
+    for L in (a, b):
+        replay_voxelize(L, fx["voxelize"])
+        L.step(40)
+    for k in FIELDS:
+        assert np.array_equal(a.fields()[k], b.fields()[k]), k

@@ -1,0 +1,484 @@
+/* wt_napi.c -- Node.js N-API addon over the libwindtunnel C ABI.
+ *
+ * A thin 1:1 binding: every export forwards to one wt_* entry point of
+ * include/windtunnel.h, passing typed-array storage as plain pointers. The
+ * JavaScript side (windtunnel_amd/js/lbm.js) rebuilds the reference's LBM
+ * object API, the lbmWorker.js message protocol and Voxel.build on top.
+ * Failures throw a JavaScript Error carrying wt_last_error(); voxelizer
+ * errors therefore carry the reference's own messages (js/voxel.js:51,56,71).
+ */
+#define NAPI_VERSION 6
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/windtunnel.h"
+
+#define CALL(env, x)                                                      \
+    do {                                                                  \
+        if ((x) != napi_ok) {                                             \
+            napi_throw_error((env), NULL, "N-API call failed: " #x);      \
+            return NULL;                                                  \
+        }                                                                 \
+    } while (0)
+
+#define WT(env, x)                                                        \
+    do {                                                                  \
+        int rc_ = (x);                                                    \
+        if (rc_ != 0) {                                                   \
+            char code_[16];                                               \
+            snprintf(code_, sizeof code_, "WT%d", rc_);                   \
+            napi_throw_error((env), code_, wt_last_error());              \
+            return NULL;                                                  \
+        }                                                                 \
+    } while (0)
+
+static napi_value args(napi_env env, napi_callback_info info, size_t want, napi_value *argv)
+{
+    size_t argc = want;
+    if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok) return NULL;
+    for (size_t k = argc; k < want; k++) napi_get_undefined(env, &argv[k]);
+    return argv[0];
+}
+
+static int is_nullish(napi_env env, napi_value v)
+{
+    napi_valuetype t;
+    napi_typeof(env, v, &t);
+    return t == napi_undefined || t == napi_null;
+}
+
+static wt_lattice *handle(napi_env env, napi_value v)
+{
+    void *p = NULL;
+    if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+        napi_throw_type_error(env, NULL, "expected a lattice handle");
+        return NULL;
+    }
+    return (wt_lattice *)p;
+}
+
+/* typed-array data pointer and byte length (NULL for null/undefined) */
+static void *tdata(napi_env env, napi_value v, size_t *bytes, int *ok)
+{
+    *ok = 1;
+    if (bytes) *bytes = 0;
+    if (is_nullish(env, v)) return NULL;
+    napi_typedarray_type type;
+    size_t len = 0, off = 0;
+    void *data = NULL;
+    napi_value ab;
+    if (napi_get_typedarray_info(env, v, &type, &len, &data, &ab, &off) != napi_ok) {
+        napi_throw_type_error(env, NULL, "expected a typed array");
+        *ok = 0;
+        return NULL;
+    }
+    size_t esz = 1;
+    switch (type) {
+    case napi_int16_array: case napi_uint16_array: esz = 2; break;
+    case napi_int32_array: case napi_uint32_array: case napi_float32_array: esz = 4; break;
+    case napi_float64_array: case napi_bigint64_array: case napi_biguint64_array: esz = 8; break;
+    default: esz = 1;
+    }
+    if (bytes) *bytes = len * esz;
+    return data;
+}
+
+static double num(napi_env env, napi_value v, double dflt)
+{
+    double d;
+    if (napi_get_value_double(env, v, &d) != napi_ok) return dflt;
+    return d;
+}
+
+static int boolish(napi_env env, napi_value v)
+{
+    napi_value b;
+    bool r = false;
+    if (napi_coerce_to_bool(env, v, &b) == napi_ok) napi_get_value_bool(env, b, &r);
+    return r ? 1 : 0;
+}
+
+static double prop_num(napi_env env, napi_value o, const char *k, double dflt)
+{
+    napi_value v;
+    bool has = false;
+    if (napi_has_named_property(env, o, k, &has) != napi_ok || !has) return dflt;
+    napi_get_named_property(env, o, k, &v);
+    return num(env, v, dflt);
+}
+
+static void set_num(napi_env env, napi_value o, const char *k, double v)
+{
+    napi_value x;
+    napi_create_double(env, v, &x);
+    napi_set_named_property(env, o, k, x);
+}
+
+static void set_bool(napi_env env, napi_value o, const char *k, int v)
+{
+    napi_value x;
+    napi_get_boolean(env, v != 0, &x);
+    napi_set_named_property(env, o, k, x);
+}
+
+static void finalize(napi_env env, void *data, void *hint)
+{
+    (void)env; (void)hint;
+    wt_destroy((wt_lattice *)data);
+}
+
+/* create(nx, ny, nz, storage, arith, layout) -> handle */
+static napi_value js_create(napi_env env, napi_callback_info info)
+{
+    napi_value a[6], out;
+    args(env, info, 6, a);
+    wt_lattice *h = NULL;
+    WT(env, wt_create((int)num(env, a[0], 0), (int)num(env, a[1], 0), (int)num(env, a[2], 0),
+                      (int)num(env, a[3], 0), (int)num(env, a[4], 0), (int)num(env, a[5], 0), &h));
+    CALL(env, napi_create_external(env, h, finalize, NULL, &out));
+    return out;
+}
+
+static napi_value js_set_params(napi_env env, napi_callback_info info)
+{
+    napi_value a[2];
+    args(env, info, 2, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    wt_params p;
+    WT(env, wt_get_params(h, &p));
+    p.tau = prop_num(env, a[1], "tau", p.tau);
+    p.csm = prop_num(env, a[1], "csm", p.csm);
+    p.lambda = prop_num(env, a[1], "lambda", p.lambda);
+    p.u0 = prop_num(env, a[1], "u0", p.u0);
+    p.les = (int)prop_num(env, a[1], "les", p.les) != 0;
+    p.trt = (int)prop_num(env, a[1], "trt", p.trt) != 0;
+    p.movingWalls = (int)prop_num(env, a[1], "movingWalls", p.movingWalls) != 0;
+    WT(env, wt_set_params(h, &p));
+    return NULL;
+}
+
+static napi_value js_get_params(napi_env env, napi_callback_info info)
+{
+    napi_value a[1], o;
+    args(env, info, 1, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    wt_params p;
+    WT(env, wt_get_params(h, &p));
+    CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "tau", p.tau); set_num(env, o, "csm", p.csm); set_num(env, o, "lambda", p.lambda);
+    set_num(env, o, "u0", p.u0); set_bool(env, o, "les", p.les); set_bool(env, o, "trt", p.trt);
+    set_bool(env, o, "movingWalls", p.movingWalls);
+    return o;
+}
+
+static napi_value js_reset(napi_env env, napi_callback_info info)
+{
+    napi_value a[1];
+    args(env, info, 1, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    WT(env, wt_reset(h));
+    return NULL;
+}
+
+static napi_value stats_object(napi_env env, const wt_body_stats *s)
+{
+    napi_value o;
+    if (napi_create_object(env, &o) != napi_ok) return NULL;
+    set_num(env, o, "frontalArea", (double)s->frontalArea);
+    set_num(env, o, "planformArea", (double)s->planformArea);
+    set_num(env, o, "solidCells", (double)s->solidCells);
+    set_num(env, o, "rideHeight", (double)s->rideHeight);
+    set_bool(env, o, "movingWalls", s->movingWalls);
+    set_bool(env, o, "ground", s->ground);
+    return o;
+}
+
+/* voxelize(h, box Int32Array(6), mask Uint8Array, wv Float32Array|null, ground, roadVel) */
+static napi_value js_voxelize(napi_env env, napi_callback_info info)
+{
+    napi_value a[6];
+    args(env, info, 6, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    size_t nb;
+    const int *box = tdata(env, a[1], &nb, &ok);
+    if (!ok) return NULL;
+    if (!box || nb < 24) { napi_throw_type_error(env, NULL, "box must be an Int32Array(6)"); return NULL; }
+    const uint8_t *mask = tdata(env, a[2], NULL, &ok);
+    if (!ok) return NULL;
+    const float *wv = tdata(env, a[3], NULL, &ok);
+    if (!ok) return NULL;
+    wt_body_stats s;
+    WT(env, wt_voxelize(h, box, mask, wv, boolish(env, a[4]), num(env, a[5], 0), &s));
+    return stats_object(env, &s);
+}
+
+/* voxelizeGrid(h, grid Uint8Array, meta {nx,ny,nz,min,inv,cell}, M Float64Array(9),
+ *              pos Float64Array(3), scale, box Int32Array(6), ground, roadVel) */
+static napi_value js_voxelize_grid(napi_env env, napi_callback_info info)
+{
+    napi_value a[9];
+    args(env, info, 9, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    const uint8_t *grid = tdata(env, a[1], NULL, &ok);
+    if (!ok) return NULL;
+    wt_voxmeta vm;
+    memset(&vm, 0, sizeof vm);
+    vm.nx = (int)prop_num(env, a[2], "nx", 0);
+    vm.ny = (int)prop_num(env, a[2], "ny", 0);
+    vm.nz = (int)prop_num(env, a[2], "nz", 0);
+    vm.inv = prop_num(env, a[2], "inv", 0);
+    vm.cell = prop_num(env, a[2], "cell", 0);
+    napi_value mn;
+    if (napi_get_named_property(env, a[2], "min", &mn) == napi_ok) {
+        for (uint32_t k = 0; k < 3; k++) {
+            napi_value e;
+            if (napi_get_element(env, mn, k, &e) == napi_ok) vm.min[k] = num(env, e, 0);
+        }
+    }
+    const double *M = tdata(env, a[3], NULL, &ok);
+    if (!ok) return NULL;
+    const double *pos = tdata(env, a[4], NULL, &ok);
+    if (!ok) return NULL;
+    const int *box = tdata(env, a[6], NULL, &ok);
+    if (!ok) return NULL;
+    wt_body_stats s;
+    WT(env, wt_voxelize_grid(h, grid, &vm, M, pos, num(env, a[5], 1), box, boolish(env, a[7]), num(env, a[8], 0), &s));
+    return stats_object(env, &s);
+}
+
+/* step(h, n, forces Float64Array|null) */
+static napi_value js_step(napi_env env, napi_callback_info info)
+{
+    napi_value a[3];
+    args(env, info, 3, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    size_t nb;
+    const int n = (int)num(env, a[1], 1);
+    double *f = tdata(env, a[2], &nb, &ok);
+    if (!ok) return NULL;
+    if (f && nb < (size_t)n * 24) { napi_throw_range_error(env, NULL, "forces array too short"); return NULL; }
+    WT(env, wt_step(h, n, f));
+    return NULL;
+}
+
+/* stepSlabs(h, n, nslabs, forces Float64Array|null) */
+static napi_value js_step_slabs(napi_env env, napi_callback_info info)
+{
+    napi_value a[4];
+    args(env, info, 4, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    size_t nb;
+    const int n = (int)num(env, a[1], 1), ns = (int)num(env, a[2], 1);
+    double *f = tdata(env, a[3], &nb, &ok);
+    if (!ok) return NULL;
+    if (f && nb < (size_t)n * ns * 24) { napi_throw_range_error(env, NULL, "forces array too short"); return NULL; }
+    WT(env, wt_step_slabs(h, n, ns, f));
+    return NULL;
+}
+
+static napi_value js_set_parity(napi_env env, napi_callback_info info)
+{
+    napi_value a[2];
+    args(env, info, 2, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    WT(env, wt_set_parity(h, (int)num(env, a[1], 0)));
+    return NULL;
+}
+
+static napi_value js_get_state(napi_env env, napi_callback_info info)
+{
+    napi_value a[1], o, f;
+    args(env, info, 1, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    wt_state_t s;
+    WT(env, wt_get_state(h, &s));
+    CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "parity", (double)s.parity);
+    set_num(env, o, "steps", (double)s.steps);
+    CALL(env, napi_create_array_with_length(env, 3, &f));
+    for (uint32_t k = 0; k < 3; k++) {
+        napi_value e;
+        napi_create_double(env, s.force[k], &e);
+        napi_set_element(env, f, k, e);
+    }
+    napi_set_named_property(env, o, "force", f);
+    return o;
+}
+
+/* getFields(h, rho, ux, uy, uz) -- typed arrays or null */
+static napi_value js_get_fields(napi_env env, napi_callback_info info)
+{
+    napi_value a[5];
+    args(env, info, 5, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    void *p[4];
+    for (int k = 0; k < 4; k++) {
+        p[k] = tdata(env, a[1 + k], NULL, &ok);
+        if (!ok) return NULL;
+    }
+    WT(env, wt_get_fields(h, p[0], p[1], p[2], p[3]));
+    return NULL;
+}
+
+static napi_value js_get_pops(napi_env env, napi_callback_info info)
+{
+    napi_value a[3];
+    args(env, info, 3, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    void *out = tdata(env, a[2], NULL, &ok);
+    if (!ok) return NULL;
+    WT(env, wt_get_pops(h, (int)num(env, a[1], 0), out));
+    return NULL;
+}
+
+static napi_value js_get_solid(napi_env env, napi_callback_info info)
+{
+    napi_value a[3];
+    args(env, info, 3, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    uint8_t *s = tdata(env, a[1], NULL, &ok);
+    if (!ok) return NULL;
+    uint8_t *nr = tdata(env, a[2], NULL, &ok);
+    if (!ok) return NULL;
+    WT(env, wt_get_solid(h, s, nr));
+    return NULL;
+}
+
+static napi_value js_time_steps(napi_env env, napi_callback_info info)
+{
+    napi_value a[2], out;
+    args(env, info, 2, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    double ms = 0;
+    WT(env, wt_time_steps(h, (int)num(env, a[1], 1), &ms));
+    CALL(env, napi_create_double(env, ms, &out));
+    return out;
+}
+
+static napi_value meta_object(napi_env env, const wt_voxmeta *m)
+{
+    napi_value o, arr;
+    if (napi_create_object(env, &o) != napi_ok) return NULL;
+    set_num(env, o, "nx", m->nx); set_num(env, o, "ny", m->ny); set_num(env, o, "nz", m->nz);
+    set_num(env, o, "cell", m->cell); set_num(env, o, "inv", m->inv);
+    const struct { const char *k; const double *v; uint32_t n; } vec[] = {
+        {"min", m->min, 3}, {"ext", m->ext, 3}, {"centre", m->centre, 3}, {"bbox", m->bbox, 6}};
+    for (int i = 0; i < 4; i++) {
+        napi_create_array_with_length(env, vec[i].n, &arr);
+        for (uint32_t k = 0; k < vec[i].n; k++) {
+            napi_value e;
+            napi_create_double(env, vec[i].v[k], &e);
+            napi_set_element(env, arr, k, e);
+        }
+        napi_set_named_property(env, o, vec[i].k, arr);
+    }
+    set_num(env, o, "tris", m->ntri);
+    set_num(env, o, "solid", (double)m->solid);
+    set_num(env, o, "disagree", (double)m->disagree);
+    set_num(env, o, "openness", m->openness);
+    return o;
+}
+
+/* voxelBounds(P Float32Array) -> [x0,y0,z0,x1,y1,z1] */
+static napi_value js_voxel_bounds(napi_env env, napi_callback_info info)
+{
+    napi_value a[1], arr;
+    args(env, info, 1, a);
+    int ok;
+    size_t nb;
+    const float *P = tdata(env, a[0], &nb, &ok);
+    if (!ok) return NULL;
+    double b[6];
+    WT(env, wt_voxel_bounds(P, (int64_t)(nb / 4), b));
+    CALL(env, napi_create_array_with_length(env, 6, &arr));
+    for (uint32_t k = 0; k < 6; k++) {
+        napi_value e;
+        napi_create_double(env, b[k], &e);
+        napi_set_element(env, arr, k, e);
+    }
+    return arr;
+}
+
+/* voxelPlan(P, res) -> meta ; voxelBuild(P, res, grid Uint8Array) -> meta */
+static napi_value js_voxel_plan(napi_env env, napi_callback_info info)
+{
+    napi_value a[2];
+    args(env, info, 2, a);
+    int ok;
+    size_t nb;
+    const float *P = tdata(env, a[0], &nb, &ok);
+    if (!ok) return NULL;
+    wt_voxmeta m;
+    WT(env, wt_voxel_plan(P, (int64_t)(nb / 4), (int)num(env, a[1], 96), &m));
+    return meta_object(env, &m);
+}
+
+static napi_value js_voxel_build(napi_env env, napi_callback_info info)
+{
+    napi_value a[3];
+    args(env, info, 3, a);
+    int ok;
+    size_t nb, gb;
+    const float *P = tdata(env, a[0], &nb, &ok);
+    if (!ok) return NULL;
+    uint8_t *grid = tdata(env, a[2], &gb, &ok);
+    if (!ok) return NULL;
+    wt_voxmeta m;
+    WT(env, wt_voxel_plan(P, (int64_t)(nb / 4), (int)num(env, a[1], 96), &m));
+    if (!grid || gb < (size_t)m.nx * m.ny * m.nz) {
+        napi_throw_range_error(env, NULL, "grid array too short");
+        return NULL;
+    }
+    WT(env, wt_voxel_build(P, (int64_t)(nb / 4), (int)num(env, a[1], 96), &m, grid));
+    return meta_object(env, &m);
+}
+
+static napi_value js_abi(napi_env env, napi_callback_info info)
+{
+    (void)info;
+    napi_value v;
+    napi_create_int32(env, wt_abi_version(), &v);
+    return v;
+}
+
+static napi_value init(napi_env env, napi_value exports)
+{
+    const struct { const char *name; napi_callback cb; } fns[] = {
+        {"abiVersion", js_abi}, {"create", js_create}, {"setParams", js_set_params},
+        {"getParams", js_get_params}, {"reset", js_reset}, {"voxelize", js_voxelize},
+        {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs},
+        {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
+        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps},
+        {"voxelBounds", js_voxel_bounds}, {"voxelPlan", js_voxel_plan}, {"voxelBuild", js_voxel_build}};
+    for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
+        napi_value f;
+        if (napi_create_function(env, fns[i].name, NAPI_AUTO_LENGTH, fns[i].cb, NULL, &f) != napi_ok) return NULL;
+        if (napi_set_named_property(env, exports, fns[i].name, f) != napi_ok) return NULL;
+    }
+    return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

@@ -73,6 +73,7 @@ void release(wt_lattice *h)
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
                     (void *)h->d_forces})
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -116,7 +117,7 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
                     (storage == WT_STORE_F16 && arith == WT_ARITH_F32) ||
                     (storage == WT_STORE_F64 && arith == WT_ARITH_F64);
     if (!ok) return wt_fail(WT_E_ARG, "unsupported storage/arith combination %d/%d", storage, arith);
-    if (layout != WT_LAYOUT_AB) return wt_fail(WT_E_ARG, "layout %d not available", layout);
+    if (layout != WT_LAYOUT_AB && layout != WT_LAYOUT_ESOTERIC) return wt_fail(WT_E_ARG, "unknown layout %d", layout);
     if (int rc = check_device()) return rc;
 
     wt_lattice *h = new wt_lattice();
@@ -126,11 +127,23 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
     h->mac_elem = storage == WT_STORE_F64 ? 8 : 4;
     h->nwx = (nx + 31) / 32;
     const size_t n = (size_t)h->n;
+    // Population planes are padded so that consecutive planes do not start
+    // at the same offset modulo large powers of two: with 2^k-sized planes
+    // all 38 streams of a step would hit the same HBM channel at once.
+    int64_t pad = 1088;
+    if (const char *e = std::getenv("WT_PLANE_PAD")) pad = std::atoll(e);
+    if (pad < 0) pad = 0;
+    h->ns = (h->n + 63) / 64 * 64 + pad;
+    const size_t ns = (size_t)h->ns;
     const int bx = wt::row_block(nx);
     const int64_t blocks = (int64_t)((nx + bx - 1) / bx) * ny * nz;
     int rc = 0;
-    if (!rc) rc = alloc(&h->d_buf[0], 19 * n * h->pop_elem);
-    if (!rc) rc = alloc(&h->d_buf[1], 19 * n * h->pop_elem);
+    if (!rc) rc = alloc(&h->d_buf[0], 19 * ns * h->pop_elem);
+    if (!rc && layout == WT_LAYOUT_AB) rc = alloc(&h->d_buf[1], 19 * ns * h->pop_elem);
+    // in place: slots no cell ever writes (outside-pointing slots of the
+    // boundary planes) are never read either; zero them for reproducibility
+    if (!rc && layout == WT_LAYOUT_ESOTERIC && hipMemset(h->d_buf[0], 0, 19 * ns * h->pop_elem) != hipSuccess)
+        rc = wt_fail(WT_E_DEVICE, "hipMemset failed");
     if (!rc) rc = alloc((void **)&h->d_solid, n);
     if (!rc) rc = alloc((void **)&h->d_sbits, sizeof(uint32_t) * h->nwx * (size_t)ny * nz);
     if (!rc) rc = alloc((void **)&h->d_nbits, sizeof(uint32_t) * h->nwx * (size_t)ny * nz);
@@ -330,17 +343,27 @@ static int run_steps(wt_lattice *h, int nsteps, int nslabs, double *forces)
         const int cnt = std::min(chunk, nsteps - s0);
         for (int s = 0; s < cnt; s++) {
             const bool macro = s0 + s == nsteps - 1;
+            if (h->timing) WT_HIP(hipEventRecord(h->tev[2 * s], h->stream));
             for (int w = 0; w < nslabs; w++) {
                 int bz0 = 0, bz1 = h->nz;
                 if (nslabs > 1) slab_range(h->nz, w, nslabs, &bz0, &bz1);
                 if (int rc = lbm_launch_step(h, bz0, bz1, macro, h->d_forces + (int64_t)s * per + 3 * w)) return rc;
             }
+            if (h->timing) WT_HIP(hipEventRecord(h->tev[2 * s + 1], h->stream));
             h->parity ^= 1;
             h->steps++;
         }
         std::vector<double> tmp((size_t)cnt * per);
         WT_HIP(hipMemcpyAsync(tmp.data(), h->d_forces, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, h->stream));
         WT_HIP(hipStreamSynchronize(h->stream));
+        if (h->timing) {
+            for (int s = 0; s < cnt; s++) {
+                float t = 0;
+                WT_HIP(hipEventElapsedTime(&t, h->tev[2 * s], h->tev[2 * s + 1]));
+                h->kernel_ms += t;
+                h->kernel_launches += nslabs;
+            }
+        }
         if (forces) std::memcpy(forces + (int64_t)s0 * per, tmp.data(), sizeof(double) * tmp.size());
         for (int k = 0; k < 3; k++) {
             double t = 0;
@@ -356,6 +379,27 @@ int wt_step(wt_lattice *h, int nsteps, double *forces) { return run_steps(h, nst
 int wt_step_slabs(wt_lattice *h, int nsteps, int nslabs, double *forces)
 {
     return run_steps(h, nsteps, nslabs, forces);
+}
+
+int wt_set_timing(wt_lattice *h, int on)
+{
+    WT_CHECK_H(h);
+    if (on && h->tev.empty()) {
+        h->tev.resize(2 * (size_t)h->forces_cap, nullptr);
+        for (hipEvent_t &e : h->tev) WT_HIP(hipEventCreate(&e));
+    }
+    h->timing = !!on;
+    h->kernel_ms = 0;
+    h->kernel_launches = 0;
+    return 0;
+}
+
+int wt_get_timing(const wt_lattice *h, double *kernel_ms, int64_t *launches)
+{
+    WT_CHECK_H(h);
+    if (kernel_ms) *kernel_ms = h->kernel_ms;
+    if (launches) *launches = h->kernel_launches;
+    return 0;
 }
 
 int wt_set_parity(wt_lattice *h, int parity)
@@ -392,14 +436,27 @@ int wt_get_pops(wt_lattice *h, int which, void *out)
     WT_CHECK_H(h);
     if (!out || (which != 0 && which != 1)) return wt_fail(WT_E_ARG, "bad population request");
     const size_t cnt = 19 * (size_t)h->n;
+    if (h->layout == WT_LAYOUT_ESOTERIC) {
+        if (which != 0) return wt_fail(WT_E_ARG, "an in-place lattice holds only the current populations");
+        const size_t eb = h->storage == WT_STORE_F64 ? 8 : 4;
+        void *d_out = nullptr;
+        if (int rc = alloc(&d_out, cnt * eb)) return rc;
+        int rc = lbm_get_pops(h, 0, d_out);
+        if (!rc && hipMemcpyAsync(out, d_out, cnt * eb, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+            rc = wt_fail(WT_E_DEVICE, "population download failed");
+        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "population read failed");
+        (void)hipFree(d_out);
+        return rc;
+    }
     const void *src = h->d_buf[h->parity ^ which];
+    const size_t row = (size_t)h->n * h->pop_elem, pitch = (size_t)h->ns * h->pop_elem;
     if (h->storage != WT_STORE_F16) {
-        WT_HIP(hipMemcpyAsync(out, src, cnt * h->pop_elem, hipMemcpyDeviceToHost, h->stream));
+        WT_HIP(hipMemcpy2DAsync(out, row, src, pitch, row, 19, hipMemcpyDeviceToHost, h->stream));
         WT_HIP(hipStreamSynchronize(h->stream));
         return 0;
     }
     std::vector<__half> tmp(cnt);
-    WT_HIP(hipMemcpyAsync(tmp.data(), src, cnt * 2, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipMemcpy2DAsync(tmp.data(), row, src, pitch, row, 19, hipMemcpyDeviceToHost, h->stream));
     WT_HIP(hipStreamSynchronize(h->stream));
     float *o = (float *)out;
     for (size_t k = 0; k < cnt; k++) o[k] = __half2float(tmp[k]) * wt::F16_INV_SCALE;

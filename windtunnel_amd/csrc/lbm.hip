@@ -11,6 +11,15 @@
 // write the zero-gradient outlet copy into x = nx-1 (js/lbm.js:504-511), so
 // no second pass and no barrier are needed.
 //
+// Two population layouts share the code:
+//  * AB: two buffers, f read and g written, the reference's own scheme
+//    (js/lbm.js:71-80); g[q][cell] holds the post-collision value at home.
+//  * esoteric pull: one buffer updated in place. In state `rep` the
+//    post-collision value g_q(m) sits in slot_of(q, m, rep); a step loads
+//    in representation phase and stores in 1-phase, and every cell reads
+//    and writes only slots no other cell touches in that step (see
+//    slot_of). This halves the population memory.
+//
 // Arithmetic type T is float (fast path) or double. With T = double and
 // float storage the expression tree is the reference's, evaluated with
 // contraction off, so the stored values are the reference's own.
@@ -44,16 +53,41 @@ template <> struct Pop<__half> {
     }
 };
 
+__host__ __device__ constexpr int64_t cell_off(int q, int nx, int ny)
+{
+    return EX[q] + (int64_t)nx * (EY[q] + (int64_t)ny * EZ[q]);
+}
+
+// Esoteric-pull slot of g_q(m) in representation rep. Pairs (i, i+1) with
+// i odd are opposite directions. rep 0: g_i(m) in slot i+1 of cell m+c_i,
+// g_{i+1}(m) in slot i of m. rep 1: g_i(m) in slot i of m+c_i, g_{i+1}(m)
+// in slot i+1 of m. The rest population stays in slot 0 of m.
+__device__ __forceinline__ int64_t slot_of(int q, int64_t m, int rep, int64_t ns, int nx, int ny)
+{
+    if (q == 0) return m;
+    if (q & 1) return (int64_t)(rep ? q : q + 1) * ns + m + cell_off(q, nx, ny);
+    return (int64_t)(rep ? q : q - 1) * ns + m;
+}
+
+// Is the slot of g_q at cell (x,y,z) inside the lattice? Only odd q place
+// the value at a neighbour.
+__device__ __forceinline__ bool slot_inside(int q, int x, int y, int z, int nx, int ny, int nz)
+{
+    if (!(q & 1)) return true;
+    const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
+    return X >= 0 && X < nx && Y >= 0 && Y < ny && Z >= 0 && Z < nz;
+}
+
 template <class S, class MAC> struct StepArgs {
-    const S *F;
-    S *G;
+    const S *F;  // AB: f (read); esoteric: the single buffer
+    S *G;        // AB: g (written); esoteric: the single buffer
     const uint8_t *solid;
     const uint32_t *sbits, *nbits;
     const float *uwx, *uwy, *uwz;
     MAC *rho, *ux, *uy, *uz;
     double *partial;
-    int64_t n, bmin, bmax;
-    int nx, ny, nz, nwx, z0, mov;
+    int64_t n, ns, bmin, bmax;  // ns: population plane stride (>= n)
+    int nx, ny, nz, nwx, z0, mov, phase;
     double tau, sq18cs2, lambda, u0;
     double eqv[Q];  // boundary equilibrium deviation at (rho=1, u0, 0, 0)
 };
@@ -135,39 +169,55 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT>
-__global__ void __launch_bounds__(1024) k_step_ab(StepArgs<S, MAC> a)
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO>
+__global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
 {
     using P = Pop<S>;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z + a.z0;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t n = a.n;
+    const int ph = a.phase, nph = a.phase ^ 1;
     double fx = 0, fy = 0, fz = 0;
+
+    // store of one value of cell m (AB: home slot in g; esoteric: slot in the new representation)
+    auto put = [&](int q, int64_t m, int cx, int cy, int cz, T v) {
+        if (ESO) {
+            if (slot_inside(q, cx, cy, cz, nx, ny, nz)) P::template st<T>(a.G, slot_of(q, m, nph, a.ns, nx, ny), v);
+        } else {
+            P::template st<T>(a.G, q * a.ns + m, v);
+        }
+    };
 
     if (x < nx - 1) {
         const int64_t i = x + (int64_t)nx * (y + (int64_t)ny * z);
         const bool out = x == nx - 2;  // also writes the outlet copy at x = nx-1
         const bool bnd = x == 0 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1;
         const bool sol = bit(a.sbits, a.nwx, ny, x, y, z);
+        const bool out_sol = out && bit(a.sbits, a.nwx, ny, x + 1, y, z);
         if (bnd || sol) {
             // Equilibrium planes (js/lbm.js:488-503): all of them write except
             // the floor where it is solid and no other plane covers the cell.
-            const bool eq = bnd && !(sol && y == 0 && x != 0 && z != 0 && z != nz - 1 && y != ny - 1);
+            // In place, a solid cell never writes: its slots are where its
+            // fluid neighbours read their bounce-back values.
+            const bool eq = bnd && (ESO ? !sol : !(sol && y == 0 && x != 0 && z != 0 && z != nz - 1 && y != ny - 1));
             if (eq) {
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
-                    P::template st<double>(a.G, q * n + i, a.eqv[q]);
-                    if (out) P::template st<double>(a.G, q * n + i + 1, a.eqv[q]);
+                    put(q, i, x, y, z, (T)a.eqv[q]);
+                    if (out && !(ESO && out_sol)) put(q, i + 1, x + 1, y, z, (T)a.eqv[q]);
                 }
                 if (MACRO) {
                     a.rho[i] = (MAC)1; a.ux[i] = (MAC)a.u0; a.uy[i] = 0; a.uz[i] = 0;
                     if (out) { a.rho[i + 1] = (MAC)1; a.ux[i + 1] = (MAC)a.u0; a.uy[i + 1] = 0; a.uz[i + 1] = 0; }
                 }
             } else if (out) {
-                // the outlet copies whatever a solid x = nx-2 cell holds
+                // the outlet copies whatever a solid x = nx-2 cell holds (AB
+                // only: in place, a solid cell holds no values of its own)
+                if (!ESO) {
 #pragma unroll
-                for (int q = 0; q < Q; q++) a.G[q * n + i + 1] = a.G[q * n + i];
+                    for (int q = 0; q < Q; q++) a.G[q * a.ns + i + 1] = a.G[q * a.ns + i];
+                }
                 if (MACRO) {
                     a.rho[i + 1] = a.rho[i]; a.ux[i + 1] = a.ux[i];
                     a.uy[i + 1] = a.uy[i]; a.uz[i + 1] = a.uz[i];
@@ -180,11 +230,11 @@ __global__ void __launch_bounds__(1024) k_step_ab(StepArgs<S, MAC> a)
                 // halfway bounce-back, moving wall, momentum exchange (js/lbm.js:334-368)
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
-                    const int64_t off = EX[q] + (int64_t)nx * (EY[q] + (int64_t)ny * EZ[q]);
-                    const int64_t j = i - off;
+                    const int64_t j = i - cell_off(q, nx, ny);
                     const int sj = a.solid[j];
                     if (sj) {
-                        const T fd = P::template ld<T>(a.F, OPP[q] * n + i);
+                        const int64_t k = ESO ? slot_of(OPP[q], i, ph, a.ns, nx, ny) : OPP[q] * a.ns + i;
+                        const T fd = P::template ld<T>(a.F, k);
                         T corr = 0;
                         if (a.mov) {
                             const T cu = (T)EX[q] * (T)a.uwx[j] + (T)EY[q] * (T)a.uwy[j] + (T)EZ[q] * (T)a.uwz[j];
@@ -199,22 +249,24 @@ __global__ void __launch_bounds__(1024) k_step_ab(StepArgs<S, MAC> a)
                             fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
                         }
                     } else {
-                        f[q] = P::template ld<T>(a.F, q * n + j);
+                        const int64_t k = ESO ? slot_of(q, j, ph, a.ns, nx, ny) : q * a.ns + j;
+                        f[q] = P::template ld<T>(a.F, k);
                     }
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
-                    const int64_t off = EX[q] + (int64_t)nx * (EY[q] + (int64_t)ny * EZ[q]);
-                    f[q] = P::template ld<T>(a.F, q * n + i - off);
+                    const int64_t j = i - cell_off(q, nx, ny);
+                    const int64_t k = ESO ? slot_of(q, j, ph, a.ns, nx, ny) : q * a.ns + j;
+                    f[q] = P::template ld<T>(a.F, k);
                 }
             }
             T g[Q], r, vx, vy, vz;
             collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
 #pragma unroll
             for (int q = 0; q < Q; q++) {
-                P::template st<T>(a.G, q * n + i, g[q]);
-                if (out) P::template st<T>(a.G, q * n + i + 1, g[q]);
+                put(q, i, x, y, z, g[q]);
+                if (out && !(ESO && out_sol)) put(q, i + 1, x + 1, y, z, g[q]);
             }
             if (MACRO) {
                 a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
@@ -262,13 +314,32 @@ __global__ void __launch_bounds__(256) k_force_sum(const double *partial, int64_
 // ------------------------------------------------------- reset / voxelize
 
 template <class S, class MAC> struct InitArgs {
-    S *f, *g;
+    S *f, *g;  // AB: f and g; esoteric: f = the single buffer, g unused
     MAC *rho, *ux, *uy, *uz;
     const uint8_t *solid, *ops;
-    int64_t n;
+    int64_t n, ns;
+    int nx, ny, nz, rep, eso;
     double u0;
     double eq_u0[Q], eq_rest[Q];
 };
+
+// Write one cell's populations as the current state (both AB buffers when
+// `both`, or the current esoteric representation).
+template <class S, class MAC>
+__device__ void write_state(const InitArgs<S, MAC> &a, int64_t i, const double *v, bool both)
+{
+    if (a.eso) {
+        const int x = (int)(i % a.nx), y = (int)((i / a.nx) % a.ny), z = (int)(i / ((int64_t)a.nx * a.ny));
+        for (int q = 0; q < Q; q++)
+            if (slot_inside(q, x, y, z, a.nx, a.ny, a.nz))
+                Pop<S>::template st<double>(a.f, slot_of(q, i, a.rep, a.ns, a.nx, a.ny), v[q]);
+        return;
+    }
+    for (int q = 0; q < Q; q++) {
+        Pop<S>::template st<double>(a.f, q * a.ns + i, v[q]);
+        if (both) Pop<S>::template st<double>(a.g, q * a.ns + i, v[q]);
+    }
+}
 
 // reset(): equilibrium at u0 (fluid) or at rest (solid) in both buffers
 // (js/lbm.js:164-174).
@@ -278,11 +349,7 @@ __global__ void __launch_bounds__(256) k_reset(InitArgs<S, MAC> a)
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const bool s = a.solid[i] != 0;
-    for (int q = 0; q < Q; q++) {
-        const double v = s ? a.eq_rest[q] : a.eq_u0[q];
-        Pop<S>::template st<double>(a.f, q * a.n + i, v);
-        Pop<S>::template st<double>(a.g, q * a.n + i, v);
-    }
+    write_state(a, i, s ? a.eq_rest : a.eq_u0, true);
     a.rho[i] = 1; a.ux[i] = s ? (MAC)0 : (MAC)a.u0; a.uy[i] = 0; a.uz[i] = 0;
 }
 
@@ -297,15 +364,29 @@ __global__ void __launch_bounds__(256) k_apply_ops(InitArgs<S, MAC> a)
     const int op = a.ops[i];
     if (!op) return;
     if (op & 1) {
-        for (int q = 0; q < Q; q++) {
-            Pop<S>::template st<double>(a.f, q * a.n + i, a.eq_u0[q]);
-            Pop<S>::template st<double>(a.g, q * a.n + i, a.eq_u0[q]);
-        }
+        write_state(a, i, a.eq_u0, true);
         a.rho[i] = 1; a.ux[i] = (MAC)a.u0; a.uy[i] = 0; a.uz[i] = 0;
     }
     if (op & 2) {
-        for (int q = 0; q < Q; q++) Pop<S>::template st<double>(a.f, q * a.n + i, a.eq_rest[q]);
+        write_state(a, i, a.eq_rest, false);
         a.rho[i] = 1; a.ux[i] = 0; a.uy[i] = 0; a.uz[i] = 0;
+    }
+}
+
+// Esoteric buffer -> the reference's f[q*n + cell] order. Values a cell
+// never stores (solid cells; out-of-lattice slots of boundary cells) read 0.
+template <class S, class OUT>
+__global__ void __launch_bounds__(256) k_eso_read(const S *B, const uint8_t *solid, OUT *out, int64_t n, int64_t ns,
+                                                  int nx, int ny, int nz, int rep)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
+    for (int q = 0; q < Q; q++) {
+        OUT v = 0;
+        if (!solid[i] && slot_inside(q, x, y, z, nx, ny, nz))
+            v = (OUT)Pop<S>::template ld<double>(B, slot_of(q, i, rep, ns, nx, ny));
+        out[q * n + i] = v;
     }
 }
 
@@ -328,8 +409,7 @@ __global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, uint32_t *s
         if (solid[i]) { sb |= 1u << b; continue; }
         if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2 || z < 1 || z > nz - 2) continue;
         for (int q = 1; q < Q; q++) {
-            const int64_t off = EX[q] + (int64_t)nx * (EY[q] + (int64_t)ny * EZ[q]);
-            if (solid[i - off]) { nb |= 1u << b; break; }
+            if (solid[i - cell_off(q, nx, ny)]) { nb |= 1u << b; break; }
         }
     }
     sbits[w] = sb;
@@ -385,11 +465,13 @@ __global__ void __launch_bounds__(256) k_grid_mask(GridArgs a)
 template <class S, class MAC> InitArgs<S, MAC> init_args(wt_lattice *h, const uint8_t *ops)
 {
     InitArgs<S, MAC> a{};
-    a.f = (S *)h->d_buf[h->parity];
-    a.g = (S *)h->d_buf[h->parity ^ 1];
+    const bool eso = h->layout == WT_LAYOUT_ESOTERIC;
+    a.f = (S *)h->d_buf[eso ? 0 : h->parity];
+    a.g = eso ? nullptr : (S *)h->d_buf[h->parity ^ 1];
     MAC *m = (MAC *)h->d_mac;
     a.rho = m; a.ux = m + h->n; a.uy = m + 2 * h->n; a.uz = m + 3 * h->n;
-    a.solid = h->d_solid; a.ops = ops; a.n = h->n; a.u0 = h->p.u0;
+    a.solid = h->d_solid; a.ops = ops; a.n = h->n; a.ns = h->ns; a.u0 = h->p.u0;
+    a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.rep = h->parity; a.eso = eso;
     eq_deviation(1, h->p.u0, 0, 0, a.eq_u0);
     eq_deviation(1, 0, 0, 0, a.eq_rest);
     return a;
@@ -413,18 +495,24 @@ template <class S, class MAC> int ops_t(wt_lattice *h, const uint8_t *d_ops)
     return 0;
 }
 
-template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT>
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO>
 int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
 {
     StepArgs<S, MAC> a{};
-    a.F = (const S *)h->d_buf[h->parity];
-    a.G = (S *)h->d_buf[h->parity ^ 1];
+    if (ESO) {
+        a.F = (const S *)h->d_buf[0];
+        a.G = (S *)h->d_buf[0];
+    } else {
+        a.F = (const S *)h->d_buf[h->parity];
+        a.G = (S *)h->d_buf[h->parity ^ 1];
+    }
+    a.phase = h->parity;
     a.solid = h->d_solid; a.sbits = h->d_sbits; a.nbits = h->d_nbits;
     a.uwx = h->d_uw; a.uwy = h->d_uw + h->n; a.uwz = h->d_uw + 2 * h->n;
     MAC *m = (MAC *)h->d_mac;
     a.rho = m; a.ux = m + h->n; a.uy = m + 2 * h->n; a.uz = m + 3 * h->n;
     a.partial = h->d_partial;
-    a.n = h->n; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx; a.z0 = bz0;
+    a.n = h->n; a.ns = h->ns; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx; a.z0 = bz0;
     a.mov = h->p.movingWalls;
     a.tau = h->p.tau; a.sq18cs2 = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
     a.u0 = h->p.u0;
@@ -441,7 +529,7 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     } else {
         a.bmin = 1; a.bmax = 0;
     }
-    hipLaunchKernelGGL((k_step_ab<S, T, MAC, MACRO, LES, TRT>), dim3(gx, h->ny, bz1 - bz0), dim3(bx), 0,
+    hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO>), dim3(gx, h->ny, bz1 - bz0), dim3(bx), 0,
                        h->stream, a);
     WT_HIP(hipGetLastError());
     if (d_force) {
@@ -451,15 +539,22 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     return 0;
 }
 
-template <class S, class T, class MAC> int step_dispatch(wt_lattice *h, int bz0, int bz1, bool macro, double *f)
+template <class S, class T, class MAC, bool ESO>
+int step_dispatch(wt_lattice *h, int bz0, int bz1, bool macro, double *f)
 {
     const bool les = h->p.les, trt = h->p.trt;
 #define WT_CASE(M, L, R) \
-    if (macro == M && les == L && trt == R) return step_t<S, T, MAC, M, L, R>(h, bz0, bz1, f);
+    if (macro == M && les == L && trt == R) return step_t<S, T, MAC, M, L, R, ESO>(h, bz0, bz1, f);
     WT_CASE(true, true, true) WT_CASE(true, true, false) WT_CASE(true, false, true) WT_CASE(true, false, false)
     WT_CASE(false, true, true) WT_CASE(false, true, false) WT_CASE(false, false, true) WT_CASE(false, false, false)
 #undef WT_CASE
     return wt_fail(WT_E_STATE, "unreachable step configuration");
+}
+
+template <class S, class T, class MAC> int step_layout(wt_lattice *h, int bz0, int bz1, bool macro, double *f)
+{
+    if (h->layout == WT_LAYOUT_ESOTERIC) return step_dispatch<S, T, MAC, true>(h, bz0, bz1, macro, f);
+    return step_dispatch<S, T, MAC, false>(h, bz0, bz1, macro, f);
 }
 
 }  // namespace
@@ -510,12 +605,30 @@ int lbm_rebuild_masks(wt_lattice *h)
 int lbm_launch_step(wt_lattice *h, int bz0, int bz1, bool macro, double *d_force)
 {
     if (h->storage == WT_STORE_F32 && h->arith == WT_ARITH_F32)
-        return step_dispatch<float, float, float>(h, bz0, bz1, macro, d_force);
+        return step_layout<float, float, float>(h, bz0, bz1, macro, d_force);
     if (h->storage == WT_STORE_F32 && h->arith == WT_ARITH_F64)
-        return step_dispatch<float, double, float>(h, bz0, bz1, macro, d_force);
+        return step_layout<float, double, float>(h, bz0, bz1, macro, d_force);
     if (h->storage == WT_STORE_F16)
-        return step_dispatch<__half, float, float>(h, bz0, bz1, macro, d_force);
-    return step_dispatch<double, double, double>(h, bz0, bz1, macro, d_force);
+        return step_layout<__half, float, float>(h, bz0, bz1, macro, d_force);
+    return step_layout<double, double, double>(h, bz0, bz1, macro, d_force);
+}
+
+int lbm_get_pops(wt_lattice *h, int which, void *d_out)
+{
+    if (h->layout != WT_LAYOUT_ESOTERIC) return wt_fail(WT_E_STATE, "not an esoteric lattice");
+    if (which != 0) return wt_fail(WT_E_ARG, "an in-place lattice holds only the current populations");
+    const unsigned nb = (unsigned)((h->n + 255) / 256);
+    if (h->storage == WT_STORE_F64)
+        hipLaunchKernelGGL((k_eso_read<double, double>), dim3(nb), dim3(256), 0, h->stream, (const double *)h->d_buf[0],
+                           h->d_solid, (double *)d_out, h->n, h->ns, h->nx, h->ny, h->nz, h->parity);
+    else if (h->storage == WT_STORE_F16)
+        hipLaunchKernelGGL((k_eso_read<__half, float>), dim3(nb), dim3(256), 0, h->stream, (const __half *)h->d_buf[0],
+                           h->d_solid, (float *)d_out, h->n, h->ns, h->nx, h->ny, h->nz, h->parity);
+    else
+        hipLaunchKernelGGL((k_eso_read<float, float>), dim3(nb), dim3(256), 0, h->stream, (const float *)h->d_buf[0],
+                           h->d_solid, (float *)d_out, h->n, h->ns, h->nx, h->ny, h->nz, h->parity);
+    WT_HIP(hipGetLastError());
+    return 0;
 }
 
 int lbm_grid_mask(wt_lattice *h, const uint8_t *d_grid, const wt_voxmeta *vm, const double M[9],

@@ -42,6 +42,7 @@ struct Body {
 struct wt_lattice {
     int nx = 0, ny = 0, nz = 0;
     int64_t n = 0;
+    int64_t ns = 0;     // population plane stride in elements (n plus padding)
     int storage = WT_STORE_F32, arith = WT_ARITH_F32, layout = WT_LAYOUT_AB;
     int pop_elem = 4;   // bytes per stored population
     int mac_elem = 4;   // bytes per macroscopic field value
@@ -58,6 +59,11 @@ struct wt_lattice {
     int forces_cap = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // per-launch timing of the step kernel (wt_set_timing)
+    int timing = 0;
+    std::vector<hipEvent_t> tev;  // 2 per step of a chunk
+    double kernel_ms = 0;
+    int64_t kernel_launches = 0;
 
     wt_params p{0.5035, 0.14, 0.25, 0.06, 1, 1, 0};
     int parity = 0;

@@ -76,6 +76,8 @@ SIGNATURES = {
     "wt_get_pops": (_I, [_P, _I, _P]),
     "wt_get_solid": (_I, [_P, _P, _P]),
     "wt_time_steps": (_I, [_P, _I, _P]),
+    "wt_set_timing": (_I, [_P, _I]),
+    "wt_get_timing": (_I, [_P, _P, _P]),
     "wt_voxel_bounds": (_I, [_P, _L, _P]),
     "wt_voxel_plan": (_I, [_P, _L, _I, _P]),
     "wt_voxel_build": (_I, [_P, _L, _I, _P, _P]),
@@ -335,6 +337,16 @@ class LBM:
         _check(lib().wt_time_steps(self._h, int(n), C.byref(ms)))
         self._fields = None
         return ms.value
+
+    def set_timing(self, on: bool):
+        _check(lib().wt_set_timing(self._h, int(bool(on))))
+
+    def timing(self):
+        """(total step-kernel milliseconds, launches) since timing was turned on."""
+        ms = C.c_double()
+        cnt = C.c_int64()
+        _check(lib().wt_get_timing(self._h, C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
 
     # ---- host accessors over the downloaded fields (js/lbm.js:515-545) ------
     def sample(self, x, y, z, out=None):
