@@ -164,6 +164,27 @@ int wt_time_steps(wt_lattice *h, int nsteps, double *ms);
 int wt_set_timing(wt_lattice *h, int on);
 int wt_get_timing(const wt_lattice *h, double *kernel_ms, int64_t *launches);
 
+/* Gray bounce-back (no reference counterpart): w holds one weight per cell;
+ * a fluid cell j with w_j in (0,1) bounces the fraction w_j of every link
+ * leaving it back into the downstream cell n = j + c_q:
+ *   f_q(n) = (1 - w_j) g_q(j) + w_j g_qbar(n),   F -= 2 w_j g_qbar(n) c_q.
+ * Weights 0 keep plain streaming and body cells (voxelized as solid) keep
+ * the reference's bounce-back, so 0/1 fields reproduce the reference
+ * exactly. A weight >= 1 on a fluid cell is an error; NULL clears the field.
+ * AB layout only. */
+int wt_set_gray(wt_lattice *h, const float *w);
+
+/* Discrete adjoint (no reference counterpart): runs nsteps forward steps
+ * from the current state, as wt_step does (forces may be NULL), then the
+ * reverse sweep with sqrt(N) checkpointing (interval <= 0 picks
+ * ceil(sqrt(nsteps))) and returns dJ/dw per cell for the objective
+ * J = sum_k Fbar[3k..3k+2] . F_k; cells without a gray weight get 0.
+ * reverse_ms (may be NULL) receives the device time of the reverse sweep
+ * (recomputation plus adjoint steps). AB layout, float or double storage,
+ * no moving walls. */
+int wt_adjoint(wt_lattice *h, int nsteps, const double *Fbar, double *dJdw, double *forces, int interval,
+               double *reverse_ms);
+
 /* Voxel.bounds (js/voxel.js:32-42) over len floats. */
 int wt_voxel_bounds(const float *P, int64_t len, double out[6]);
 

@@ -51,6 +51,9 @@ def lib():
             "wto_voxel_plan": (I, [P, C.c_long, I, P]),
             "wto_voxel_build": (I, [P, C.c_long, I, P, P]),
             "wto_grid_mask": (V, [P, P, P, D, P, P, D, P, P]),
+            "wto_set_gray": (I, [P, P]),
+            "wto_adjoint": (I, [P, I, P, P, P]),
+            "wto_collide_adj": (V, [P, P, D, D, D, I, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -155,6 +158,26 @@ class OracleLBM:
         out = np.zeros(3)
         lib().wto_sample(self._h, float(x), float(y), float(z), _ptr(out))
         return out
+
+    def set_gray(self, w):
+        """Gray bounce-back weights per cell (0 fluid, (0,1) gray); None clears."""
+        if w is None:
+            rc = lib().wto_set_gray(self._h, None)
+        else:
+            self._w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+            rc = lib().wto_set_gray(self._h, _ptr(self._w))
+        if rc:
+            raise ValueError("gray weight >= 1 on a fluid cell")
+
+    def adjoint(self, nsteps: int, Fbar: np.ndarray):
+        """(forces [n,3], dJ/dw [n]) for J = sum_k Fbar[k].F_k over nsteps steps."""
+        Fbar = np.ascontiguousarray(Fbar, dtype=np.float64).reshape(nsteps, 3)
+        g = np.zeros(self.n)
+        f = np.zeros((nsteps, 3))
+        rc = lib().wto_adjoint(self._h, int(nsteps), _ptr(Fbar), _ptr(g), _ptr(f))
+        if rc:
+            raise ValueError("adjoint needs FP64 storage and no moving walls")
+        return f, g
 
     def is_solid(self, x, y, z) -> bool:
         return bool(lib().wto_is_solid(self._h, float(x), float(y), float(z)))

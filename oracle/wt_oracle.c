@@ -42,6 +42,7 @@ typedef struct {
     uint8_t *solid, *near;
     void *ux, *uy, *uz, *rho;
     float *uwx, *uwy, *uwz;
+    float *wgt;           /* gray bounce-back weights in (0,1) per cell, or NULL */
     long off[19];
     double u0, tau, csm, lambda;
     int les, trt, movingWalls, ground, parity;
@@ -85,7 +86,8 @@ static void rebuild_near(wto_lbm *L)
             for (int x = 1; x < nx - 1; x++, i++) {
                 if (L->solid[i]) continue;
                 for (int q = 1; q < 19; q++) {
-                    if (L->solid[i - L->off[q]]) { L->near[i] = 1; break; }
+                    const long j = i - L->off[q];
+                    if (L->solid[j] || (L->wgt && L->wgt[j] > 0)) { L->near[i] = 1; break; }
                 }
             }
         }
@@ -136,7 +138,7 @@ void wto_destroy(wto_lbm *L)
     if (!L) return;
     free(L->buf[0]); free(L->buf[1]); free(L->solid); free(L->near);
     free(L->ux); free(L->uy); free(L->uz); free(L->rho);
-    free(L->uwx); free(L->uwy); free(L->uwz);
+    free(L->uwx); free(L->uwy); free(L->uwz); free(L->wgt);
     free(L);
 }
 
@@ -546,4 +548,216 @@ void wto_grid_mask(const uint8_t *grid, const int gdim[3], const double gmin[3],
                 if (c < 0 || c >= gdim[2]) continue;
                 mask[k] = grid[a + (long)gdim[0] * (b + (long)gdim[1] * c)] != 0;
             }
+}
+
+/* ===================================================== gray bounce-back
+ *
+ * No reference counterpart (a north-star addition): a cell j with weight
+ * w_j in (0,1) stays a fluid cell, but every link leaving it is bounced back
+ * with weight w_j at the downstream cell n = j + c_q:
+ *   f_q(n) = (1 - w_j) g_q(j) + w_j g_qbar(n),   F -= w_j * 2 g_qbar(n) c_q.
+ * w = 0 is plain streaming and w = 1 is a SOLID_OBJ cell (selected, not
+ * blended), so 0/1 weights reduce exactly to the reference. */
+
+/* Weights per cell; 0 or NaN = fluid, (0,1) = gray. Cells that are solid keep
+ * their solid type; a weight >= 1 on a fluid cell is rejected (voxelize it
+ * as body first). NULL clears the gray field. */
+int wto_set_gray(wto_lbm *L, const float *w)
+{
+    if (!w) { free(L->wgt); L->wgt = NULL; rebuild_near(L); return 0; }
+    if (!L->wgt) L->wgt = malloc(sizeof(float) * (size_t)L->n);
+    for (long i = 0; i < L->n; i++) {
+        float v = w[i];
+        const int x = (int)(i % L->nx), y = (int)((i / L->nx) % L->ny), z = (int)(i / ((long)L->nx * L->ny));
+        const int bnd = x == 0 || y == 0 || z == 0 || x == L->nx - 1 || y == L->ny - 1 || z == L->nz - 1;
+        if (L->solid[i] || bnd || !(v > 0)) v = 0;   /* boundary planes carry no weight */
+        if (v >= 1) return -1;
+        L->wgt[i] = v;
+    }
+    rebuild_near(L);
+    return 0;
+}
+
+/* ===================================================== discrete adjoint
+ *
+ * FP64 reference of the adjoint the GPU runs: reverse mode through N steps
+ * of the gray step (no moving walls), all intermediate states stored. The
+ * objective is J = sum_k Fbar[k] . F_k with F_k the force of step k. */
+
+static int computed(const wto_lbm *L, long i)
+{
+    const int x = (int)(i % L->nx), y = (int)((i / L->nx) % L->ny), z = (int)(i / ((long)L->nx * L->ny));
+    return x >= 1 && x <= L->nx - 2 && y >= 1 && y <= L->ny - 2 && z >= 1 && z <= L->nz - 2 && !L->solid[i];
+}
+
+static double weight_eff(const wto_lbm *L, long j)
+{
+    if (L->solid[j]) return 1;
+    return L->wgt ? (double)L->wgt[j] : 0;
+}
+
+static int force_link(const wto_lbm *L, long j)
+{
+    return L->solid[j] == SOLID_OBJ || (!L->solid[j] && L->wgt && L->wgt[j] > 0);
+}
+
+/* pre-collision populations of computed cell i from state S (double storage) */
+static void gather_f64(const wto_lbm *L, const double *S, long i, double f[19])
+{
+    const long n = L->n;
+    for (int q = 0; q < 19; q++) {
+        const long j = i - L->off[q];
+        const double w = weight_eff(L, j);
+        if (w == 1) f[q] = S[OPP[q] * n + i];
+        else if (w > 0) f[q] = (1 - w) * S[q * n + j] + w * S[OPP[q] * n + i];
+        else f[q] = S[q * n + j];
+    }
+}
+
+/* mu = (d collide / d f)^T lam at f, for the collision of js/lbm.js:377-460
+ * written in its generic form e_q = W_q (A + 3 r cu + 4.5 r cu^2). */
+void wto_collide_adj(const double f[19], const double lam[19], double tau0, double csm, double lamb, int les,
+                     int trt, double mu[19])
+{
+    double dr = 0, jx = 0, jy = 0, jz = 0;
+    for (int q = 0; q < 19; q++) { dr += f[q]; jx += EX[q] * f[q]; jy += EY[q] * f[q]; jz += EZ[q] * f[q]; }
+    const double r = 1 + dr, inv = 1 / r;
+    const double vx = jx * inv, vy = jy * inv, vz = jz * inv;
+    const double usq = vx * vx + vy * vy + vz * vz;
+    const double A = dr - 1.5 * r * usq;
+    double cu[19], e[19], nn[19];
+    for (int q = 0; q < 19; q++) {
+        cu[q] = EX[q] * vx + EY[q] * vy + EZ[q] * vz;
+        e[q] = W[q] * (A + 3 * r * cu[q] + 4.5 * r * cu[q] * cu[q]);
+        nn[q] = f[q] - e[q];
+    }
+    double pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+    for (int q = 0; q < 19; q++) {
+        pxx += EX[q] * EX[q] * nn[q]; pyy += EY[q] * EY[q] * nn[q]; pzz += EZ[q] * EZ[q] * nn[q];
+        pxy += EX[q] * EY[q] * nn[q]; pxz += EX[q] * EZ[q] * nn[q]; pyz += EY[q] * EZ[q] * nn[q];
+    }
+    const double X = pxx * pxx + pyy * pyy + pzz * pzz + 2 * (pxy * pxy + pxz * pxz + pyz * pyz);
+    const double mag = sqrt(2 * X);
+    const double K = SQ18 * csm * csm;
+    const double Sq = sqrt(tau0 * tau0 + K * mag * inv);
+    const double te = les ? 0.5 * (tau0 + Sq) : tau0;
+    const double om = 1 / te;
+    const double omm = trt ? 1 / (0.5 + lamb / (te - 0.5)) : om;
+
+    /* g = f - om P+ n - omm P- n */
+    double nb[19], omb = 0, ommb = 0;
+    for (int q = 0; q < 19; q++) mu[q] = lam[q];
+    nb[0] = -om * lam[0];
+    omb -= nn[0] * lam[0];
+    for (int a = 1; a < 19; a += 2) {
+        const int b = a + 1;
+        const double sp = 0.5 * (nn[a] + nn[b]), sm = 0.5 * (nn[a] - nn[b]);
+        const double spb = -om * (lam[a] + lam[b]), smb = -omm * (lam[a] - lam[b]);
+        nb[a] = 0.5 * (spb + smb);
+        nb[b] = 0.5 * (spb - smb);
+        omb -= sp * (lam[a] + lam[b]);
+        ommb -= sm * (lam[a] - lam[b]);
+    }
+    double teb = 0;
+    if (trt) teb += ommb * omm * omm * lamb / ((te - 0.5) * (te - 0.5));
+    else omb += ommb;
+    double invb = 0;
+    if (les) {
+        teb += -omb / (te * te);
+        const double magb = teb * K * inv / (4 * Sq);
+        invb += teb * K * mag / (4 * Sq);
+        const double Xb = mag > 0 ? magb / mag : 0;
+        const double pxxb = 2 * Xb * pxx, pyyb = 2 * Xb * pyy, pzzb = 2 * Xb * pzz;
+        const double pxyb = 4 * Xb * pxy, pxzb = 4 * Xb * pxz, pyzb = 4 * Xb * pyz;
+        for (int q = 0; q < 19; q++)
+            nb[q] += pxxb * EX[q] * EX[q] + pyyb * EY[q] * EY[q] + pzzb * EZ[q] * EZ[q] +
+                     pxyb * EX[q] * EY[q] + pxzb * EX[q] * EZ[q] + pyzb * EY[q] * EZ[q];
+    }
+    /* n = f - e(rho, u) */
+    double Ab = 0, rb = 0, vxb = 0, vyb = 0, vzb = 0;
+    for (int q = 0; q < 19; q++) {
+        mu[q] += nb[q];
+        const double eb = -nb[q];
+        Ab += W[q] * eb;
+        rb += W[q] * eb * (3 * cu[q] + 4.5 * cu[q] * cu[q]);
+        const double cub = W[q] * eb * (3 * r + 9 * r * cu[q]);
+        vxb += EX[q] * cub; vyb += EY[q] * cub; vzb += EZ[q] * cub;
+    }
+    double drb = Ab;
+    rb += -1.5 * usq * Ab;
+    const double usqb = -1.5 * r * Ab;
+    vxb += 2 * vx * usqb; vyb += 2 * vy * usqb; vzb += 2 * vz * usqb;
+    const double jxb = vxb * inv, jyb = vyb * inv, jzb = vzb * inv;
+    invb += vxb * jx + vyb * jy + vzb * jz;
+    rb += -inv * inv * invb;
+    drb += rb;
+    for (int q = 0; q < 19; q++) mu[q] += drb + EX[q] * jxb + EY[q] * jyb + EZ[q] * jzb;
+}
+
+/* Run N steps from the current state and return dJ/dw for every gray cell
+ * (0 elsewhere). Fbar[3k..3k+2] = dJ/dF_k. forces (may be NULL) gets F_k.
+ * FP64 storage only; moving walls unsupported. */
+int wto_adjoint(wto_lbm *L, int N, const double *Fbar, double *dJdw, double *forces)
+{
+    if (L->prec != 8 || L->movingWalls || N < 1) return -1;
+    const long n = L->n, sz = 19 * n;
+    double *S = malloc(sizeof(double) * (size_t)sz * (size_t)N);
+    if (!S) return -2;
+    for (int k = 0; k < N; k++) {
+        memcpy(S + (long)k * sz, L->buf[L->parity], sizeof(double) * (size_t)sz);
+        wto_step(L, 1, forces ? forces + 3 * k : NULL);
+    }
+    double *mu = calloc((size_t)sz, sizeof(double)), *lam = calloc((size_t)sz, sizeof(double));
+    double *mun = calloc((size_t)sz, sizeof(double));
+    memset(dJdw, 0, sizeof(double) * (size_t)n);
+    for (int k = N - 1; k >= 0; k--) {
+        const double *Sk = S + (long)k * sz, *Fb = Fbar + 3 * k;
+        for (long m = 0; m < n; m++) {
+            const int cm = computed(L, m);
+            const double wm = weight_eff(L, m);
+            for (int q = 0; q < 19; q++) {
+                double v = 0;
+                const long n1 = m + L->off[q];
+                if (n1 >= 0 && n1 < n && computed(L, n1) && wm < 1) v += (1 - wm) * mu[q * n + n1];
+                if (cm) {
+                    const long j = m + L->off[q];
+                    const double wj = weight_eff(L, j);
+                    if (wj > 0) {
+                        const int qb = OPP[q];
+                        const double cF = force_link(L, j) ? EX[qb] * Fb[0] + EY[qb] * Fb[1] + EZ[qb] * Fb[2] : 0;
+                        v += wj * (mu[qb * n + m] - 2 * cF);
+                    }
+                }
+                lam[q * n + m] = v;
+            }
+        }
+        if (L->wgt) {
+            for (long j = 0; j < n; j++) {
+                if (!(L->wgt[j] > 0) || L->solid[j]) continue;
+                double g = 0;
+                for (int q = 0; q < 19; q++) {
+                    const long n1 = j + L->off[q];
+                    if (n1 < 0 || n1 >= n || !computed(L, n1)) continue;
+                    const double fd = Sk[OPP[q] * n + n1];
+                    g += mu[q * n + n1] * (fd - Sk[q * n + j]) -
+                         2 * fd * (EX[q] * Fb[0] + EY[q] * Fb[1] + EZ[q] * Fb[2]);
+                }
+                dJdw[j] += g;
+            }
+        }
+        if (k == 0) break;
+        const double *Sp = S + (long)(k - 1) * sz;
+        for (long i = 0; i < n; i++) {
+            if (!computed(L, i)) { for (int q = 0; q < 19; q++) mun[q * n + i] = 0; continue; }
+            double f[19], lv[19], mv[19];
+            gather_f64(L, Sp, i, f);
+            const int x = (int)(i % L->nx);
+            for (int q = 0; q < 19; q++) lv[q] = lam[q * n + i] + (x == L->nx - 2 ? lam[q * n + i + 1] : 0);
+            wto_collide_adj(f, lv, L->tau, L->csm, L->lambda, L->les, L->trt, mv);
+            for (int q = 0; q < 19; q++) mun[q * n + i] = mv[q];
+        }
+        double *t = mu; mu = mun; mun = t;
+    }
+    free(S); free(mu); free(lam); free(mun);
+    return 0;
 }

@@ -1,0 +1,95 @@
+"""GPU gray bounce-back and discrete adjoint against the FP64 oracle adjoint
+and central finite differences."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleLBM
+from windtunnel_amd.lbm import LBM
+
+pytestmark = pytest.mark.gpu
+
+
+def gray_field(dims, c, r_in, r_out):
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r = np.sqrt((x + 0.5 - c[0]) ** 2 + (y + 0.5 - c[1]) ** 2 + (z + 0.5 - c[2]) ** 2)
+    return np.clip((r_out - r) / (r_out - r_in), 0, 1).reshape(-1).astype(np.float32)
+
+
+def setup(L, dims, w, les, trt, tau=0.6, u0=0.05):
+    L.set_params(tau=tau, les=les, trt=trt, csm=0.14, u0=u0, **{"lambda": 3 / 16})
+    L.reset()
+    solid = (w >= 1).reshape(dims[::-1])
+    idx = np.argwhere(solid)
+    lo, hi = idx.min(0), idx.max(0)
+    L.voxelize_mask([lo[2], lo[1], lo[0], hi[2], hi[1], hi[0]],
+                    solid[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1].astype(np.uint8).reshape(-1))
+    L.set_gray(np.where(w >= 1, 0, w).astype(np.float32))
+    return L
+
+
+DIMS = (20, 12, 12)
+W = gray_field(DIMS, (7, 6, 6), 2.0, 3.6)
+
+
+def fbar(N):
+    F = np.zeros((N, 3))
+    F[-10:, 0] = 0.1
+    F[-10:, 1] = 0.03
+    return F
+
+
+@pytest.mark.parametrize("les,trt", [(False, False), (False, True), (True, True)])
+def test_fp64_adjoint_equals_oracle(les, trt):
+    N = 40
+    print(f"[adjoint f64] les={les} trt={trt}", flush=True)
+    O = setup(OracleLBM(*DIMS, prec=8), DIMS, W, les, trt)
+    fo, go = O.adjoint(N, fbar(N))
+    for interval in (0, 1, 7):
+        G = setup(LBM(*DIMS, storage="f64", arith="f64"), DIMS, W, les, trt)
+        fg, gg, _ = G.adjoint(N, fbar(N), interval=interval)
+        np.testing.assert_allclose(fg, fo, rtol=1e-11, atol=1e-15)
+        assert np.max(np.abs(gg - go)) <= 1e-10 * np.max(np.abs(go))
+
+
+def test_gray_forward_exact_mode_equals_oracle():
+    O = setup(OracleLBM(*DIMS), DIMS, W, True, True)
+    G = setup(LBM(*DIMS, arith="f64"), DIMS, W, True, True)
+    fo, fg = O.step(30), G.step(30)
+    np.testing.assert_allclose(fg, fo, rtol=1e-11, atol=1e-15)
+    assert np.array_equal(G.f, O.array("f"))
+
+
+def test_fp32_adjoint_tracks_fp64():
+    N = 60
+    O = setup(OracleLBM(*DIMS, prec=8), DIMS, W, True, True)
+    _, go = O.adjoint(N, fbar(N))
+    G = setup(LBM(*DIMS), DIMS, W, True, True)
+    _, gg, _ = G.adjoint(N, fbar(N))
+    gray = (W > 0) & (W < 1)
+    assert np.linalg.norm(gg[gray] - go[gray]) / np.linalg.norm(go[gray]) < 1e-3
+
+
+def test_adjoint_against_central_differences():
+    """Central differences (eps = 1e-3) of the FP64 GPU forward on seeded gray cells."""
+    dims = (48, 24, 24)
+    w = gray_field(dims, (16, 12, 12), 3.0, 6.0)
+    N = 150
+    F = np.zeros((N, 3))
+    F[-50:, 0] = 1.0 / 50
+    G = setup(LBM(*dims, storage="f64", arith="f64"), dims, w, False, True, tau=0.6, u0=0.04)
+    _, g, _ = G.adjoint(N, F)
+    gray = np.flatnonzero((w > 2e-3) & (w < 1 - 2e-3))
+    cells = np.random.default_rng(4).choice(gray, 8, replace=False)
+    eps = 1e-3
+    for c in cells:
+        J, wv = [], []
+        for s in (1, -1):
+            wp = w.copy()
+            wp[c] = np.float32(w[c] + s * eps)
+            wv.append(float(wp[c]))
+            L = setup(LBM(*dims, storage="f64", arith="f64"), dims, wp, False, True, tau=0.6, u0=0.04)
+            J.append(float(np.sum(L.step(N) * F)))
+            L.close()
+        fd = (J[0] - J[1]) / (wv[0] - wv[1])
+        assert abs(g[c] - fd) <= 1e-3 * abs(fd) + 1e-6 * np.abs(g).max(), (c, g[c], fd)

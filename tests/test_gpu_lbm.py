@@ -137,14 +137,22 @@ def test_fp32_c1_force_and_fields():
     assert abs(cd_g - cd_o) / abs(cd_o) < 1e-4
 
 
-def test_uniform_flow_is_a_fixed_point():
-    """No body: the equilibrium at u0 is an exact fixed point of the step."""
+def test_uniform_flow_stays_uniform():
+    """No body: uniform inflow stays uniform. In the reference's own
+    arithmetic the fields are an exact fixed point while the stored
+    populations drift by rounding only (below 1e-9), and the binary64 path
+    reproduces that drift bit for bit."""
     L = LBM(40, 20, 18, arith="f64")
     L.set_params(tau=0.6, les=True, trt=True, u0=0.07)
     L.reset()
     f0 = L.f.copy()
     L.step(25)
-    assert np.array_equal(L.f, f0)
+    O = OracleLBM(40, 20, 18)
+    O.set_params(tau=0.6, les=True, trt=True, u0=0.07)
+    O.reset()
+    O.step(25)
+    assert np.array_equal(L.f, O.array("f"))
+    assert np.max(np.abs(L.f - f0)) < 1e-9
     assert np.all(L.ux == np.float32(0.07)) and np.all(L.rho == 1)
     L32 = LBM(40, 20, 18)
     L32.set_params(tau=0.6, les=True, trt=True, u0=0.07)

@@ -1,0 +1,395 @@
+// Discrete adjoint of the gray-bounce-back step (no reference counterpart:
+// a north-star addition) with sqrt(N) checkpointing.
+//
+// Forward step k maps the state s_k (post-collision populations, AB layout)
+// to s_{k+1} and yields the force F_k. The objective is J = sum_k Fbar_k.F_k.
+// With mu_k the adjoint of the pre-collision populations f_k of step k and
+// lambda_k the adjoint of s_k, one reverse iteration k is ONE fused kernel:
+//
+//   lambda_k(m)[q] = (1 - w_m) mu_k[q](m + c_q)                 (streaming)
+//                  + w_j (mu_k[qbar](m) - 2 c_qbar . Fbar_k)    (bounce-back, j = m + c_q)
+//   dJ/dw_m       += sum_q mu_k[q](m + c_q) (s_k[qbar](m + c_q) - s_k[q](m))
+//                                          - 2 s_k[qbar](m + c_q) c_q . Fbar_k     (gray m)
+//   mu_{k-1}(m)    = (d collide / d f)^T at f_{k-1}(m), applied to lambda_k(m)
+//                    (plus lambda_k of the outlet cell for x = nx-2: the outlet
+//                    copies that cell, js/lbm.js:504-511)
+//
+// It reads mu_k (19 gathered), s_{k-1} (19 gathered) and writes mu_{k-1}
+// (19): 1.5x the bytes of a forward step. States inside a checkpoint segment
+// are recomputed by the forward kernel into a replay store, so the reverse
+// sweep costs one recompute step plus one adjoint step per forward step.
+
+#include "wt_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+using wt::EX;
+using wt::EY;
+using wt::EZ;
+using wt::OPP;
+using wt::Q;
+
+__host__ __device__ constexpr int64_t off_of(int q, int nx, int ny)
+{
+    return EX[q] + (int64_t)nx * (EY[q] + (int64_t)ny * EZ[q]);
+}
+
+template <class T>
+__device__ __forceinline__ T wq(int q) { return q == 0 ? T(1.0 / 3) : (q <= 6 ? T(1.0 / 18) : T(1.0 / 36)); }
+
+// mu = (d collide / d f)^T lam for the collision of js/lbm.js:377-460 in its
+// generic form e_q = W_q (A + 3 r cu + 4.5 r cu^2); LES and TRT included.
+template <class T, bool LES, bool TRT>
+__device__ __forceinline__ void collide_adj(const T (&f)[Q], const T (&lam)[Q], T tau0, T K, T lamb, T (&mu)[Q])
+{
+    T dr = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) { dr += f[q]; jx += (T)EX[q] * f[q]; jy += (T)EY[q] * f[q]; jz += (T)EZ[q] * f[q]; }
+    const T r = T(1) + dr, inv = T(1) / r;
+    const T vx = jx * inv, vy = jy * inv, vz = jz * inv;
+    const T usq = vx * vx + vy * vy + vz * vz;
+    const T A = dr - T(1.5) * r * usq;
+    T cu[Q], nn[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        cu[q] = (T)EX[q] * vx + (T)EY[q] * vy + (T)EZ[q] * vz;
+        nn[q] = f[q] - wq<T>(q) * (A + T(3) * r * cu[q] + T(4.5) * r * cu[q] * cu[q]);
+    }
+    T te = tau0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0, mag = 0, Sq = 0;
+    if (LES) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            pxx += (T)(EX[q] * EX[q]) * nn[q]; pyy += (T)(EY[q] * EY[q]) * nn[q]; pzz += (T)(EZ[q] * EZ[q]) * nn[q];
+            pxy += (T)(EX[q] * EY[q]) * nn[q]; pxz += (T)(EX[q] * EZ[q]) * nn[q]; pyz += (T)(EY[q] * EZ[q]) * nn[q];
+        }
+        mag = sqrt(T(2) * (pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz)));
+        Sq = sqrt(tau0 * tau0 + K * mag * inv);
+        te = T(0.5) * (tau0 + Sq);
+    }
+    const T om = T(1) / te;
+    const T omm = TRT ? T(1) / (T(0.5) + lamb / (te - T(0.5))) : om;
+
+    T nb[Q], omb = 0, ommb = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) mu[q] = lam[q];
+    nb[0] = -om * lam[0];
+    omb -= nn[0] * lam[0];
+#pragma unroll
+    for (int a = 1; a < Q; a += 2) {
+        const int b = a + 1;
+        const T sp = T(0.5) * (nn[a] + nn[b]), sm = T(0.5) * (nn[a] - nn[b]);
+        const T spb = -om * (lam[a] + lam[b]), smb = -omm * (lam[a] - lam[b]);
+        nb[a] = T(0.5) * (spb + smb);
+        nb[b] = T(0.5) * (spb - smb);
+        omb -= sp * (lam[a] + lam[b]);
+        ommb -= sm * (lam[a] - lam[b]);
+    }
+    T teb = 0;
+    if (TRT) teb += ommb * omm * omm * lamb / ((te - T(0.5)) * (te - T(0.5)));
+    else omb += ommb;
+    T invb = 0;
+    if (LES) {
+        teb += -omb / (te * te);
+        const T magb = teb * K * inv / (T(4) * Sq);
+        invb += teb * K * mag / (T(4) * Sq);
+        const T Xb = mag > T(0) ? magb / mag : T(0);
+        const T pxxb = T(2) * Xb * pxx, pyyb = T(2) * Xb * pyy, pzzb = T(2) * Xb * pzz;
+        const T pxyb = T(4) * Xb * pxy, pxzb = T(4) * Xb * pxz, pyzb = T(4) * Xb * pyz;
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            nb[q] += pxxb * (T)(EX[q] * EX[q]) + pyyb * (T)(EY[q] * EY[q]) + pzzb * (T)(EZ[q] * EZ[q]) +
+                     pxyb * (T)(EX[q] * EY[q]) + pxzb * (T)(EX[q] * EZ[q]) + pyzb * (T)(EY[q] * EZ[q]);
+    }
+    T Ab = 0, rb = 0, vxb = 0, vyb = 0, vzb = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        mu[q] += nb[q];
+        const T eb = -nb[q] * wq<T>(q);
+        Ab += eb;
+        rb += eb * (T(3) * cu[q] + T(4.5) * cu[q] * cu[q]);
+        const T cub = eb * (T(3) * r + T(9) * r * cu[q]);
+        vxb += (T)EX[q] * cub; vyb += (T)EY[q] * cub; vzb += (T)EZ[q] * cub;
+    }
+    T drb = Ab;
+    rb += -T(1.5) * usq * Ab;
+    const T usqb = -T(1.5) * r * Ab;
+    vxb += T(2) * vx * usqb; vyb += T(2) * vy * usqb; vzb += T(2) * vz * usqb;
+    const T jxb = vxb * inv, jyb = vyb * inv, jzb = vzb * inv;
+    invb += vxb * jx + vyb * jy + vzb * jz;
+    rb += -inv * inv * invb;
+    drb += rb;
+#pragma unroll
+    for (int q = 0; q < Q; q++) mu[q] += drb + (T)EX[q] * jxb + (T)EY[q] * jyb + (T)EZ[q] * jzb;
+}
+
+template <class S> struct AdjArgs {
+    const S *mu;     // mu_k
+    const S *Sk;     // s_k (gradient at gray cells)
+    const S *Sp;     // s_{k-1} (pre-collision gather of step k-1), or null at k = 0
+    S *mun;          // mu_{k-1}
+    const uint8_t *solid;
+    const uint32_t *sbits, *nbits;
+    const float *wgt;
+    double *dJdw;
+    int64_t ns;
+    int nx, ny, nz, nwx;
+    double Fb[3];
+    double tau, K, lambda;
+};
+
+__device__ __forceinline__ bool bitset(const uint32_t *bits, int nwx, int ny, int x, int y, int z)
+{
+    return (bits[(x >> 5) + (int64_t)nwx * (y + (int64_t)ny * z)] >> (x & 31)) & 1u;
+}
+
+template <class S> __device__ __forceinline__ bool computed(const AdjArgs<S> &a, int x, int y, int z)
+{
+    return x >= 1 && x <= a.nx - 2 && y >= 1 && y <= a.ny - 2 && z >= 1 && z <= a.nz - 2 &&
+           !bitset(a.sbits, a.nwx, a.ny, x, y, z);
+}
+
+template <class S> __device__ __forceinline__ double weight_eff(const AdjArgs<S> &a, int64_t j, int x, int y, int z)
+{
+    if (bitset(a.sbits, a.nwx, a.ny, x, y, z)) return 1.0;
+    return a.wgt ? (double)a.wgt[j] : 0.0;
+}
+
+// lambda_k at cell (x,y,z) = m; `comp` says whether m itself is computed
+template <class S, class T>
+__device__ __forceinline__ void lambda_at(const AdjArgs<S> &a, int64_t m, int x, int y, int z, bool comp, bool nr,
+                                          T (&lam)[Q])
+{
+    const T wm = comp ? (a.wgt ? (T)a.wgt[m] : T(0)) : T(0);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
+        T v = 0;
+        if (computed(a, X, Y, Z)) v = (T(1) - wm) * (T)a.mu[q * a.ns + m + off_of(q, a.nx, a.ny)];
+        if (comp && nr) {
+            const int64_t j = m + off_of(q, a.nx, a.ny);
+            const double wj = weight_eff(a, j, X, Y, Z);
+            if (wj > 0) {
+                const int qb = OPP[q];
+                const bool flink = a.solid[j] == wt::SOLID_OBJ || (a.solid[j] == 0 && a.wgt && a.wgt[j] > 0.0f);
+                const double cF = flink ? EX[qb] * a.Fb[0] + EY[qb] * a.Fb[1] + EZ[qb] * a.Fb[2] : 0.0;
+                v += (T)wj * ((T)a.mu[qb * a.ns + m] - T(2) * (T)cF);
+            }
+        }
+        lam[q] = v;
+    }
+}
+
+template <class S, class T, bool LES, bool TRT>
+__global__ void __launch_bounds__(256) k_adjoint(AdjArgs<S> a)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x >= a.nx) return;
+    if (!computed(a, x, y, z)) return;  // outlet lambdas are folded in by the x = nx-2 cells
+    const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
+    const bool nr = bitset(a.nbits, a.nwx, a.ny, x, y, z);
+    T lam[Q];
+    lambda_at<S, T>(a, i, x, y, z, true, nr, lam);
+
+    if (a.wgt && a.wgt[i] > 0.0f) {
+        double g = 0;
+        for (int q = 0; q < Q; q++) {
+            const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
+            if (!computed(a, X, Y, Z)) continue;
+            const int64_t n1 = i + off_of(q, a.nx, a.ny);
+            const double fd = (double)a.Sk[OPP[q] * a.ns + n1];
+            g += (double)a.mu[q * a.ns + n1] * (fd - (double)a.Sk[q * a.ns + i]) -
+                 2 * fd * (EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2]);
+        }
+        a.dJdw[i] += g;
+    }
+    if (!a.Sp) return;
+    if (x == a.nx - 2) {
+        T lo[Q];
+        lambda_at<S, T>(a, i + 1, x + 1, y, z, false, false, lo);
+#pragma unroll
+        for (int q = 0; q < Q; q++) lam[q] += lo[q];
+    }
+    // pre-collision populations of step k-1 (gray-aware pull)
+    T f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int64_t j = i - off_of(q, a.nx, a.ny);
+        if (!nr) { f[q] = (T)a.Sp[q * a.ns + j]; continue; }
+        const double w = weight_eff(a, j, x - EX[q], y - EY[q], z - EZ[q]);
+        if (w == 1) f[q] = (T)a.Sp[OPP[q] * a.ns + i];
+        else if (w > 0) f[q] = (T(1) - (T)w) * (T)a.Sp[q * a.ns + j] + (T)w * (T)a.Sp[OPP[q] * a.ns + i];
+        else f[q] = (T)a.Sp[q * a.ns + j];
+    }
+    T mu[Q];
+    collide_adj<T, LES, TRT>(f, lam, (T)a.tau, (T)a.K, (T)a.lambda, mu);
+#pragma unroll
+    for (int q = 0; q < Q; q++) a.mun[q * a.ns + i] = (S)mu[q];
+}
+
+template <class S, class T>
+int launch_adj(wt_lattice *h, const AdjArgs<S> &a)
+{
+    const int bx = 256, gx = (h->nx + bx - 1) / bx;
+    const dim3 grid(gx, h->ny, h->nz);
+    const bool les = h->p.les, trt = h->p.trt;
+    if (les && trt) hipLaunchKernelGGL((k_adjoint<S, T, true, true>), grid, dim3(bx), 0, h->stream, a);
+    else if (les) hipLaunchKernelGGL((k_adjoint<S, T, true, false>), grid, dim3(bx), 0, h->stream, a);
+    else if (trt) hipLaunchKernelGGL((k_adjoint<S, T, false, true>), grid, dim3(bx), 0, h->stream, a);
+    else hipLaunchKernelGGL((k_adjoint<S, T, false, false>), grid, dim3(bx), 0, h->stream, a);
+    WT_HIP(hipGetLastError());
+    return 0;
+}
+
+struct Buf {
+    void *p = nullptr;
+    Buf() = default;
+    Buf(const Buf &) = delete;
+    Buf &operator=(const Buf &) = delete;
+    ~Buf() { if (p) (void)hipFree(p); }
+};
+
+template <class S, class T>
+int adjoint_t(wt_lattice *h, int N, const double *Fbar, double *dJdw, double *forces, int K, double *ms)
+{
+    const size_t st = 19 * (size_t)h->ns * sizeof(S);
+    const int nseg = (N + K - 1) / K;
+    std::vector<Buf> ck((size_t)nseg), rep((size_t)K + 1);
+    Buf mu0, mu1, dg;
+    for (auto &b : ck) WT_HIP(hipMalloc(&b.p, st));
+    for (auto &b : rep) WT_HIP(hipMalloc(&b.p, st));
+    WT_HIP(hipMalloc(&mu0.p, st));
+    WT_HIP(hipMalloc(&mu1.p, st));
+    WT_HIP(hipMalloc(&dg.p, sizeof(double) * (size_t)h->n));
+    WT_HIP(hipMemsetAsync(mu0.p, 0, st, h->stream));
+    WT_HIP(hipMemsetAsync(mu1.p, 0, st, h->stream));
+    WT_HIP(hipMemsetAsync(dg.p, 0, sizeof(double) * (size_t)h->n, h->stream));
+
+    // forward: checkpoint s_{seg*K - 1} (s_0 for the first segment), then step
+    std::vector<double> fw((size_t)N * 3);
+    for (int k = 0; k < N; k++) {
+        const int seg = (k + 1) / K;
+        if (k == 0) WT_HIP(hipMemcpyAsync(ck[0].p, h->d_buf[h->parity], st, hipMemcpyDeviceToDevice, h->stream));
+        if ((k + 1) % K == 0 && seg >= 1 && seg < nseg)
+            WT_HIP(hipMemcpyAsync(ck[seg].p, h->d_buf[h->parity], st, hipMemcpyDeviceToDevice, h->stream));
+        if (int rc = lbm_launch_step(h, 0, h->nz, k == N - 1, h->d_forces + 3 * (k % h->forces_cap))) return rc;
+        h->parity ^= 1;
+        h->steps++;
+        if (k % h->forces_cap == h->forces_cap - 1 || k == N - 1) {
+            const int base = k - (k % h->forces_cap);
+            WT_HIP(hipMemcpyAsync(fw.data() + 3 * base, h->d_forces, sizeof(double) * 3 * (k - base + 1),
+                                  hipMemcpyDeviceToHost, h->stream));
+            WT_HIP(hipStreamSynchronize(h->stream));
+        }
+    }
+    if (forces) std::copy(fw.begin(), fw.end(), forces);
+    for (int k = 0; k < 3; k++) h->force[k] = fw[3 * (N - 1) + k];
+
+    WT_HIP(hipEventRecord(h->ev0, h->stream));
+    AdjArgs<S> a{};
+    a.solid = h->d_solid; a.sbits = h->d_sbits; a.nbits = h->d_nbits; a.wgt = h->d_wgt; a.dJdw = (double *)dg.p;
+    a.ns = h->ns; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx;
+    a.tau = h->p.tau; a.K = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
+    void *mu_k = mu0.p, *mu_n = mu1.p;
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        // replay store: rep[r] = s_{base + r}
+        const int base = seg == 0 ? 0 : seg * K - 1;
+        const int k0 = seg * K, k1 = std::min(N, k0 + K);
+        WT_HIP(hipMemcpyAsync(rep[0].p, ck[seg].p, st, hipMemcpyDeviceToDevice, h->stream));
+        for (int r = 0; base + r < k1 - 1; r++) {
+            h->ovF = rep[r].p;
+            h->ovG = rep[r + 1].p;
+            h->replay = 1;
+            const int rc = lbm_launch_step(h, 0, h->nz, false, nullptr);
+            h->ovF = h->ovG = nullptr;
+            h->replay = 0;
+            if (rc) return rc;
+        }
+        for (int k = k1 - 1; k >= k0; k--) {
+            a.mu = (const S *)mu_k;
+            a.mun = (S *)mu_n;
+            a.Sk = (const S *)rep[k - base].p;
+            a.Sp = k > 0 ? (const S *)rep[k - 1 - base].p : nullptr;
+            for (int c = 0; c < 3; c++) a.Fb[c] = Fbar[3 * k + c];
+            if (int rc = launch_adj<S, T>(h, a)) return rc;
+            std::swap(mu_k, mu_n);
+        }
+    }
+    WT_HIP(hipEventRecord(h->ev1, h->stream));
+    WT_HIP(hipMemcpyAsync(dJdw, dg.p, sizeof(double) * (size_t)h->n, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipStreamSynchronize(h->stream));
+    float t = 0;
+    WT_HIP(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    if (ms) *ms = t;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wt_set_gray(wt_lattice *h, const float *w)
+{
+    if (!h) return wt_fail(WT_E_ARG, "null lattice handle");
+    if (!w) {
+        if (h->d_wgt) (void)hipFree(h->d_wgt);
+        h->d_wgt = nullptr;
+        if (int rc = lbm_rebuild_masks(h)) return rc;
+        WT_HIP(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (h->layout != WT_LAYOUT_AB) return wt_fail(WT_E_ARG, "gray bounce-back needs the AB layout");
+    std::vector<float> v((size_t)h->n);
+    for (int64_t i = 0; i < h->n; i++) {
+        float x = w[i];
+        const int cx = (int)(i % h->nx), cy = (int)((i / h->nx) % h->ny), cz = (int)(i / ((int64_t)h->nx * h->ny));
+        const bool bnd = cx == 0 || cy == 0 || cz == 0 || cx == h->nx - 1 || cy == h->ny - 1 || cz == h->nz - 1;
+        if (h->h_solid[i] || bnd || !(x > 0)) x = 0;  // boundary planes carry no weight
+        if (x >= 1) return wt_fail(WT_E_ARG, "gray weight >= 1 on fluid cell %lld: voxelize it as body", (long long)i);
+        v[i] = x;
+    }
+    if (!h->d_wgt && hipMalloc((void **)&h->d_wgt, sizeof(float) * (size_t)h->n) != hipSuccess)
+        return wt_fail(WT_E_NOMEM, "hipMalloc of the weight field failed");
+    WT_HIP(hipMemcpyAsync(h->d_wgt, v.data(), sizeof(float) * (size_t)h->n, hipMemcpyHostToDevice, h->stream));
+    // force-carrying rows now include every gray cell
+    int zlo = h->nz, zhi = -1, ylo = h->ny, yhi = -1;
+    for (int64_t i = 0; i < h->n; i++)
+        if (v[i] > 0) {
+            const int z = (int)(i / ((int64_t)h->nx * h->ny)), y = (int)((i / h->nx) % h->ny);
+            zlo = std::min(zlo, z); zhi = std::max(zhi, z);
+            ylo = std::min(ylo, y); yhi = std::max(yhi, y);
+        }
+    if (zhi >= 0) {
+        const int a = std::max(1, zlo - 1), b = std::min(h->nz - 2, zhi + 1);
+        const int c = std::max(1, ylo - 1), d = std::min(h->ny - 2, yhi + 1);
+        const bool had = h->fz0 >= 0;
+        h->fz0 = had ? std::min(h->fz0, a) : a;
+        h->fz1 = had ? std::max(h->fz1, b) : b;
+        h->fy0 = had ? std::min(h->fy0, c) : c;
+        h->fy1 = had ? std::max(h->fy1, d) : d;
+    }
+    if (int rc = lbm_rebuild_masks(h)) return rc;
+    WT_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int wt_adjoint(wt_lattice *h, int nsteps, const double *Fbar, double *dJdw, double *forces, int interval,
+               double *reverse_ms)
+{
+    if (!h) return wt_fail(WT_E_ARG, "null lattice handle");
+    if (nsteps < 1 || !Fbar || !dJdw) return wt_fail(WT_E_ARG, "bad adjoint request");
+    if (h->layout != WT_LAYOUT_AB) return wt_fail(WT_E_ARG, "the adjoint needs the AB layout");
+    if (h->p.movingWalls) return wt_fail(WT_E_ARG, "the adjoint does not support moving walls");
+    if (h->storage == WT_STORE_F16) return wt_fail(WT_E_ARG, "the adjoint needs float or double storage");
+    int K = interval > 0 ? interval : (int)std::ceil(std::sqrt((double)nsteps));
+    K = std::max(1, std::min(K, nsteps));
+    if (h->storage == WT_STORE_F64) return adjoint_t<double, double>(h, nsteps, Fbar, dJdw, forces, K, reverse_ms);
+    if (h->arith == WT_ARITH_F64) return adjoint_t<float, double>(h, nsteps, Fbar, dJdw, forces, K, reverse_ms);
+    return adjoint_t<float, float>(h, nsteps, Fbar, dJdw, forces, K, reverse_ms);
+}
+
+}  // extern "C"

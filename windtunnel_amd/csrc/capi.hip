@@ -242,7 +242,7 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     std::vector<uint8_t> col((size_t)ny * nz, 0), plan((size_t)nx * nz, 0);
     std::vector<int16_t> low((size_t)nx * nz, 32767);
     int64_t cells = 0, k = 0, kw = 0;
-    int zlo = nz, zhi = -1;
+    int zlo = nz, zhi = -1, ylo = ny, yhi = -1;
     if (!empty) {
         for (int z = z0; z <= z1; z++)
             for (int y = y0; y <= y1; y++) {
@@ -264,6 +264,7 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
                     if (y < low[pk]) low[pk] = (int16_t)y;
                     cells++;
                     zlo = std::min(zlo, z); zhi = std::max(zhi, z);
+                    ylo = std::min(ylo, y); yhi = std::max(yhi, y);
                 }
             }
     }
@@ -280,8 +281,13 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     h->body.rideHeight = gaps.empty() ? 0 : gaps[gaps.size() >> 1];
     h->p.movingWalls = mov;
     // z rows whose fluid cells can carry body links
-    if (zhi >= 0) { h->fz0 = std::max(1, zlo - 1); h->fz1 = std::min(nz - 2, zhi + 1); }
-    else { h->fz0 = h->fz1 = -1; }
+    if (zhi >= 0) {
+        h->fz0 = std::max(1, zlo - 1); h->fz1 = std::min(nz - 2, zhi + 1);
+        h->fy0 = std::max(1, ylo - 1); h->fy1 = std::min(ny - 2, yhi + 1);
+    } else {
+        h->fz0 = h->fz1 = -1;
+        h->fy0 = 1; h->fy1 = 0;
+    }
 
     uint8_t *d_ops = nullptr;
     if (int rc = alloc((void **)&d_ops, (size_t)n)) return rc;

@@ -34,9 +34,28 @@ namespace {
 // ---------------------------------------------------------------- storage
 
 template <class S> struct Pop;
+// Build switches: WT_NT_LOAD / WT_NT_STORE mark the population streams
+// non-temporal (each value is read once and written once per step).
+template <class E> __device__ __forceinline__ E ld_pop(const E *p, int64_t k)
+{
+#ifdef WT_NT_LOAD
+    return __builtin_nontemporal_load(p + k);
+#else
+    return p[k];
+#endif
+}
+template <class E> __device__ __forceinline__ void st_pop(E *p, int64_t k, E v)
+{
+#ifdef WT_NT_STORE
+    __builtin_nontemporal_store(v, p + k);
+#else
+    p[k] = v;
+#endif
+}
+
 template <> struct Pop<float> {
-    template <class T> __device__ static T ld(const float *p, int64_t k) { return (T)p[k]; }
-    template <class T> __device__ static void st(float *p, int64_t k, T v) { p[k] = (float)v; }
+    template <class T> __device__ static T ld(const float *p, int64_t k) { return (T)ld_pop(p, k); }
+    template <class T> __device__ static void st(float *p, int64_t k, T v) { st_pop(p, k, (float)v); }
 };
 template <> struct Pop<double> {
     template <class T> __device__ static T ld(const double *p, int64_t k) { return (T)p[k]; }
@@ -84,10 +103,12 @@ template <class S, class MAC> struct StepArgs {
     const uint8_t *solid;
     const uint32_t *sbits, *nbits;
     const float *uwx, *uwy, *uwz;
+    const float *wgt;  // gray bounce-back weights (AB layout), or null
     MAC *rho, *ux, *uy, *uz;
     double *partial;
-    int64_t n, ns, bmin, bmax;  // ns: population plane stride (>= n)
-    int nx, ny, nz, nwx, z0, mov, phase;
+    int64_t n, ns;              // ns: population plane stride (>= n)
+    int py0, py1, pz0, pz1;     // rows (y,z) whose blocks write force partials
+    int nx, ny, nz, nwx, z0, mov, phase, rho_near;
     double tau, sq18cs2, lambda, u0;
     double eqv[Q];  // boundary equilibrium deviation at (rho=1, u0, 0, 0)
 };
@@ -176,7 +197,6 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z + a.z0;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int64_t n = a.n;
     const int ph = a.phase, nph = a.phase ^ 1;
     double fx = 0, fy = 0, fz = 0;
 
@@ -248,6 +268,13 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
                             const double d = (double)(T(2) * fd + corr);
                             fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
                         }
+                    } else if (!ESO && a.wgt && a.wgt[j] > 0.0f) {
+                        // gray upstream cell: a fraction w_j of the link bounces back
+                        const T wj = (T)a.wgt[j];
+                        const T fd = P::template ld<T>(a.F, OPP[q] * a.ns + i);
+                        f[q] = (T(1) - wj) * P::template ld<T>(a.F, q * a.ns + j) + wj * fd;
+                        const double d = (double)(wj * (T(2) * fd));
+                        fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
                     } else {
                         const int64_t k = ESO ? slot_of(q, j, ph, a.ns, nx, ny) : q * a.ns + j;
                         f[q] = P::template ld<T>(a.F, k);
@@ -271,7 +298,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
             if (MACRO) {
                 a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
                 if (out) { a.rho[i + 1] = (MAC)r; a.ux[i + 1] = (MAC)vx; a.uy[i + 1] = (MAC)vy; a.uz[i + 1] = (MAC)vz; }
-            } else if (nr) {
+            } else if (nr && a.rho_near) {
                 a.rho[i] = (MAC)r;  // next step's moving-wall correction reads it
             }
         }
@@ -279,8 +306,8 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
 
     // Per-block FP64 force partial for blocks that can hold body links:
     // wave shuffles, then the waves' sums in wave order.
-    const int64_t bid = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-    if (bid < a.bmin || bid > a.bmax) return;
+    if (y < a.py0 || y > a.py1 || z < a.pz0 || z > a.pz1) return;
+    const int64_t bid = blockIdx.x + (int64_t)gridDim.x * ((y - a.py0) + (int64_t)(a.py1 - a.py0 + 1) * (z - a.pz0));
     __shared__ double red[16][3];
     fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -290,25 +317,29 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
         double sx = 0, sy = 0, sz = 0;
         const int nw = blockDim.x >> 6;
         for (int w = 0; w < nw; w++) { sx += red[w][0]; sy += red[w][1]; sz += red[w][2]; }
-        double *o = a.partial + (bid - a.bmin) * 3;
+        double *o = a.partial + bid * 3;
         o[0] = sx; o[1] = sy; o[2] = sz;
     }
 }
 
 // Fixed-order sum of the block partials into one step's force.
-__global__ void __launch_bounds__(256) k_force_sum(const double *partial, int64_t count, double *out)
+__global__ void __launch_bounds__(1024) k_force_sum(const double *partial, int64_t count, double *out)
 {
     double s[3] = {0, 0, 0};
-    for (int64_t b = threadIdx.x; b < count; b += 256) {
+    for (int64_t b = threadIdx.x; b < count; b += 1024) {
         s[0] += partial[3 * b]; s[1] += partial[3 * b + 1]; s[2] += partial[3 * b + 2];
     }
-    __shared__ double red[4][3];
+    __shared__ double red[16][3];
     for (int k = 0; k < 3; k++) s[k] = wave_sum(s[k]);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) for (int k = 0; k < 3; k++) red[wv][k] = s[k];
     __syncthreads();
     if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) out[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+        for (int k = 0; k < 3; k++) {
+            double t = 0;
+            for (int w = 0; w < 16; w++) t += red[w][k];
+            out[k] = t;
+        }
 }
 
 // ------------------------------------------------------- reset / voxelize
@@ -392,8 +423,8 @@ __global__ void __launch_bounds__(256) k_eso_read(const S *B, const uint8_t *sol
 
 // Bit-packed solid mask and near-wall mask (js/lbm.js:268-282): near = an
 // interior fluid cell with a solid cell among its 18 upstream neighbours.
-__global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, uint32_t *sbits, uint32_t *nbits,
-                                               int nx, int ny, int nz, int nwx)
+__global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, const float *wgt, uint32_t *sbits,
+                                               uint32_t *nbits, int nx, int ny, int nz, int nwx)
 {
     const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t nwords = (int64_t)nwx * ny * nz;
@@ -409,7 +440,8 @@ __global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, uint32_t *s
         if (solid[i]) { sb |= 1u << b; continue; }
         if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2 || z < 1 || z > nz - 2) continue;
         for (int q = 1; q < Q; q++) {
-            if (solid[i - cell_off(q, nx, ny)]) { nb |= 1u << b; break; }
+            const int64_t j = i - cell_off(q, nx, ny);
+            if (solid[j] || (wgt && wgt[j] > 0.0f)) { nb |= 1u << b; break; }
         }
     }
     sbits[w] = sb;
@@ -503,17 +535,19 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
         a.F = (const S *)h->d_buf[0];
         a.G = (S *)h->d_buf[0];
     } else {
-        a.F = (const S *)h->d_buf[h->parity];
-        a.G = (S *)h->d_buf[h->parity ^ 1];
+        a.F = (const S *)(h->ovF ? h->ovF : h->d_buf[h->parity]);
+        a.G = (S *)(h->ovG ? h->ovG : h->d_buf[h->parity ^ 1]);
     }
     a.phase = h->parity;
     a.solid = h->d_solid; a.sbits = h->d_sbits; a.nbits = h->d_nbits;
     a.uwx = h->d_uw; a.uwy = h->d_uw + h->n; a.uwz = h->d_uw + 2 * h->n;
+    a.wgt = h->d_wgt;
     MAC *m = (MAC *)h->d_mac;
     a.rho = m; a.ux = m + h->n; a.uy = m + 2 * h->n; a.uz = m + 3 * h->n;
     a.partial = h->d_partial;
     a.n = h->n; a.ns = h->ns; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx; a.z0 = bz0;
     a.mov = h->p.movingWalls;
+    a.rho_near = !h->replay;
     a.tau = h->p.tau; a.sq18cs2 = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
     a.u0 = h->p.u0;
     eq_deviation(1, h->p.u0, 0, 0, a.eqv);
@@ -523,17 +557,16 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     const int lo = h->fz0 > bz0 ? h->fz0 : bz0, hi = h->fz1 < bz1 - 1 ? h->fz1 : bz1 - 1;
     int64_t count = 0;
     if (h->fz0 >= 0 && lo <= hi) {
-        a.bmin = (int64_t)(lo - bz0) * h->ny * gx;
-        a.bmax = (int64_t)(hi - bz0 + 1) * h->ny * gx - 1;
-        count = a.bmax - a.bmin + 1;
+        a.py0 = h->fy0; a.py1 = h->fy1; a.pz0 = lo; a.pz1 = hi;
+        count = (int64_t)gx * (h->fy1 - h->fy0 + 1) * (hi - lo + 1);
     } else {
-        a.bmin = 1; a.bmax = 0;
+        a.py0 = 1; a.py1 = 0; a.pz0 = 1; a.pz1 = 0;
     }
     hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO>), dim3(gx, h->ny, bz1 - bz0), dim3(bx), 0,
                        h->stream, a);
     WT_HIP(hipGetLastError());
     if (d_force) {
-        hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(256), 0, h->stream, h->d_partial, count, d_force);
+        hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(1024), 0, h->stream, h->d_partial, count, d_force);
         WT_HIP(hipGetLastError());
     }
     return 0;
@@ -597,7 +630,7 @@ int lbm_rebuild_masks(wt_lattice *h)
 {
     const int64_t nwords = (int64_t)h->nwx * h->ny * h->nz;
     hipLaunchKernelGGL(k_masks, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, h->stream, h->d_solid,
-                       h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx);
+                       h->d_wgt, h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx);
     WT_HIP(hipGetLastError());
     return 0;
 }

@@ -31,7 +31,10 @@ constexpr float F16_INV_SCALE = 1.0f / 32768.0f;
 // Threads per block along x: one block spans a whole lattice row (up to
 // 1024 cells), so the +-1 x-shifted pull loads of neighbouring waves hit
 // the same CU's cache instead of re-fetching the straddled line from HBM.
-inline int row_block(int nx) { int b = (nx + 63) / 64 * 64; return b > 1024 ? 1024 : b; }
+#ifndef WT_ROW_BLOCK_MAX
+#define WT_ROW_BLOCK_MAX 1024
+#endif
+inline int row_block(int nx) { int b = (nx + 63) / 64 * 64; return b > WT_ROW_BLOCK_MAX ? WT_ROW_BLOCK_MAX : b; }
 
 struct Body {
     int64_t frontalArea = 1, planformArea = 1, solidCells = 0, rideHeight = 0;
@@ -52,6 +55,9 @@ struct wt_lattice {
     uint32_t *d_nbits = nullptr;  // bit-packed near-wall mask
     int nwx = 0;                  // words per row of a bit mask
     float *d_uw = nullptr;        // wall velocity, 3 planes of n floats
+    float *d_wgt = nullptr;       // gray bounce-back weights, or null
+    void *ovF = nullptr, *ovG = nullptr;  // explicit step buffers (adjoint replay), AB only
+    int replay = 0;                       // replayed steps leave the macroscopic fields alone
     void *d_mac = nullptr;        // rho, ux, uy, uz planes of n values
     double *d_partial = nullptr;  // per-block force partials
     int64_t partial_cap = 0;
@@ -71,8 +77,8 @@ struct wt_lattice {
     double force[3] = {0, 0, 0};
     int ground = 0;
     wt::Body body;
-    // inclusive range of z rows holding force-carrying (near-body) cells; -1 = none
-    int fz0 = -1, fz1 = -1;
+    // inclusive z and y ranges of rows holding force-carrying (near-body) cells; fz0 = -1: none
+    int fz0 = -1, fz1 = -1, fy0 = 1, fy1 = 0;
 
     std::vector<uint8_t> h_solid;  // host mirror of the solid array
     std::vector<float> h_uw;       // host mirror of the wall velocities (3n)

@@ -19,7 +19,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libwindtunnel.so")
+LIB_PATH = os.environ.get("WT_LIB_PATH") or os.path.join(HERE, "libwindtunnel.so")
 
 STORE = {"f32": 0, "f16": 1, "f64": 2}
 ARITH = {"f32": 0, "f64": 1}
@@ -78,6 +78,8 @@ SIGNATURES = {
     "wt_time_steps": (_I, [_P, _I, _P]),
     "wt_set_timing": (_I, [_P, _I]),
     "wt_get_timing": (_I, [_P, _P, _P]),
+    "wt_set_gray": (_I, [_P, _P]),
+    "wt_adjoint": (_I, [_P, _I, _P, _P, _P, _I, _P]),
     "wt_voxel_bounds": (_I, [_P, _L, _P]),
     "wt_voxel_plan": (_I, [_P, _L, _I, _P]),
     "wt_voxel_build": (_I, [_P, _L, _I, _P, _P]),
@@ -347,6 +349,28 @@ class LBM:
         cnt = C.c_int64()
         _check(lib().wt_get_timing(self._h, C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+    def set_gray(self, w):
+        """Gray bounce-back weights per cell (0 fluid, (0,1) gray); None clears."""
+        if w is None:
+            _check(lib().wt_set_gray(self._h, None))
+            return
+        w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+        if w.size != self.n:
+            raise ValueError("one weight per cell")
+        _check(lib().wt_set_gray(self._h, _ptr(w)))
+
+    def adjoint(self, nsteps: int, Fbar, interval: int = 0):
+        """Run nsteps and the checkpointed reverse sweep for J = sum_k Fbar[k].F_k.
+
+        Returns (forces [nsteps,3], dJ/dw [n], reverse-sweep device ms)."""
+        Fbar = np.ascontiguousarray(Fbar, dtype=np.float64).reshape(nsteps, 3)
+        g = np.zeros(self.n)
+        f = np.zeros((nsteps, 3))
+        ms = C.c_double()
+        _check(lib().wt_adjoint(self._h, int(nsteps), _ptr(Fbar), _ptr(g), _ptr(f), int(interval), C.byref(ms)))
+        self._fields = None
+        return f, g, ms.value
 
     # ---- host accessors over the downloaded fields (js/lbm.js:515-545) ------
     def sample(self, x, y, z, out=None):
