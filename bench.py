@@ -124,7 +124,18 @@ def main():
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 512 MiB device copy three times (HBM counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        src = torch.ones(512 * 1024 * 1024 // 4, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("copied 3 x 512 MiB", flush=True)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only; the multi-GPU path (z-slab halo exchange) is not built",
               file=sys.stderr)

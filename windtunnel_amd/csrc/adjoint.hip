@@ -32,6 +32,7 @@ using wt::EY;
 using wt::EZ;
 using wt::OPP;
 using wt::Q;
+using wt::pix;
 
 __host__ __device__ constexpr int64_t off_of(int q, int nx, int ny)
 {
@@ -136,6 +137,7 @@ template <class S> struct AdjArgs {
     const float *wgt;
     double *dJdw;
     int64_t ns;
+    wt::Lay lay;
     int nx, ny, nz, nwx;
     double Fb[3];
     double tau, K, lambda;
@@ -168,7 +170,7 @@ __device__ __forceinline__ void lambda_at(const AdjArgs<S> &a, int64_t m, int x,
     for (int q = 0; q < Q; q++) {
         const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
         T v = 0;
-        if (computed(a, X, Y, Z)) v = (T(1) - wm) * (T)a.mu[q * a.ns + m + off_of(q, a.nx, a.ny)];
+        if (computed(a, X, Y, Z)) v = (T(1) - wm) * (T)a.mu[pix(q, m + off_of(q, a.nx, a.ny), a.lay)];
         if (comp && nr) {
             const int64_t j = m + off_of(q, a.nx, a.ny);
             const double wj = weight_eff(a, j, X, Y, Z);
@@ -176,7 +178,7 @@ __device__ __forceinline__ void lambda_at(const AdjArgs<S> &a, int64_t m, int x,
                 const int qb = OPP[q];
                 const bool flink = a.solid[j] == wt::SOLID_OBJ || (a.solid[j] == 0 && a.wgt && a.wgt[j] > 0.0f);
                 const double cF = flink ? EX[qb] * a.Fb[0] + EY[qb] * a.Fb[1] + EZ[qb] * a.Fb[2] : 0.0;
-                v += (T)wj * ((T)a.mu[qb * a.ns + m] - T(2) * (T)cF);
+                v += (T)wj * ((T)a.mu[pix(qb, m, a.lay)] - T(2) * (T)cF);
             }
         }
         lam[q] = v;
@@ -201,8 +203,8 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<S> a)
             const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
             if (!computed(a, X, Y, Z)) continue;
             const int64_t n1 = i + off_of(q, a.nx, a.ny);
-            const double fd = (double)a.Sk[OPP[q] * a.ns + n1];
-            g += (double)a.mu[q * a.ns + n1] * (fd - (double)a.Sk[q * a.ns + i]) -
+            const double fd = (double)a.Sk[pix(OPP[q], n1, a.lay)];
+            g += (double)a.mu[pix(q, n1, a.lay)] * (fd - (double)a.Sk[pix(q, i, a.lay)]) -
                  2 * fd * (EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2]);
         }
         a.dJdw[i] += g;
@@ -219,16 +221,16 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<S> a)
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int64_t j = i - off_of(q, a.nx, a.ny);
-        if (!nr) { f[q] = (T)a.Sp[q * a.ns + j]; continue; }
+        if (!nr) { f[q] = (T)a.Sp[pix(q, j, a.lay)]; continue; }
         const double w = weight_eff(a, j, x - EX[q], y - EY[q], z - EZ[q]);
-        if (w == 1) f[q] = (T)a.Sp[OPP[q] * a.ns + i];
-        else if (w > 0) f[q] = (T(1) - (T)w) * (T)a.Sp[q * a.ns + j] + (T)w * (T)a.Sp[OPP[q] * a.ns + i];
-        else f[q] = (T)a.Sp[q * a.ns + j];
+        if (w == 1) f[q] = (T)a.Sp[pix(OPP[q], i, a.lay)];
+        else if (w > 0) f[q] = (T(1) - (T)w) * (T)a.Sp[pix(q, j, a.lay)] + (T)w * (T)a.Sp[pix(OPP[q], i, a.lay)];
+        else f[q] = (T)a.Sp[pix(q, j, a.lay)];
     }
     T mu[Q];
     collide_adj<T, LES, TRT>(f, lam, (T)a.tau, (T)a.K, (T)a.lambda, mu);
 #pragma unroll
-    for (int q = 0; q < Q; q++) a.mun[q * a.ns + i] = (S)mu[q];
+    for (int q = 0; q < Q; q++) a.mun[pix(q, i, a.lay)] = (S)mu[q];
 }
 
 template <class S, class T>
@@ -292,7 +294,7 @@ int adjoint_t(wt_lattice *h, int N, const double *Fbar, double *dJdw, double *fo
     WT_HIP(hipEventRecord(h->ev0, h->stream));
     AdjArgs<S> a{};
     a.solid = h->d_solid; a.sbits = h->d_sbits; a.nbits = h->d_nbits; a.wgt = h->d_wgt; a.dJdw = (double *)dg.p;
-    a.ns = h->ns; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx;
+    a.ns = h->ns; a.lay = wt::Lay{h->ns, h->bsh}; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx;
     a.tau = h->p.tau; a.K = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
     void *mu_k = mu0.p, *mu_n = mu1.p;
     for (int seg = nseg - 1; seg >= 0; seg--) {

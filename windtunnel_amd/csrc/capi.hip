@@ -69,7 +69,7 @@ int alloc(void **p, size_t bytes)
 
 void release(wt_lattice *h)
 {
-    for (void *p : {(void *)h->d_buf[0], (void *)h->d_buf[1], (void *)h->d_solid, (void *)h->d_sbits,
+    for (void *p : {(void *)h->d_buf[0], (void *)h->d_solid, (void *)h->d_sbits,
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
                     (void *)h->d_forces})
         if (p) (void)hipFree(p);
@@ -130,16 +130,33 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
     // Population planes are padded so that consecutive planes do not start
     // at the same offset modulo large powers of two: with 2^k-sized planes
     // all 38 streams of a step would hit the same HBM channel at once.
-    int64_t pad = 1088;
-    if (const char *e = std::getenv("WT_PLANE_PAD")) pad = std::atoll(e);
-    if (pad < 0) pad = 0;
-    h->ns = (h->n + 63) / 64 * 64 + pad;
+    // WT_BUFFER_PAD (elements) offsets the second AB buffer: a tuning knob.
+    int64_t bpad = 0;
+    if (const char *e = std::getenv("WT_BUFFER_PAD")) bpad = std::atoll(e);
+    if (bpad < 0) bpad = 0;
+    // Population layout, measured on MI355X: 64-cell blocks for 4- and
+    // 8-byte values, planar planes for binary16 (WT_BLOCK_SHIFT overrides
+    // it, 0 = planar: a tuning knob).
+    h->bsh = storage == WT_STORE_F16 ? 0 : 6;
+    if (const char *e = std::getenv("WT_BLOCK_SHIFT")) {
+        const int v = std::atoi(e);
+        h->bsh = v <= 0 ? 0 : std::max(5, std::min(10, v));
+    }
+    // planar planes are padded so that they do not start at the same offset
+    // modulo large powers of two (WT_PLANE_PAD, elements, overrides it)
+    int64_t ppad = 512;
+    if (const char *e = std::getenv("WT_PLANE_PAD")) ppad = std::max<int64_t>(0, std::atoll(e));
+    if (h->bsh) h->ns = (h->n + (int64_t(1) << h->bsh) - 1) >> h->bsh << h->bsh;
+    else h->ns = (h->n + 63) / 64 * 64 + (ppad + 63) / 64 * 64;
     const size_t ns = (size_t)h->ns;
+    // both AB buffers live in one allocation, the second starting bpad
+    // elements (rounded to 64) after the end of the first
+    const size_t bstride = 19 * ns + (size_t)(bpad + 63) / 64 * 64;
     const int bx = wt::row_block(nx);
     const int64_t blocks = (int64_t)((nx + bx - 1) / bx) * ny * nz;
     int rc = 0;
-    if (!rc) rc = alloc(&h->d_buf[0], 19 * ns * h->pop_elem);
-    if (!rc && layout == WT_LAYOUT_AB) rc = alloc(&h->d_buf[1], 19 * ns * h->pop_elem);
+    if (!rc) rc = alloc(&h->d_buf[0], (layout == WT_LAYOUT_AB ? bstride + 19 * ns : 19 * ns) * h->pop_elem);
+    if (!rc && layout == WT_LAYOUT_AB) h->d_buf[1] = (char *)h->d_buf[0] + bstride * h->pop_elem;
     // in place: slots no cell ever writes (outside-pointing slots of the
     // boundary planes) are never read either; zero them for reproducibility
     if (!rc && layout == WT_LAYOUT_ESOTERIC && hipMemset(h->d_buf[0], 0, 19 * ns * h->pop_elem) != hipSuccess)
@@ -442,31 +459,17 @@ int wt_get_pops(wt_lattice *h, int which, void *out)
     WT_CHECK_H(h);
     if (!out || (which != 0 && which != 1)) return wt_fail(WT_E_ARG, "bad population request");
     const size_t cnt = 19 * (size_t)h->n;
-    if (h->layout == WT_LAYOUT_ESOTERIC) {
-        if (which != 0) return wt_fail(WT_E_ARG, "an in-place lattice holds only the current populations");
-        const size_t eb = h->storage == WT_STORE_F64 ? 8 : 4;
-        void *d_out = nullptr;
-        if (int rc = alloc(&d_out, cnt * eb)) return rc;
-        int rc = lbm_get_pops(h, 0, d_out);
-        if (!rc && hipMemcpyAsync(out, d_out, cnt * eb, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
-            rc = wt_fail(WT_E_DEVICE, "population download failed");
-        if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "population read failed");
-        (void)hipFree(d_out);
-        return rc;
-    }
-    const void *src = h->d_buf[h->parity ^ which];
-    const size_t row = (size_t)h->n * h->pop_elem, pitch = (size_t)h->ns * h->pop_elem;
-    if (h->storage != WT_STORE_F16) {
-        WT_HIP(hipMemcpy2DAsync(out, row, src, pitch, row, 19, hipMemcpyDeviceToHost, h->stream));
-        WT_HIP(hipStreamSynchronize(h->stream));
-        return 0;
-    }
-    std::vector<__half> tmp(cnt);
-    WT_HIP(hipMemcpy2DAsync(tmp.data(), row, src, pitch, row, 19, hipMemcpyDeviceToHost, h->stream));
-    WT_HIP(hipStreamSynchronize(h->stream));
-    float *o = (float *)out;
-    for (size_t k = 0; k < cnt; k++) o[k] = __half2float(tmp[k]) * wt::F16_INV_SCALE;
-    return 0;
+    if (h->layout == WT_LAYOUT_ESOTERIC && which != 0)
+        return wt_fail(WT_E_ARG, "an in-place lattice holds only the current populations");
+    const size_t eb = h->storage == WT_STORE_F64 ? 8 : 4;
+    void *d_out = nullptr;
+    if (int rc = alloc(&d_out, cnt * eb)) return rc;
+    int rc = lbm_get_pops(h, which, d_out);
+    if (!rc && hipMemcpyAsync(out, d_out, cnt * eb, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+        rc = wt_fail(WT_E_DEVICE, "population download failed");
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "population read failed");
+    (void)hipFree(d_out);
+    return rc;
 }
 
 int wt_get_solid(wt_lattice *h, uint8_t *solid, uint8_t *near)

@@ -27,6 +27,7 @@
 #include "wt_internal.h"
 
 #include <cmath>
+#include <type_traits>
 
 namespace wt {
 namespace {
@@ -81,11 +82,11 @@ __host__ __device__ constexpr int64_t cell_off(int q, int nx, int ny)
 // i odd are opposite directions. rep 0: g_i(m) in slot i+1 of cell m+c_i,
 // g_{i+1}(m) in slot i of m. rep 1: g_i(m) in slot i of m+c_i, g_{i+1}(m)
 // in slot i+1 of m. The rest population stays in slot 0 of m.
-__device__ __forceinline__ int64_t slot_of(int q, int64_t m, int rep, int64_t ns, int nx, int ny)
+__device__ __forceinline__ int64_t slot_of(int q, int64_t m, int rep, Lay sh, int nx, int ny)
 {
-    if (q == 0) return m;
-    if (q & 1) return (int64_t)(rep ? q : q + 1) * ns + m + cell_off(q, nx, ny);
-    return (int64_t)(rep ? q : q - 1) * ns + m;
+    if (q == 0) return pix(0, m, sh);
+    if (q & 1) return pix(rep ? q : q + 1, m + cell_off(q, nx, ny), sh);
+    return pix(rep ? q : q - 1, m, sh);
 }
 
 // Is the slot of g_q at cell (x,y,z) inside the lattice? Only odd q place
@@ -106,7 +107,8 @@ template <class S, class MAC> struct StepArgs {
     const float *wgt;  // gray bounce-back weights (AB layout), or null
     MAC *rho, *ux, *uy, *uz;
     double *partial;
-    int64_t n, ns;              // ns: population plane stride (>= n)
+    int64_t n, ns;              // ns: cells per population plane (whole blocks)
+    Lay lay;                    // population layout
     int py0, py1, pz0, pz1;     // rows (y,z) whose blocks write force partials
     int nx, ny, nz, nwx, z0, mov, phase, rho_near;
     double tau, sq18cs2, lambda, u0;
@@ -183,6 +185,64 @@ __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq1
 #undef WT_PAIR
 }
 
+// The same collision for float arithmetic, rearranged for instruction count:
+// opposite pairs (a, a+1) enter through s = f_a + f_b and d = f_a - f_b, the
+// equilibrium pair shares its even part W(A + 4.5 r cu^2) and odd part
+// 3 W r cu, products are fused, and 1/x and sqrt use the hardware's
+// single-precision approximations (about 1 ulp). Same model, same results to
+// float rounding; the binary64 path above keeps the reference's order.
+template <bool LES, bool TRT>
+__device__ __forceinline__ void collide_fast(const float (&f)[Q], float tau0, float omega0, float sq18cs2, float lam,
+                                             float (&g)[Q], float &r, float &vx, float &vy, float &vz)
+{
+    float s[9], d[9];
+#pragma unroll
+    for (int p = 0; p < 9; p++) { s[p] = f[2 * p + 1] + f[2 * p + 2]; d[p] = f[2 * p + 1] - f[2 * p + 2]; }
+    const float dr = f[0] + (((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])) + s[8]);
+    r = 1.0f + dr;
+    const float inv = __builtin_amdgcn_rcpf(r);
+    // pair directions: x, y, z, x+y, x-y, x+z, x-z, y+z, y-z
+    vx = (d[0] + d[3] + d[4] + d[5] + d[6]) * inv;
+    vy = (d[1] + d[3] - d[4] + d[7] + d[8]) * inv;
+    vz = (d[2] + d[5] - d[6] + d[7] - d[8]) * inv;
+    const float usq = __builtin_fmaf(vx, vx, __builtin_fmaf(vy, vy, vz * vz));
+    const float A = __builtin_fmaf(-1.5f * r, usq, dr);
+    const float r45 = 4.5f * r;
+    const float cu[9] = {vx, vy, vz, vx + vy, vx - vy, vx + vz, vx - vz, vy + vz, vy - vz};
+    const float WF = 1.0f / 18, WE = 1.0f / 36;
+    const float oF = 3.0f * WF * r, oE = 3.0f * WE * r;
+    float sp[9], sm[9];  // (n_a + n_b)/2 and (n_a - n_b)/2 of each pair
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+        const float w = p < 3 ? WF : WE, o = p < 3 ? oF : oE;
+        const float even = w * __builtin_fmaf(r45 * cu[p], cu[p], A);
+        sp[p] = __builtin_fmaf(0.5f, s[p], -even);
+        sm[p] = __builtin_fmaf(0.5f, d[p], -o * cu[p]);
+    }
+    const float n0 = __builtin_fmaf(-1.0f / 3, A, f[0]);
+    float om = omega0, tauEff = tau0;
+    if (LES) {
+        const float pxx = sp[0] + sp[3] + sp[4] + sp[5] + sp[6];
+        const float pyy = sp[1] + sp[3] + sp[4] + sp[7] + sp[8];
+        const float pzz = sp[2] + sp[5] + sp[6] + sp[7] + sp[8];
+        const float pxy = sp[3] - sp[4], pxz = sp[5] - sp[6], pyz = sp[7] - sp[8];
+        // the sums above are half the tensor components: mag = sqrt(8 (...))
+        const float q2 = __builtin_fmaf(pxx, pxx, __builtin_fmaf(pyy, pyy, pzz * pzz)) +
+                         2.0f * __builtin_fmaf(pxy, pxy, __builtin_fmaf(pxz, pxz, pyz * pyz));
+        const float mag = __builtin_amdgcn_sqrtf(8.0f * q2);
+        tauEff = 0.5f * (tau0 + __builtin_amdgcn_sqrtf(__builtin_fmaf(tau0, tau0, sq18cs2 * mag * inv)));
+        om = __builtin_amdgcn_rcpf(tauEff);
+    }
+    const float omm = TRT ? __builtin_amdgcn_rcpf(__builtin_fmaf(lam, __builtin_amdgcn_rcpf(tauEff - 0.5f), 0.5f)) : om;
+    g[0] = __builtin_fmaf(-om, n0, f[0]);
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+        const float x = om * sp[p];
+        g[2 * p + 1] = __builtin_fmaf(-omm, sm[p], f[2 * p + 1] - x);
+        g[2 * p + 2] = __builtin_fmaf(omm, sm[p], f[2 * p + 2] - x);
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -203,9 +263,9 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
     // store of one value of cell m (AB: home slot in g; esoteric: slot in the new representation)
     auto put = [&](int q, int64_t m, int cx, int cy, int cz, T v) {
         if (ESO) {
-            if (slot_inside(q, cx, cy, cz, nx, ny, nz)) P::template st<T>(a.G, slot_of(q, m, nph, a.ns, nx, ny), v);
+            if (slot_inside(q, cx, cy, cz, nx, ny, nz)) P::template st<T>(a.G, slot_of(q, m, nph, a.lay, nx, ny), v);
         } else {
-            P::template st<T>(a.G, q * a.ns + m, v);
+            P::template st<T>(a.G, pix(q, m, a.lay), v);
         }
     };
 
@@ -236,7 +296,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
                 // only: in place, a solid cell holds no values of its own)
                 if (!ESO) {
 #pragma unroll
-                    for (int q = 0; q < Q; q++) a.G[q * a.ns + i + 1] = a.G[q * a.ns + i];
+                    for (int q = 0; q < Q; q++) a.G[pix(q, i + 1, a.lay)] = a.G[pix(q, i, a.lay)];
                 }
                 if (MACRO) {
                     a.rho[i + 1] = a.rho[i]; a.ux[i + 1] = a.ux[i];
@@ -253,7 +313,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
                     const int64_t j = i - cell_off(q, nx, ny);
                     const int sj = a.solid[j];
                     if (sj) {
-                        const int64_t k = ESO ? slot_of(OPP[q], i, ph, a.ns, nx, ny) : OPP[q] * a.ns + i;
+                        const int64_t k = ESO ? slot_of(OPP[q], i, ph, a.lay, nx, ny) : pix(OPP[q], i, a.lay);
                         const T fd = P::template ld<T>(a.F, k);
                         T corr = 0;
                         if (a.mov) {
@@ -271,12 +331,12 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
                     } else if (!ESO && a.wgt && a.wgt[j] > 0.0f) {
                         // gray upstream cell: a fraction w_j of the link bounces back
                         const T wj = (T)a.wgt[j];
-                        const T fd = P::template ld<T>(a.F, OPP[q] * a.ns + i);
-                        f[q] = (T(1) - wj) * P::template ld<T>(a.F, q * a.ns + j) + wj * fd;
+                        const T fd = P::template ld<T>(a.F, pix(OPP[q], i, a.lay));
+                        f[q] = (T(1) - wj) * P::template ld<T>(a.F, pix(q, j, a.lay)) + wj * fd;
                         const double d = (double)(wj * (T(2) * fd));
                         fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
                     } else {
-                        const int64_t k = ESO ? slot_of(q, j, ph, a.ns, nx, ny) : q * a.ns + j;
+                        const int64_t k = ESO ? slot_of(q, j, ph, a.lay, nx, ny) : pix(q, j, a.lay);
                         f[q] = P::template ld<T>(a.F, k);
                     }
                 }
@@ -284,12 +344,16 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
                     const int64_t j = i - cell_off(q, nx, ny);
-                    const int64_t k = ESO ? slot_of(q, j, ph, a.ns, nx, ny) : q * a.ns + j;
+                    const int64_t k = ESO ? slot_of(q, j, ph, a.lay, nx, ny) : pix(q, j, a.lay);
                     f[q] = P::template ld<T>(a.F, k);
                 }
             }
             T g[Q], r, vx, vy, vz;
-            collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+            if constexpr (std::is_same<T, float>::value)
+                collide_fast<LES, TRT>(f, (float)a.tau, (float)(1 / a.tau), (float)a.sq18cs2, (float)a.lambda, g, r,
+                                       vx, vy, vz);
+            else
+                collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 put(q, i, x, y, z, g[q]);
@@ -349,6 +413,7 @@ template <class S, class MAC> struct InitArgs {
     MAC *rho, *ux, *uy, *uz;
     const uint8_t *solid, *ops;
     int64_t n, ns;
+    Lay lay;
     int nx, ny, nz, rep, eso;
     double u0;
     double eq_u0[Q], eq_rest[Q];
@@ -363,12 +428,12 @@ __device__ void write_state(const InitArgs<S, MAC> &a, int64_t i, const double *
         const int x = (int)(i % a.nx), y = (int)((i / a.nx) % a.ny), z = (int)(i / ((int64_t)a.nx * a.ny));
         for (int q = 0; q < Q; q++)
             if (slot_inside(q, x, y, z, a.nx, a.ny, a.nz))
-                Pop<S>::template st<double>(a.f, slot_of(q, i, a.rep, a.ns, a.nx, a.ny), v[q]);
+                Pop<S>::template st<double>(a.f, slot_of(q, i, a.rep, a.lay, a.nx, a.ny), v[q]);
         return;
     }
     for (int q = 0; q < Q; q++) {
-        Pop<S>::template st<double>(a.f, q * a.ns + i, v[q]);
-        if (both) Pop<S>::template st<double>(a.g, q * a.ns + i, v[q]);
+        Pop<S>::template st<double>(a.f, pix(q, i, a.lay), v[q]);
+        if (both) Pop<S>::template st<double>(a.g, pix(q, i, a.lay), v[q]);
     }
 }
 
@@ -404,19 +469,21 @@ __global__ void __launch_bounds__(256) k_apply_ops(InitArgs<S, MAC> a)
     }
 }
 
-// Esoteric buffer -> the reference's f[q*n + cell] order. Values a cell
+// Device buffer -> the reference's f[q*n + cell] order. AB: the buffer's
+// value of every cell. Esoteric: the current representation; values a cell
 // never stores (solid cells; out-of-lattice slots of boundary cells) read 0.
 template <class S, class OUT>
-__global__ void __launch_bounds__(256) k_eso_read(const S *B, const uint8_t *solid, OUT *out, int64_t n, int64_t ns,
-                                                  int nx, int ny, int nz, int rep)
+__global__ void __launch_bounds__(256) k_read_pops(const S *B, const uint8_t *solid, OUT *out, int64_t n, int nx,
+                                                   int ny, int nz, int rep, int eso, Lay lay)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
     for (int q = 0; q < Q; q++) {
         OUT v = 0;
-        if (!solid[i] && slot_inside(q, x, y, z, nx, ny, nz))
-            v = (OUT)Pop<S>::template ld<double>(B, slot_of(q, i, rep, ns, nx, ny));
+        if (!eso) v = (OUT)Pop<S>::template ld<double>(B, pix(q, i, lay));
+        else if (!solid[i] && slot_inside(q, x, y, z, nx, ny, nz))
+            v = (OUT)Pop<S>::template ld<double>(B, slot_of(q, i, rep, lay, nx, ny));
         out[q * n + i] = v;
     }
 }
@@ -502,7 +569,7 @@ template <class S, class MAC> InitArgs<S, MAC> init_args(wt_lattice *h, const ui
     a.g = eso ? nullptr : (S *)h->d_buf[h->parity ^ 1];
     MAC *m = (MAC *)h->d_mac;
     a.rho = m; a.ux = m + h->n; a.uy = m + 2 * h->n; a.uz = m + 3 * h->n;
-    a.solid = h->d_solid; a.ops = ops; a.n = h->n; a.ns = h->ns; a.u0 = h->p.u0;
+    a.solid = h->d_solid; a.ops = ops; a.n = h->n; a.ns = h->ns; a.lay = Lay{h->ns, h->bsh}; a.u0 = h->p.u0;
     a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.rep = h->parity; a.eso = eso;
     eq_deviation(1, h->p.u0, 0, 0, a.eq_u0);
     eq_deviation(1, 0, 0, 0, a.eq_rest);
@@ -545,7 +612,7 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     MAC *m = (MAC *)h->d_mac;
     a.rho = m; a.ux = m + h->n; a.uy = m + 2 * h->n; a.uz = m + 3 * h->n;
     a.partial = h->d_partial;
-    a.n = h->n; a.ns = h->ns; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx; a.z0 = bz0;
+    a.n = h->n; a.ns = h->ns; a.lay = Lay{h->ns, h->bsh}; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx; a.z0 = bz0;
     a.mov = h->p.movingWalls;
     a.rho_near = !h->replay;
     a.tau = h->p.tau; a.sq18cs2 = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
@@ -648,18 +715,19 @@ int lbm_launch_step(wt_lattice *h, int bz0, int bz1, bool macro, double *d_force
 
 int lbm_get_pops(wt_lattice *h, int which, void *d_out)
 {
-    if (h->layout != WT_LAYOUT_ESOTERIC) return wt_fail(WT_E_STATE, "not an esoteric lattice");
-    if (which != 0) return wt_fail(WT_E_ARG, "an in-place lattice holds only the current populations");
+    const bool eso = h->layout == WT_LAYOUT_ESOTERIC;
+    if (eso && which != 0) return wt_fail(WT_E_ARG, "an in-place lattice holds only the current populations");
+    const void *B = eso ? h->d_buf[0] : h->d_buf[h->parity ^ which];
     const unsigned nb = (unsigned)((h->n + 255) / 256);
     if (h->storage == WT_STORE_F64)
-        hipLaunchKernelGGL((k_eso_read<double, double>), dim3(nb), dim3(256), 0, h->stream, (const double *)h->d_buf[0],
-                           h->d_solid, (double *)d_out, h->n, h->ns, h->nx, h->ny, h->nz, h->parity);
+        hipLaunchKernelGGL((k_read_pops<double, double>), dim3(nb), dim3(256), 0, h->stream, (const double *)B,
+                           h->d_solid, (double *)d_out, h->n, h->nx, h->ny, h->nz, h->parity, (int)eso, Lay{h->ns, h->bsh});
     else if (h->storage == WT_STORE_F16)
-        hipLaunchKernelGGL((k_eso_read<__half, float>), dim3(nb), dim3(256), 0, h->stream, (const __half *)h->d_buf[0],
-                           h->d_solid, (float *)d_out, h->n, h->ns, h->nx, h->ny, h->nz, h->parity);
+        hipLaunchKernelGGL((k_read_pops<__half, float>), dim3(nb), dim3(256), 0, h->stream, (const __half *)B,
+                           h->d_solid, (float *)d_out, h->n, h->nx, h->ny, h->nz, h->parity, (int)eso, Lay{h->ns, h->bsh});
     else
-        hipLaunchKernelGGL((k_eso_read<float, float>), dim3(nb), dim3(256), 0, h->stream, (const float *)h->d_buf[0],
-                           h->d_solid, (float *)d_out, h->n, h->ns, h->nx, h->ny, h->nz, h->parity);
+        hipLaunchKernelGGL((k_read_pops<float, float>), dim3(nb), dim3(256), 0, h->stream, (const float *)B,
+                           h->d_solid, (float *)d_out, h->n, h->nx, h->ny, h->nz, h->parity, (int)eso, Lay{h->ns, h->bsh});
     WT_HIP(hipGetLastError());
     return 0;
 }

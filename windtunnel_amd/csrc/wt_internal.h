@@ -22,6 +22,24 @@ constexpr int EZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1
 constexpr int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
 enum { FLUID = 0, SOLID_OBJ = 1, SOLID_WALL = 2 };
 
+// Populations are stored in blocks of 2^bsh cells (64 = one wavefront): the values
+// of one direction for 64 consecutive cells are contiguous (coalesced per
+// direction), and a block's 19 directions follow each other, so a wave's 19
+// loads come from a few kilobytes of neighbouring rows and its 19 stores go
+// to one contiguous 4.75 KiB run (19 planes a lattice apart cost ~11% of
+// HBM bandwidth on MI355X). Any nx works; readback converts to the
+// reference's f[q*n + cell] order.
+// sh = 0 selects the planar layout instead: plane q starts at q*ns.
+struct Lay {
+    int64_t ns;  // cells per plane
+    int sh;      // log2 of the cells per block, or 0 for planar
+};
+__host__ __device__ __forceinline__ int64_t pix(int q, int64_t cell, Lay L)
+{
+    if (L.sh == 0) return q * L.ns + cell;
+    return (((cell >> L.sh) * Q + q) << L.sh) + (cell & ((int64_t(1) << L.sh) - 1));
+}
+
 // binary16 storage keeps (f_q - w_q) * F16_SCALE: a power of two, so the
 // scaling itself is exact, sized so |f_q - w_q| up to 1.99 stays below the
 // binary16 maximum while resolvable deviations stay far above its subnormals.
@@ -45,7 +63,8 @@ struct Body {
 struct wt_lattice {
     int nx = 0, ny = 0, nz = 0;
     int64_t n = 0;
-    int64_t ns = 0;     // population plane stride in elements (n plus padding)
+    int64_t ns = 0;     // cells per population plane, n rounded up to a whole block
+    int bsh = 6;        // log2 of the cells per storage block, 0 = planar planes
     int storage = WT_STORE_F32, arith = WT_ARITH_F32, layout = WT_LAYOUT_AB;
     int pop_elem = 4;   // bytes per stored population
     int mac_elem = 4;   // bytes per macroscopic field value
