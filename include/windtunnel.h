@@ -75,10 +75,13 @@ typedef struct {
     int32_t movingWalls, ground;
 } wt_body_stats;
 
-/* Observable scalar state of an LBM object (js/lbm.js:80,113,117). */
+/* Observable scalar state of an LBM object (js/lbm.js:80,113,117), plus
+ * the device storage layout: block_cells = cells per storage block of the
+ * population buffers (0 = one plane per direction). */
 typedef struct {
     int64_t parity, steps;
     double force[3];
+    int32_t block_cells, reserved;
 } wt_state_t;
 
 /* Voxel grid geometry and statistics: the object Voxel.build returns

@@ -7,6 +7,9 @@
   density, velocity and C_p, and of the force-coefficient series, at
   laminar Reynolds numbers;
 * FP64 storage must equal the oracle's FP64-storage run bit for bit.
+
+The bitwise tests run in both population layouts: planar planes and
+blocks of 2^k cells (WT_BLOCK_SHIFT = k, any k whose block divides nx).
 """
 import numpy as np
 import pytest
@@ -18,6 +21,21 @@ from windtunnel_amd.lbm import LBM
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("rho", "ux", "uy", "uz")
+
+
+# storage layouts: planar, and 8-cell blocks (every golden grid has nx % 8 == 0)
+LAYOUTS = [0, 3]
+
+
+@pytest.fixture
+def block_shift(request, monkeypatch):
+    """Select the population layout of lattices created in the test."""
+    monkeypatch.setenv("WT_BLOCK_SHIFT", str(request.param))
+    return request.param
+
+
+def expect_layout(L, shift):
+    assert L.block_cells == (1 << shift if shift else 0)
 
 
 def make(fx, **kw):
@@ -45,11 +63,13 @@ def check_hashes(L, want, keys=("f", "g", "rho", "ux", "uy", "uz")):
     assert L.parity == want["parity"] and L.steps == want["steps"]
 
 
+@pytest.mark.parametrize("block_shift", LAYOUTS, indirect=True)
 @pytest.mark.parametrize("name", ["lbm_bgk_sphere", "lbm_trt_les", "lbm_moving", "lbm_outlet_body", "lbm_revox"])
-def test_exact_mode_bitwise(name):
+def test_exact_mode_bitwise(name, block_shift):
     fx = load(name)
     L = make(fx, arith="f64")
-    print(f"[exact] {name} dims={fx['dims']}", flush=True)
+    expect_layout(L, block_shift)
+    print(f"[exact] {name} dims={fx['dims']} block_shift={block_shift}", flush=True)
     replay_voxelize(L, fx["voxelize"])
     s, nr = L.solid_near()
     assert sha(s) == fx["voxelize"]["after"]["solid"]
@@ -85,10 +105,12 @@ def test_exact_mode_init_and_voxelize():
         assert L.movingWalls == rec["result"]["movingWalls"]
 
 
-def test_exact_mode_c1_2000_steps():
+@pytest.mark.parametrize("block_shift", [0, 5], indirect=True)
+def test_exact_mode_c1_2000_steps(block_shift):
     """96x48x48 BGK sphere, the CPU-runnable workload, 2000 steps bitwise."""
     fx = load("lbm_c1")
     L = make(fx, arith="f64")
+    expect_layout(L, block_shift)
     replay_voxelize(L, fx["voxelize"])
     want = arr(fx["forces"])
     done = 0
@@ -174,10 +196,12 @@ def test_fp64_storage_matches_fp64_oracle():
         assert np.array_equal(L.fields()[k], O.array(k))
 
 
-def test_pool_slabs_exact():
+@pytest.mark.parametrize("block_shift", LAYOUTS, indirect=True)
+def test_pool_slabs_exact(block_shift):
     """Worker-pool form: 3 slabs, per-slab forces, fields equal to the fixture."""
     fx = load("lbm_pool3")
     L = LBM(*fx["dims"], arith="f64")
+    expect_layout(L, block_shift)
     set_params(L, fx["worker_params"])
     L.reset()
     replay_voxelize(L, fx["voxelize"])
@@ -250,13 +274,15 @@ def stored_mask(dims, solid):
     return np.concatenate(ok)
 
 
+@pytest.mark.parametrize("block_shift", LAYOUTS, indirect=True)
 @pytest.mark.parametrize("name", ["lbm_bgk_sphere", "lbm_trt_les", "lbm_moving", "lbm_revox"])
-def test_esoteric_exact_mode(name):
+def test_esoteric_exact_mode(name, block_shift):
     """In-place streaming with binary64 collision: the macroscopic fields are
     the reference's bit for bit, and every stored population equals the
     oracle's f."""
     fx = load(name)
     L = make(fx, arith="f64", layout="esoteric")
+    expect_layout(L, block_shift)
     print(f"[esoteric exact] {name}", flush=True)
     replay_voxelize(L, fx["voxelize"])
     O = oracle_for(fx)
@@ -309,3 +335,61 @@ def test_esoteric_pool_slabs_and_f16():
         L.step(40)
     for k in FIELDS:
         assert np.array_equal(a.fields()[k], b.fields()[k]), k
+
+
+def sphere_lattice(dims, **kw):
+    L = LBM(*dims, **kw)
+    L.set_params(tau=0.52, les=True, csm=0.1, trt=True, u0=0.08, **{"lambda": 3 / 16})
+    L.reset()
+    L.voxelize_mask(*sphere_mask(dims))
+    return L
+
+
+@pytest.mark.parametrize("arith,storage", [("f64", "f32"), ("f32", "f32"), ("f32", "f16")])
+def test_layouts_agree_bitwise(arith, storage, monkeypatch):
+    """128x40x36 sphere at the production defaults (64-cell blocks for 4-byte
+    storage): planar, blocked and in-place lattices compute the same
+    populations, fields and forces bit for bit, and in binary64 arithmetic
+    they equal the oracle."""
+    dims = (128, 40, 36)
+    runs = {}
+    for shift in (0, 6, 4):
+        for layout in ("ab", "esoteric"):
+            monkeypatch.setenv("WT_BLOCK_SHIFT", str(shift))
+            L = sphere_lattice(dims, arith=arith, storage=storage, layout=layout)
+            expect_layout(L, shift)
+            print(f"[layouts] {arith}/{storage} shift={shift} {layout}", flush=True)
+            f = L.step(37)
+            runs[(shift, layout)] = (f, {k: v.copy() for k, v in L.fields().items()}, L.solid.copy())
+            L.close()
+    base = runs[(0, "ab")]
+    for key, (f, F, _) in runs.items():
+        for k in FIELDS:
+            assert np.array_equal(F[k], base[1][k]), (key, k)
+        np.testing.assert_allclose(f, base[0], rtol=1e-12, atol=1e-15, err_msg=str(key))
+    if arith == "f64":
+        monkeypatch.setenv("WT_BLOCK_SHIFT", "6")
+        L = sphere_lattice(dims, arith=arith, storage=storage)
+        O = OracleLBM(*dims)
+        O.set_params(tau=0.52, les=True, csm=0.1, trt=True, u0=0.08, **{"lambda": 3 / 16})
+        O.reset()
+        O.voxelize_mask(*sphere_mask(dims))
+        assert np.array_equal(O.array("solid"), base[2])
+        fo = O.step(37)
+        L.step(37)
+        assert np.array_equal(L.f, O.array("f"))
+        for k in FIELDS:
+            assert np.array_equal(base[1][k], O.array(k)), k
+        np.testing.assert_allclose(base[0], fo, rtol=1e-11, atol=1e-14)
+
+
+def sphere_mask(dims):
+    """Off-centre sphere (radius 0.22 ny) at cell centres, over its box."""
+    nx, ny, nz = dims
+    c = np.array([0.3 * nx, ny / 2 - 0.3, nz / 2 + 0.2])
+    R = 0.22 * ny
+    lo = np.floor(c - R).astype(int) - 1
+    hi = np.ceil(c + R).astype(int) + 1
+    z, y, x = np.meshgrid(*(np.arange(lo[k], hi[k] + 1) for k in (2, 1, 0)), indexing="ij")
+    m = ((x + 0.5 - c[0]) ** 2 + (y + 0.5 - c[1]) ** 2 + (z + 0.5 - c[2]) ** 2 <= R * R).astype(np.uint8)
+    return [int(lo[0]), int(lo[1]), int(lo[2]), int(hi[0]), int(hi[1]), int(hi[2])], m.reshape(-1)

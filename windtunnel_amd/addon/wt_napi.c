@@ -311,6 +311,7 @@ static napi_value js_get_state(napi_env env, napi_callback_info info)
     CALL(env, napi_create_object(env, &o));
     set_num(env, o, "parity", (double)s.parity);
     set_num(env, o, "steps", (double)s.steps);
+    set_num(env, o, "blockCells", (double)s.block_cells);
     CALL(env, napi_create_array_with_length(env, 3, &f));
     for (uint32_t k = 0; k < 3; k++) {
         napi_value e;

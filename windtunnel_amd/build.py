@@ -43,6 +43,31 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     return out
 
 
+ADDON_SRC = os.path.join(HERE, "addon", "wt_napi.c")
+ADDON = os.path.join(HERE, "wt_napi.node")
+NODE_INCLUDE = os.environ.get("NODE_INCLUDE", "/usr/include/node")
+
+
+def build_addon(verbose: bool = False, out: str = ADDON):
+    """Compile the N-API addon (plain C over the C-ABI) next to the library;
+    it finds libwindtunnel.so through its own directory. Returns None when
+    the Node.js headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print(f"node_api.h not found under {NODE_INCLUDE}: N-API addon not built", flush=True)
+        return None
+    tmp = out + ".tmp"
+    cmd = ["gcc", "-O2", "-std=c11", "-shared", "-fPIC", "-Wall", f"-I{NODE_INCLUDE}",
+           f"-I{os.path.join(HERE, '..', 'include')}", "-o", tmp, ADDON_SRC,
+           f"-L{HERE}", "-lwindtunnel", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
     print(LIB)
+    print(build_addon(verbose="-v" in sys.argv))

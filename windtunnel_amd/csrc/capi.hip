@@ -135,13 +135,16 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
     if (const char *e = std::getenv("WT_BUFFER_PAD")) bpad = std::atoll(e);
     if (bpad < 0) bpad = 0;
     // Population layout, measured on MI355X: 64-cell blocks for 4- and
-    // 8-byte values, planar planes for binary16 (WT_BLOCK_SHIFT overrides
-    // it, 0 = planar: a tuning knob).
+    // 8-byte values, planar planes for binary16 (WT_BLOCK_SHIFT = log2 of
+    // the block overrides it, 0 = planar: a tuning and test knob).
     h->bsh = storage == WT_STORE_F16 ? 0 : 6;
     if (const char *e = std::getenv("WT_BLOCK_SHIFT")) {
         const int v = std::atoi(e);
-        h->bsh = v <= 0 ? 0 : std::max(5, std::min(10, v));
+        h->bsh = v <= 0 ? 0 : std::max(2, std::min(10, v));
     }
+    // The step addresses a block-laid row as whole blocks, so the blocked
+    // layout needs rows of a whole number of blocks; other widths use planes.
+    if (h->bsh && nx % (1 << h->bsh)) h->bsh = 0;
     // planar planes are padded so that they do not start at the same offset
     // modulo large powers of two (WT_PLANE_PAD, elements, overrides it)
     int64_t ppad = 512;
@@ -439,6 +442,8 @@ int wt_get_state(const wt_lattice *h, wt_state_t *out)
     if (!out) return wt_fail(WT_E_ARG, "null output");
     out->parity = h->parity; out->steps = h->steps;
     for (int k = 0; k < 3; k++) out->force[k] = h->force[k];
+    out->block_cells = h->bsh ? 1 << h->bsh : 0;
+    out->reserved = 0;
     return 0;
 }
 

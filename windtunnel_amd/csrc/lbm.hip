@@ -59,19 +59,31 @@ template <> struct Pop<float> {
     template <class T> __device__ static void st(float *p, int64_t k, T v) { st_pop(p, k, (float)v); }
 };
 template <> struct Pop<double> {
-    template <class T> __device__ static T ld(const double *p, int64_t k) { return (T)p[k]; }
-    template <class T> __device__ static void st(double *p, int64_t k, T v) { p[k] = (double)v; }
+    template <class T> __device__ static T ld(const double *p, int64_t k) { return (T)ld_pop(p, k); }
+    template <class T> __device__ static void st(double *p, int64_t k, T v) { st_pop(p, k, (double)v); }
 };
 template <> struct Pop<__half> {
     template <class T> __device__ static T ld(const __half *p, int64_t k)
     {
-        return (T)(__half2float(p[k]) * F16_INV_SCALE);
+        return (T)(__half2float(ld_pop(p, k)) * F16_INV_SCALE);
     }
     template <class T> __device__ static void st(__half *p, int64_t k, T v)
     {
-        p[k] = __float2half_rn((float)v * F16_SCALE);
+        st_pop(p, k, __float2half_rn((float)v * F16_SCALE));
     }
 };
+
+// Population at byte address base + off: base is uniform across the wave,
+// off a 32-bit per-lane offset (one scalar base register pair + one vector
+// register per access).
+template <class S, class T> __device__ __forceinline__ T ld_at(const S *base, uint32_t off)
+{
+    return Pop<S>::template ld<T>((const S *)((const char *)base + off), 0);
+}
+template <class S, class T> __device__ __forceinline__ void st_at(S *base, uint32_t off, T v)
+{
+    Pop<S>::template st<T>((S *)((char *)base + off), 0, v);
+}
 
 __host__ __device__ constexpr int64_t cell_off(int q, int nx, int ny)
 {
@@ -112,8 +124,28 @@ template <class S, class MAC> struct StepArgs {
     int py0, py1, pz0, pz1;     // rows (y,z) whose blocks write force partials
     int nx, ny, nz, nwx, z0, mov, phase, rho_near;
     double tau, sq18cs2, lambda, u0;
+    float tauf, omf, sq18f, lamf;  // the same in float: tau, 1/tau, 18 sqrt(2) Cs^2, Lambda
     double eqv[Q];  // boundary equilibrium deviation at (rho=1, u0, 0, 0)
+    // Interior-cell addressing. Rows are whole storage blocks, so the
+    // address of every population a cell reads or writes is
+    //   lp[q] (or sp[q]) + row term (uniform per lattice row)
+    //                    + lane term of x - shift (3 distinct values per cell)
+    // lp/sp: address of direction q's load/store slot for the cell at the
+    // start of row 0, including the y/z part of the streaming shift and, in
+    // place, the slot of the current phase (it may lie before the buffer:
+    // only base + row term + lane term is dereferenced). Row term:
+    // (row >> lsh) * lstride elements; lane term of xs:
+    // (xs >> lsh) * lstride + (xs & lmask).
+    const S *lp[Q];
+    S *sp[Q];
+    int lsh, lstride, lmask;
 };
+
+// x shift of the cell whose slot holds the value a cell pulls along q (AB:
+// the upstream cell; in place, odd q sit at home) and of the cell whose
+// slot receives its post-collision g_q (in place, odd q go downstream).
+template <bool ESO> __device__ constexpr int ld_shift(int q) { return ESO && (q & 1) ? 0 : -EX[q]; }
+template <bool ESO> __device__ constexpr int st_shift(int q) { return ESO && (q & 1) ? EX[q] : 0; }
 
 __device__ __forceinline__ bool bit(const uint32_t *bits, int nwx, int ny, int x, int y, int z)
 {
@@ -275,7 +307,42 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
         const bool bnd = x == 0 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1;
         const bool sol = bit(a.sbits, a.nwx, ny, x, y, z);
         const bool out_sol = out && bit(a.sbits, a.nwx, ny, x + 1, y, z);
-        if (bnd || sol) {
+        const bool nr = bit(a.nbits, a.nwx, ny, x, y, z);  // only interior fluid cells are near
+        if (!(bnd || sol || nr)) {
+            // Interior fluid cell with no wall link: pull, collide, store.
+            // Rows are whole blocks, so every address is a per-direction
+            // uniform base plus the lane term of x-1, x or x+1.
+            const int64_t row = (int64_t)nx * (y + (int64_t)ny * z);
+            const int64_t rb = (row >> a.lsh) * a.lstride;
+            auto lane = [&](int xs) {
+                return (uint32_t)(((xs >> a.lsh) * a.lstride + (xs & a.lmask)) * (int)sizeof(S));
+            };
+            const uint32_t o[3] = {lane(x - 1), lane(x), lane(x + 1)};
+            T f[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) f[q] = ld_at<S, T>(a.lp[q] + rb, o[1 + ld_shift<ESO>(q)]);
+            T g[Q], r, vx, vy, vz;
+            if constexpr (std::is_same<T, float>::value)
+                collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
+                                       vx, vy, vz);
+            else
+                collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+#pragma unroll
+            for (int q = 0; q < Q; q++) st_at<S, T>(a.sp[q] + rb, o[1 + st_shift<ESO>(q)], g[q]);
+            if (out && !(ESO && out_sol)) {
+                // zero-gradient outlet: the same values again for x = nx-1
+                const uint32_t o2 = lane(x + 2);
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const int s = 1 + st_shift<ESO>(q);
+                    st_at<S, T>(a.sp[q] + rb, s == 0 ? o[1] : (s == 1 ? o[2] : o2), g[q]);
+                }
+            }
+            if (MACRO) {
+                a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
+                if (out) { a.rho[i + 1] = (MAC)r; a.ux[i + 1] = (MAC)vx; a.uy[i + 1] = (MAC)vy; a.uz[i + 1] = (MAC)vz; }
+            }
+        } else if (bnd || sol) {
             // Equilibrium planes (js/lbm.js:488-503): all of them write except
             // the floor where it is solid and no other plane covers the cell.
             // In place, a solid cell never writes: its slots are where its
@@ -304,10 +371,10 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
                 }
             }
         } else {
+            // near-wall cell: halfway bounce-back, moving wall, momentum
+            // exchange (js/lbm.js:334-368)
             T f[Q];
-            const bool nr = bit(a.nbits, a.nwx, ny, x, y, z);
-            if (nr) {
-                // halfway bounce-back, moving wall, momentum exchange (js/lbm.js:334-368)
+            {
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
                     const int64_t j = i - cell_off(q, nx, ny);
@@ -340,17 +407,10 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
                         f[q] = P::template ld<T>(a.F, k);
                     }
                 }
-            } else {
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    const int64_t j = i - cell_off(q, nx, ny);
-                    const int64_t k = ESO ? slot_of(q, j, ph, a.lay, nx, ny) : pix(q, j, a.lay);
-                    f[q] = P::template ld<T>(a.F, k);
-                }
             }
             T g[Q], r, vx, vy, vz;
             if constexpr (std::is_same<T, float>::value)
-                collide_fast<LES, TRT>(f, (float)a.tau, (float)(1 / a.tau), (float)a.sq18cs2, (float)a.lambda, g, r,
+                collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
                                        vx, vy, vz);
             else
                 collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
@@ -362,7 +422,7 @@ __global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
             if (MACRO) {
                 a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
                 if (out) { a.rho[i + 1] = (MAC)r; a.ux[i + 1] = (MAC)vx; a.uy[i + 1] = (MAC)vy; a.uz[i + 1] = (MAC)vz; }
-            } else if (nr && a.rho_near) {
+            } else if (a.rho_near) {
                 a.rho[i] = (MAC)r;  // next step's moving-wall correction reads it
             }
         }
@@ -616,8 +676,31 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     a.mov = h->p.movingWalls;
     a.rho_near = !h->replay;
     a.tau = h->p.tau; a.sq18cs2 = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
+    a.tauf = (float)a.tau; a.omf = (float)(1 / a.tau); a.sq18f = (float)a.sq18cs2; a.lamf = (float)a.lambda;
     a.u0 = h->p.u0;
     eq_deviation(1, h->p.u0, 0, 0, a.eqv);
+    // interior addressing tables (rows are whole blocks: nx % 2^bsh == 0)
+    a.lsh = h->bsh;
+    a.lstride = h->bsh ? Q << h->bsh : 1;
+    a.lmask = h->bsh ? (1 << h->bsh) - 1 : 0;
+    {
+        const int ph = h->parity, nph = ph ^ 1;
+        // element offset of slot `slot` of the row-0 cell shifted by dyz cells (a multiple of nx)
+        auto uoff = [&](int slot, int64_t dyz) -> int64_t {
+            return (h->bsh ? (int64_t)slot << h->bsh : slot * h->ns) + (dyz >> h->bsh) * a.lstride;
+        };
+        for (int q = 0; q < Q; q++) {
+            const int64_t dyz = (int64_t)h->nx * (EY[q] + (int64_t)h->ny * EZ[q]);
+            int ls = q, ss = q;
+            int64_t ld = -dyz, sd = 0;
+            if (ESO && q) {
+                if (q & 1) { ls = ph ? q : q + 1; ld = 0; ss = nph ? q : q + 1; sd = dyz; }
+                else { ls = ph ? q : q - 1; ss = nph ? q : q - 1; }
+            }
+            a.lp[q] = (const S *)((uintptr_t)a.F + (uintptr_t)(uoff(ls, ld) * (int64_t)sizeof(S)));
+            a.sp[q] = (S *)((uintptr_t)a.G + (uintptr_t)(uoff(ss, sd) * (int64_t)sizeof(S)));
+        }
+    }
     const int bx = row_block(h->nx);
     const int gx = (h->nx + bx - 1) / bx;
     // blocks of the z rows that can hold body links write partials

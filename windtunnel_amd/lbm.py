@@ -46,7 +46,8 @@ class BodyStats(C.Structure):
 
 
 class State(C.Structure):
-    _fields_ = [("parity", C.c_int64), ("steps", C.c_int64), ("force", C.c_double * 3)]
+    _fields_ = [("parity", C.c_int64), ("steps", C.c_int64), ("force", C.c_double * 3),
+                ("block_cells", C.c_int32), ("reserved", C.c_int32)]
 
 
 class VoxMeta(C.Structure):
@@ -208,6 +209,11 @@ class LBM:
     @property
     def steps(self) -> int:
         return int(self._state().steps)
+
+    @property
+    def block_cells(self) -> int:
+        """Cells per storage block of the population buffers (0: planar)."""
+        return int(self._state().block_cells)
 
     @property
     def force(self) -> np.ndarray:
