@@ -1,26 +1,43 @@
-"""Throughput of the D3Q19 step on one MI355X: MLUPS and fraction of the HBM roofline.
+"""Throughput of the D3Q19 step on one MI355X: MLUPS and fraction of the HBM
+roofline, fp32 against binary16-offset storage, plus the adjoint/forward cost.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2]
+                    [--no-cpu] [--no-f16] [--no-precision] [--no-adjoint]
+                    [--fd-cells 64]
 
-Workload (default): D3Q19 TRT (Lambda = 3/16) with Smagorinsky LES (Cs = 0.1),
-tau = 0.505, inflow u0 = 0.08, on a 512x256x256 lattice (33.5 M cells) around
-the synthetic bluff body scaled 2x (ellipsoid 80x28x24 semi-axes unioned
-with a 60x32x20 box). One step is one fused stream/collide/bounce-back/force
-launch over the whole lattice plus the per-step force reduction; the populations
-are resident in HBM before timing starts. The headline value is the float32
-storage path; the binary16-offset storage path runs on the same lattice and is
-reported beside it.
+Throughput workload (default): D3Q19 TRT (Lambda = 3/16) with Smagorinsky LES
+(Cs = 0.1), tau = 0.505, inflow u0 = 0.08, on a 512x256x256 lattice (33.5 M
+cells) around the synthetic bluff-body mesh scaled 2x (ellipsoid 40x14x12
+unioned with a 30x16x10 box, ~20k jittered triangles), voxelized on the GPU
+(Voxel.build at res 200, then the grid-backed body lookup). One step is one
+fused stream/collide/bounce-back/force launch over the whole lattice plus the
+per-step force reduction; the populations are resident in HBM before timing
+starts. The headline value is the float32 storage path; the binary16-offset
+storage path runs on the same lattice and is reported beside it.
 
-Roofline: HBM bound. Algorithmic bytes per cell update = 19 populations read +
-19 written = 152 B (float32) or 76 B (binary16); MLUPS counts all nx*ny*nz
-cells as the reference's benchmark does (js/main.js:621).
+Roofline: HBM bound. Algorithmic bytes per cell update = 19 populations read
++ 19 written = 152 B (float32) or 76 B (binary16); MLUPS counts all nx*ny*nz
+cells as the reference's benchmark does (js/main.js:621). `achieved` divides
+the algorithmic bytes of one launch by the average step-kernel duration,
+measured with HIP events on the lattice's stream over the timed steps.
+
+Precision leg: the same body at half scale on 128x64x64, 4000 steps in
+binary16, float32 and binary64 storage (the binary64 GPU path equals the
+oracle's binary64 run bit for bit); time-averaged C_d over the last 2000
+steps (pair-averaged, 16-block SEM) and field rel-L2 at the last step.
+
+Adjoint leg: 192x96x96 TRT (tau 0.6, u0 0.04) around a gray-walled ellipsoid
+(seeded smooth weights, 1 inside, 0 outside), 3000 steps with sqrt(N)
+checkpointing, objective J = mean C_d over the last 500 steps; cost ratio =
+(recompute + adjoint) device time per step / forward device time per step;
+central differences (eps 1e-3, binary64) on seeded gray cells.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import platform
+import re
 import sys
 import time
 
@@ -31,47 +48,60 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "D3Q19 MLUPS & % HBM roofline (fp32 vs fp16-store); adjoint/forward cost"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_hbm_traffic_512x256x256.md")
 
 WORKLOADS = {
-    # TRT+LES around the synthetic body; the 512^2x256 form is where the
-    # throughput targets are quoted, 256x128x128 is its half-scale sibling.
+    # TRT+LES around the synthetic body; 512x256x256 is where the throughput
+    # targets are quoted, 256x128x128 is its half-scale sibling.
     "c3": dict(dims=(512, 256, 256), scale=2, tau=0.505, u0=0.08, lam=3 / 16, csm=0.1),
     "c2": dict(dims=(256, 128, 128), scale=1, tau=0.505, u0=0.08, lam=3 / 16, csm=0.1),
 }
 
 
-def body_mask(dims, scale):
-    """Ellipsoid (40,14,12)*s semi-axes unioned with a (30,16,10)*s box centred
-    (22,-6,0)*s from the ellipsoid centre; centre at (0.3 nx, ny/2, nz/2).
-    Point test at cell centres over the clamped box, x fastest."""
-    nx, ny, nz = dims
-    s = scale
-    c = np.array([0.3 * nx, ny / 2, nz / 2])
-    ex, ey, ez = 40 * s, 14 * s, 12 * s
-    bc = c + np.array([22 * s, -6 * s, 0])
-    bh = np.array([15 * s, 8 * s, 5 * s])
-    lo = np.floor(np.minimum(c - [ex, ey, ez], bc - bh)).astype(int) - 1
-    hi = np.ceil(np.maximum(c + [ex, ey, ez], bc + bh)).astype(int) + 1
-    lo = np.maximum(lo, 1)
-    hi = np.minimum(hi, np.array(dims) - 2)
-    z, y, x = np.meshgrid(np.arange(lo[2], hi[2] + 1), np.arange(lo[1], hi[1] + 1),
-                          np.arange(lo[0], hi[0] + 1), indexing="ij")
-    X, Y, Z = x + 0.5, y + 0.5, z + 0.5
-    a, b, cc = (X - c[0]) / ex, (Y - c[1]) / ey, (Z - c[2]) / ez
-    ins = a * a + b * b + cc * cc <= 1
-    ins |= (np.abs(X - bc[0]) <= bh[0]) & (np.abs(Y - bc[1]) <= bh[1]) & (np.abs(Z - bc[2]) <= bh[2])
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------ bodies
+_MESH = {}
+
+
+def body_mesh():
+    """The ~20k-triangle synthetic bluff body, in cells around its centre."""
+    if "P" not in _MESH:
+        from windtunnel_amd.synth import ellipsoid_box
+        _MESH["P"] = ellipsoid_box()
+    return _MESH["P"]
+
+
+def place_mesh_body(L, scale: float, res: int = 200) -> dict:
+    """Voxelize the body mesh on the GPU (Voxel.build) and put it into the
+    lattice at (0.3 nx, ny/2, nz/2), `scale` cells per mesh unit, through the
+    grid-backed body lookup over its box grown by 2 cells."""
+    from windtunnel_amd.lbm import Voxel
+    t0 = time.perf_counter()
+    vox = Voxel.build(body_mesh(), res)
+    pos = np.array([0.3 * L.nx, L.ny / 2, L.nz / 2])
+    b = np.array(vox["bbox"])
+    lo = np.floor(pos + scale * b[:3]).astype(int) - 2
+    hi = np.ceil(pos + scale * b[3:]).astype(int) + 2
     box = [int(lo[0]), int(lo[1]), int(lo[2]), int(hi[0]), int(hi[1]), int(hi[2])]
-    return box, ins.astype(np.uint8).reshape(-1)
+    L.voxelize_grid(vox, np.eye(3).reshape(-1), pos, scale, L.clamp_box(box))
+    return dict(voxelize_s=time.perf_counter() - t0, grid=[vox["nx"], vox["ny"], vox["nz"]],
+                tris=vox["stats"]["tris"], openness=vox["stats"]["openness"])
 
 
+def set_flow(L, wl) -> None:
+    L.set_params(tau=wl["tau"], les=True, csm=wl["csm"], trt=True, u0=wl["u0"], **{"lambda": wl["lam"]})
+
+
+# -------------------------------------------------------------- throughput
 def run_gpu(wl, storage, steps, warmup):
     from windtunnel_amd.lbm import LBM
-    dims = wl["dims"]
-    L = LBM(*dims, storage=storage)
-    L.set_params(tau=wl["tau"], les=True, csm=wl["csm"], trt=True, u0=wl["u0"], **{"lambda": wl["lam"]})
+    L = LBM(*wl["dims"], storage=storage)
+    set_flow(L, wl)
     L.reset()
-    box, mask = body_mask(dims, wl["scale"])
-    L.voxelize_mask(box, mask)
+    vinfo = place_mesh_body(L, wl["scale"])
     L.step(warmup)
     L.set_timing(True)
     t0 = time.perf_counter()
@@ -84,26 +114,182 @@ def run_gpu(wl, storage, steps, warmup):
     kavg = kms / launches
     res = dict(
         mlups=n * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps, kernel_ms=kavg,
-        achieved_gbs=n * bpc / (kavg * 1e-3) / 1e9, bytes_per_cell=bpc,
-        solid_cells=L.solidCells, frontal_area=L.frontalArea,
+        achieved_gbs=n * bpc / (kavg * 1e-3) / 1e9, bytes_per_cell=bpc, block_cells=L.block_cells,
+        solid_cells=L.solidCells, frontal_area=L.frontalArea, body=vinfo,
         finite=bool(np.all(np.isfinite(forces))),
-        cd_mean=float(np.mean(forces[:, 0]) / (0.5 * wl["u0"] ** 2 * L.frontalArea)),
     )
     L.close()
     return res
+
+
+def measured_traffic():
+    """HBM bytes per step launch from the committed rocprofv3 PMC summary
+    (lines `traffic[f32] = <bytes>`), or None per storage type."""
+    out = {}
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for m in re.finditer(r"traffic\[(\w+)\]\s*=\s*([0-9.eE+]+)", fh.read()):
+                out[m.group(1)] = float(m.group(2))
+    except OSError:
+        pass
+    return out
+
+
+# --------------------------------------------------------------- precision
+def run_precision(steps=4000, avg=2000):
+    """binary16 / float32 / binary64 storage on 128x64x64, same body at half scale."""
+    from windtunnel_amd.lbm import LBM
+    from windtunnel_amd.stats import coefficient_stats
+    wl = dict(dims=(128, 64, 64), scale=0.5, tau=0.505, u0=0.08, lam=3 / 16, csm=0.1)
+    out = {}
+    for storage, arith in (("f64", "f64"), ("f32", "f32"), ("f16", "f32")):
+        L = LBM(*wl["dims"], storage=storage, arith=arith)
+        set_flow(L, wl)
+        L.reset()
+        place_mesh_body(L, wl["scale"])
+        F = L.step(steps)
+        st = coefficient_stats(F[-avg:], wl["u0"], L.frontalArea, settle=0)
+        fld = {k: np.asarray(v, dtype=np.float64).copy() for k, v in L.fields().items()}
+        out[storage] = dict(cd=st["cd"]["mean"], cd_sem=st["cd"]["sem"], cl=st["cl"]["mean"], fields=fld,
+                            finite=bool(np.all(np.isfinite(F))))
+        L.close()
+    ref = out["f64"]
+
+    def rl2(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    def speed(f):
+        return np.sqrt(f["ux"] ** 2 + f["uy"] ** 2 + f["uz"] ** 2)
+    res = dict(grid="128x64x64", steps=steps, cd_window=avg, cd_fp64=ref["cd"], cd_sem_fp64=ref["cd_sem"])
+    for s in ("f32", "f16"):
+        r = out[s]
+        res[s] = dict(cd=r["cd"], cd_sem=r["cd_sem"], cd_rel_err=abs(r["cd"] - ref["cd"]) / abs(ref["cd"]),
+                      rho_rel_l2=rl2(r["fields"]["rho"] - 1, ref["fields"]["rho"] - 1),
+                      speed_rel_l2=rl2(speed(r["fields"]), speed(ref["fields"])), finite=r["finite"])
+    return res
+
+
+# ----------------------------------------------------------------- adjoint
+def gray_ellipsoid(dims, seed=4):
+    """Gray weights of the differentiation workload: an ellipsoid of semi-axes
+    (24, 9, 8) at (0.3 nx, ny/2, nz/2) whose wall is a ~3-cell band where the
+    weight falls from 1 to 0, displaced by a seeded smooth field."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5, indexing="ij")
+    c = (0.3 * nx, ny / 2, nz / 2)
+    s = np.sqrt(((x - c[0]) / 24) ** 2 + ((y - c[1]) / 9) ** 2 + ((z - c[2]) / 8) ** 2)
+    phi = np.zeros_like(s)
+    for _ in range(4):
+        k = rng.normal(scale=0.25, size=3)
+        phi += np.sin(k[0] * x + k[1] * y + k[2] * z + rng.uniform(0, 2 * np.pi)) / 4
+    d = (s - 1) * 8 + 0.8 * phi
+    return np.clip(0.5 - d / 3, 0, 1).reshape(-1).astype(np.float32)
+
+
+def run_adjoint(fd_cells=64, N=3000, K=500, eps=1e-3):
+    from windtunnel_amd.lbm import LBM
+    dims = (192, 96, 96)
+    w = gray_ellipsoid(dims)
+    solid = (w >= 1).reshape(dims[::-1])
+    idx = np.argwhere(solid)
+    lo, hi = idx.min(0), idx.max(0)
+    box = [int(lo[2]), int(lo[1]), int(lo[0]), int(hi[2]), int(hi[1]), int(hi[0])]
+    mask = solid[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1].astype(np.uint8).reshape(-1)
+    wg = np.where(w >= 1, 0, w).astype(np.float32)
+
+    def lattice(storage, arith):
+        L = LBM(*dims, storage=storage, arith=arith)
+        L.set_params(tau=0.6, les=False, trt=True, u0=0.04, **{"lambda": 3 / 16})
+        L.reset()
+        L.voxelize_mask(box, mask)
+        return L
+
+    def prepare(L, weights):
+        L.reset()
+        L.set_gray(weights)
+
+    res = dict(grid="192x96x96", steps=N, objective=f"mean C_d over the last {K} steps", gray_cells=int(
+        np.count_nonzero((wg > 0) & (wg < 1))))
+    # float32: forward device time per step, then recorded forward + reverse
+    L = lattice("f32", "f32")
+    q = 0.5 * 0.04 ** 2 * L.frontalArea
+    Fbar = np.zeros((N, 3))
+    Fbar[N - K:, 0] = 1.0 / (K * q)
+    prepare(L, wg)
+    fwd_ms = L.time_steps(200) / 200
+    prepare(L, wg)
+    t0 = time.perf_counter()
+    F32 = L.adjoint_forward(N)
+    fwd_wall = time.perf_counter() - t0
+    g32, rev_ms = L.adjoint_backward(Fbar)
+    L.adjoint_release()
+    L.close()
+    res.update(forward_ms_per_step=fwd_ms, reverse_ms_per_step=rev_ms / N,
+               cost_ratio=(rev_ms / N) / fwd_ms, recorded_forward_ms_per_step=fwd_wall * 1e3 / N,
+               J_fp32=float(np.sum(Fbar * F32)))
+    log(f"adjoint fp32: forward {fwd_ms:.4f} ms/step, reverse {rev_ms / N:.4f} ms/step, "
+        f"ratio {res['cost_ratio']:.2f}")
+    # binary64: adjoint and central differences
+    D = lattice("f64", "f64")
+    prepare(D, wg)
+    F64 = D.adjoint_forward(N)
+    g64, _ = D.adjoint_backward(Fbar)
+    D.adjoint_release()
+    gray = np.flatnonzero((wg > 0) & (wg < 1))
+    res["J_fp64"] = float(np.sum(Fbar * F64))
+    res["grad_fp32_vs_fp64_rel_l2"] = float(np.linalg.norm(g32[gray] - g64[gray]) / np.linalg.norm(g64[gray]))
+    if fd_cells > 0:
+        ok = np.flatnonzero((wg > 2 * eps) & (wg < 1 - 2 * eps))
+        cells = np.random.default_rng(4).choice(ok, min(fd_cells, ok.size), replace=False)
+        errs, fds = [], []
+        t0 = time.perf_counter()
+        for c in cells:
+            J, wv = [], []
+            for sgn in (1, -1):
+                wp = wg.copy()
+                wp[c] = np.float32(wg[c] + sgn * eps)
+                wv.append(float(wp[c]))
+                prepare(D, wp)
+                J.append(float(np.sum(D.step(N) * Fbar)))
+            fd = (J[0] - J[1]) / (wv[0] - wv[1])
+            fds.append(fd)
+            errs.append(abs(g64[c] - fd))
+        fds, errs = np.array(fds), np.array(errs)
+        floor = 1e-3 * np.max(np.abs(fds))
+        rel = errs / np.maximum(np.abs(fds), floor)
+        res["fd"] = dict(cells=int(cells.size), eps=eps, max_rel_err=float(rel.max()),
+                         median_rel_err=float(np.median(rel)), within_1e3=int(np.sum(rel <= 1e-3)),
+                         seconds=round(time.perf_counter() - t0, 1))
+        log(f"adjoint FD: {res['fd']}")
+    D.close()
+    return res
+
+
+# --------------------------------------------------------------------- CPU
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def run_cpu(wl, threads):
     """The oracle (C restatement of js/lbm.js with the worker-pool slab split
     of js/lbmWorker.js) on the host cores: 2 warm-up steps, then best of 3 x 2
     steps on the full lattice, the reference benchmark's best-of protocol
-    (js/main.js:614-620) shortened to bound the run time."""
+    (js/main.js:614-620) shortened to bound the run time. The body is the
+    same mesh body: its solid map is taken from the GPU lattice."""
     from oracle.oracle import OracleLBM
     dims = wl["dims"]
+    box, mask = _cpu_mask(dims, wl["scale"])
     O = OracleLBM(*dims)
     O.set_params(tau=wl["tau"], les=True, csm=wl["csm"], trt=True, u0=wl["u0"], **{"lambda": wl["lam"]})
     O.reset()
-    box, mask = body_mask(dims, wl["scale"])
     O.voxelize_mask(box, mask)
     O.step_pool(2, threads, threaded=True)
     best = float("inf")
@@ -116,6 +302,19 @@ def run_cpu(wl, threads):
     return n / best / 1e6
 
 
+def _cpu_mask(dims, scale):
+    """Solid mask of the mesh body as the GPU lattice holds it (downloaded)."""
+    from windtunnel_amd.lbm import LBM
+    L = LBM(*dims, storage="f16")
+    place_mesh_body(L, scale)
+    s = np.asarray(L.solid).reshape(dims[::-1])
+    L.close()
+    idx = np.argwhere(s == 1)
+    lo, hi = idx.min(0), idx.max(0)
+    box = [int(lo[2]), int(lo[1]), int(lo[0]), int(hi[2]), int(hi[1]), int(hi[0])]
+    return box, (s[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] == 1).astype(np.uint8).reshape(-1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +323,9 @@ def main():
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--fd-cells", type=int, default=64, help="central-difference cells of the adjoint leg (0: none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy three times (HBM counter calibration)")
     a = ap.parse_args()
@@ -137,43 +339,56 @@ def main():
         print("copied 3 x 512 MiB", flush=True)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only; the multi-GPU path (z-slab halo exchange) is not built",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: the z-slab halo exchange that would shard the lattice "
+              "over several GPUs is not built", file=sys.stderr)
         sys.exit(2)
     wl = WORKLOADS[a.workload]
     nx, ny, nz = wl["dims"]
-    print(f"[bench] {a.workload} {nx}x{ny}x{nz} f32 warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+    traffic = measured_traffic()
+    log(f"{a.workload} {nx}x{ny}x{nz} f32 warmup={a.warmup} steps={a.steps}")
     r32 = run_gpu(wl, "f32", a.steps, a.warmup)
-    print(f"[bench] f32 {r32['mlups']:.0f} MLUPS kernel {r32['kernel_ms']:.3f} ms", file=sys.stderr, flush=True)
+    log(f"f32 {r32['mlups']:.0f} MLUPS kernel {r32['kernel_ms']:.4f} ms")
     out = {
         "metric": METRIC, "value": round(r32["mlups"], 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(r32["ms_per_step"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
+        "data": "synthetic (seeded mesh body, reset() initial state)",
         "config": {"workload": f"D3Q19 TRT(Lambda=3/16)+LES(Cs=0.1) {nx}x{ny}x{nz}, tau=0.505, u0=0.08, "
-                               f"synthetic ellipsoid+box body x{wl['scale']} (analytic point test)",
+                               f"synthetic ellipsoid+box mesh body x{wl['scale']} (GPU voxelized, res 200)",
                    "nx": nx, "ny": ny, "nz": nz, "storage": "f32", "arith": "f32", "layout": "ab",
-                   "solid_cells": r32["solid_cells"]},
+                   "block_cells": r32["block_cells"], "solid_cells": r32["solid_cells"],
+                   "frontal_area": r32["frontal_area"], "body": r32["body"]},
         "roofline": {"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel_ms": round(r32["kernel_ms"], 5), "bytes_per_cell": 152},
+                     "unit": "GB/s", "frac": round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic.get("f32"), "kernel_ms": round(r32["kernel_ms"], 5),
+                     "bytes_per_cell": 152, "bytes_per_launch": nx * ny * nz * 152},
     }
     if not a.no_f16:
-        print("[bench] f16", file=sys.stderr, flush=True)
+        log("f16")
         r16 = run_gpu(wl, "f16", a.steps, a.warmup)
+        log(f"f16 {r16['mlups']:.0f} MLUPS kernel {r16['kernel_ms']:.4f} ms")
         out["fp16"] = {"value": round(r16["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r16["ms_per_step"], 4),
-                       "speedup_vs_fp32": round(r16["mlups"] / r32["mlups"], 3),
+                       "speedup_vs_fp32": round(r16["mlups"] / r32["mlups"], 3), "block_cells": r16["block_cells"],
                        "roofline": {"bound": "hbm", "achieved": round(r16["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                                    "kernel_ms": round(r16["kernel_ms"], 5), "bytes_per_cell": 76}}
+                                    "traffic": traffic.get("f16"), "kernel_ms": round(r16["kernel_ms"], 5),
+                                    "bytes_per_cell": 76, "bytes_per_launch": nx * ny * nz * 76}}
+    if not a.no_precision:
+        log("precision 128x64x64, 4000 steps, f64/f32/f16 storage")
+        out["precision"] = run_precision()
+        log(f"precision: {json.dumps(out['precision'])}")
+    if not a.no_adjoint:
+        log("adjoint 192x96x96, 3000 steps")
+        out["adjoint"] = run_adjoint(a.fd_cells)
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        print(f"[bench] cpu baseline, {threads} threads", file=sys.stderr, flush=True)
+        log(f"cpu baseline, {threads} threads")
         v = run_cpu(wl, threads)
         out["cpu_baseline"] = {"value": round(v, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-                               "sample": f"oracle (C restatement of js/lbm.js, binary64 arithmetic) on the full "
-                                         f"{nx}x{ny}x{nz} lattice, {threads} z-slab threads, best of 3x2 steps "
-                                         f"after 2 warm-up steps; host {platform.processor() or platform.machine()}"}
+                               "sample": f"oracle (C restatement of js/lbm.js, binary64 arithmetic, float32 "
+                                         f"storage) on the full {nx}x{ny}x{nz} lattice with the same body, "
+                                         f"{threads} z-slab threads, best of 3x2 steps after 2 warm-up steps; "
+                                         f"host {cpu_model()}"}
     print(json.dumps(out), flush=True)
 
 

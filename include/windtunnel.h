@@ -76,11 +76,14 @@ typedef struct {
 } wt_body_stats;
 
 /* Observable scalar state of an LBM object (js/lbm.js:80,113,117), plus
- * the device storage layout: block_cells = cells per storage block of the
- * population buffers (0 = one plane per direction). */
+ * its geometry and types as created, and the device storage layout:
+ * block_cells = cells per storage block of the population buffers (0 = one
+ * plane per direction). */
 typedef struct {
     int64_t parity, steps;
     double force[3];
+    int32_t nx, ny, nz;
+    int32_t storage, arith, layout;
     int32_t block_cells, reserved;
 } wt_state_t;
 
@@ -129,6 +132,20 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
 /* step() nsteps times (js/lbm.js:285-291). forces (may be NULL) receives the
  * 3 force components after each step, FP64, nsteps*3 doubles. */
 int wt_step(wt_lattice *h, int nsteps, double *forces);
+
+/* stepSlab(z0,z1) (js/lbm.js:293-465): cells of the z rows [z0,z1),
+ * 1 <= z0 <= z1 <= nz-1, pull from the current populations and write the
+ * next ones; the x/y boundary planes and the outlet copy of those rows are
+ * written in the same launch (the step is fused), so _boundaries only has
+ * the z = 0 and z = nz-1 planes left. No parity flip and no step count, as
+ * in the reference; force receives the slab's force. */
+int wt_step_slab(wt_lattice *h, int z0, int z1, double force[3]);
+
+/* _boundaries(g, z0, z1, doLo, doHi) (js/lbm.js:467-512) after wt_step_slab:
+ * writes the z = 0 plane when doLo and the z = nz-1 plane when doHi, with
+ * their outlet copies; the planes of the interior rows of [z0,z1) came with
+ * wt_step_slab. */
+int wt_boundaries(wt_lattice *h, int z0, int z1, int doLo, int doHi);
 
 /* Worker-pool form (js/pool.js:97-125, js/lbmWorker.js:43-79): the
  * interior is split into nslabs z-slabs as the worker 'init' handler does
@@ -187,6 +204,20 @@ int wt_set_gray(wt_lattice *h, const float *w);
  * no moving walls. */
 int wt_adjoint(wt_lattice *h, int nsteps, const double *Fbar, double *dJdw, double *forces, int interval,
                double *reverse_ms);
+
+/* The same as a forward/backward pair (torch.autograd.Function semantics;
+ * no reference counterpart). wt_adjoint_forward runs nsteps from the current
+ * state like wt_step and records sqrt(N) checkpoints on the lattice
+ * (ceil(nsteps/interval) states in device memory); forces may be NULL.
+ * wt_adjoint_backward takes the cotangent Fbar of the forward's forces
+ * [nsteps][3] (nsteps must be the recorded count) and returns dJ/dw; it
+ * leaves the lattice's state, parity and step count alone and may be called
+ * any number of times. It fails with WT_E_STATE when no forward was
+ * recorded or when parameters, body or weights changed since. wt_adjoint_release frees the record (so do the
+ * next forward and wt_destroy). */
+int wt_adjoint_forward(wt_lattice *h, int nsteps, int interval, double *forces);
+int wt_adjoint_backward(wt_lattice *h, int nsteps, const double *Fbar, double *dJdw, double *reverse_ms);
+int wt_adjoint_release(wt_lattice *h);
 
 /* Voxel.bounds (js/voxel.js:32-42) over len floats. */
 int wt_voxel_bounds(const float *P, int64_t len, double out[6]);

@@ -260,6 +260,48 @@ if (want('lbm_c1')) {
   console.log(`  lbm_c1: ${((Date.now() - t0) / 1000).toFixed(1)} s, ${(tStep / 2000).toFixed(2)} ms/step`);
 }
 
+if (want('lbm_c2')) {
+  // TRT (Lambda = 3/16) + LES (Cs = 0.1) wake of the synthetic mesh body
+  // through the app's own mesh pipeline: Shapes.addModel (orient to unit
+  // length, Voxel.build at res 200) and the grid-backed Body.makeTest, at
+  // scale = 1/k so the body keeps its size in cells. 256x128x128, 5000 steps.
+  const t0 = Date.now();
+  const NX = 256, NY = 128, NZ = 128, NSTEPS = 5000;
+  const P0 = synth.ellipsoidBox();
+  const P2 = P0.slice();
+  const k = Shapes.orient(P2, 0, 1, true, false);
+  const def = Shapes.addModel('c2body', 'c2body', P0.slice(), { forward: 0, up: 1, flip: true, ground: false, res: 200 });
+  const L = new LBM(NX, NY, NZ);
+  L.trt = true; L.lambda = 3 / 16; L.les = true; L.csm = 0.1; L.tau = 0.505; L.u0 = 0.08;
+  L.reset();
+  const b = new Shapes.Body('c2body');
+  b.scale = 1 / k;
+  b.pos = [0.3 * NX, NY / 2, NZ / 2];
+  b._updateMat();
+  const vox = voxelizeRecorded(L, b.makeTest(), b.box(), false);
+  vox.body = { scale: b.scale, pos: b.pos, M: b.M, ext: b.ext };
+  const v = def.vox;
+  const forces = new Float64Array(NSTEPS * 3);
+  const snaps = {};
+  let tStep = 0;
+  for (let s = 0; s < NSTEPS; s++) {
+    const ts = Date.now();
+    L.step();
+    tStep += Date.now() - ts;
+    forces.set(L.force, 3 * s);
+    if ([1, 100, 1000, NSTEPS].includes(L.steps)) snaps[L.steps] = stateHashes(L);
+    if (L.steps % 500 === 0) console.log(`  lbm_c2: step ${L.steps}, ${(tStep / L.steps).toFixed(1)} ms/step`);
+  }
+  write('lbm_c2', { dims: [NX, NY, NZ], params: params(L), nsteps: NSTEPS,
+    mesh: { sha256: sha(P0), ntri: P0.length / 9, oriented_sha256: sha(def.tris), k,
+      addModel: { forward: 0, up: 1, flip: true, ground: false, res: 200 } },
+    vox: { nx: v.nx, ny: v.ny, nz: v.nz, cell: v.cell, min: v.min, inv: v.inv, ext: v.ext,
+      grid_sha256: sha(v.grid), stats: v.stats },
+    voxelize: vox, forces: pack(forces, [NSTEPS, 3]), snaps,
+    node_ms_per_step: tStep / NSTEPS, node_version: process.version });
+  console.log(`  lbm_c2: ${((Date.now() - t0) / 1000).toFixed(1)} s, ${(tStep / NSTEPS).toFixed(2)} ms/step`);
+}
+
 // test.html procedure on two of its bluff bodies (js body via Shapes, replayed as a mask)
 for (const shape of ['cube', 'sphere']) {
   const name = 'lbm_testpage_' + shape;

@@ -76,6 +76,59 @@ check('worker pool protocol reproduces lbm_pool3', async () => {
   assert(L.parity === fx.pooled.parity && L.steps === fx.pooled.steps, 'parity/steps');
 });
 
+check('lbmWorker.js step loop through stepSlab/_boundaries reproduces lbm_pool3', () => {
+  const fx = load('lbm_pool3');
+  const L = new LBM(...fx.dims, { arith: 'f64' });
+  applyParams(L, fx.worker_params);
+  L.reset();
+  replay(L, fx.voxelize);
+  const nz = fx.dims[2], NW = fx.nWorkers, want = arr(fx.forces);
+  for (let s = 0; s < fx.nsteps; s++) {
+    const tot = [0, 0, 0];
+    for (let id = 0; id < NW; id++) {
+      const z0 = 1 + Math.floor((nz - 2) * id / NW), z1 = 1 + Math.floor((nz - 2) * (id + 1) / NW);
+      const bz0 = id === 0 ? 0 : z0, bz1 = id === NW - 1 ? nz : z1;
+      L.stepSlab(z0, z1);
+      L._boundaries(L.g, bz0, bz1, id === 0, id === NW - 1);
+      for (let k = 0; k < 3; k++) tot[k] += L._sf[k];
+    }
+    L.parity ^= 1;
+    close(tot, want.subarray(3 * s, 3 * s + 3), 1e-11, 1e-14, 'slab force ' + s);
+  }
+  for (const k of ['rho', 'ux', 'uy', 'uz']) assert(sha(L[k]) === fx.pooled[k], k);
+  assert(L.idx(3, 2, 1) === 3 + L.nx * (2 + L.ny), 'idx');
+});
+
+check('gray adjoint pair: backward is linear in Fbar and repeatable', () => {
+  const L = new LBM(20, 12, 12);
+  L.tau = 0.6; L.u0 = 0.05; L.les = false; L.trt = true; L.lambda = 3 / 16;
+  L.reset();
+  L.voxelize((x, y, z) => (x - 7) ** 2 + (y - 6) ** 2 + (z - 6) ** 2 <= 4, { x0: 4, y0: 3, z0: 3, x1: 10, y1: 9, z1: 9 }, false);
+  const w = new Float32Array(L.n);
+  for (let z = 0; z < 12; z++) for (let y = 0; y < 12; y++) for (let x = 0; x < 20; x++) {
+    const r = Math.sqrt((x + 0.5 - 7) ** 2 + (y + 0.5 - 6) ** 2 + (z + 0.5 - 6) ** 2);
+    const i = L.idx(x, y, z);
+    w[i] = L.solid[i] ? 0 : Math.min(0.99, Math.max(0, (3.6 - r) / 1.6));
+  }
+  L.setGray(w);
+  const N = 30, f = L.adjointForward(N, 5);
+  assert(f.length === 3 * N && L.steps === N, 'forward');
+  const Fb = new Float64Array(3 * N);
+  for (let k = N - 10; k < N; k++) Fb[3 * k] = 0.1;
+  const a = L.adjointBackward(Fb).dJdw, b = L.adjointBackward(Fb.map(v => 2 * v)).dJdw;
+  let nz = 0, worst = 0;
+  for (let i = 0; i < L.n; i++) {
+    if (a[i] !== 0) nz++;
+    worst = Math.max(worst, Math.abs(b[i] - 2 * a[i]));
+  }
+  assert(nz > 0, 'gradient is zero everywhere');
+  assert(worst <= 1e-6 * Math.max(...a.map(Math.abs)), 'linearity ' + worst);
+  let threw = false;
+  try { L.adjointBackward(new Float64Array(3)); } catch (e) { threw = /Fbar/.test(e.message); }
+  L.adjointRelease();
+  assert(threw, 'short Fbar must be refused');
+});
+
 check('Voxel.build is bit-exact (vox_box, vox_hitcap, vox_sphere)', () => {
   for (const name of ['vox_box', 'vox_hitcap', 'vox_sphere']) {
     const fx = load(name);

@@ -93,3 +93,43 @@ def test_adjoint_against_central_differences():
             L.close()
         fd = (J[0] - J[1]) / (wv[0] - wv[1])
         assert abs(g[c] - fd) <= 1e-3 * abs(fd) + 1e-6 * np.abs(g).max(), (c, g[c], fd)
+
+
+def test_forward_backward_pair_and_autograd():
+    """wt_adjoint_forward / wt_adjoint_backward equal the fused wt_adjoint,
+    backward repeats and is linear in Fbar, leaves the lattice's state
+    alone, refuses a changed configuration; the torch.autograd.Function
+    gives the same gradient."""
+    import torch
+
+    from windtunnel_amd.autograd import gray_forces
+    from windtunnel_amd.lbm import WindtunnelError
+
+    N = 40
+    Fb = fbar(N)
+    A = setup(LBM(*DIMS), DIMS, W, True, True)
+    fa, ga, _ = A.adjoint(N, Fb, interval=6)
+    B = setup(LBM(*DIMS), DIMS, W, True, True)
+    fb = B.adjoint_forward(N, interval=6)
+    f_state = B.f.copy()
+    g1, _ = B.adjoint_backward(Fb)
+    g2, _ = B.adjoint_backward(2 * Fb)
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(ga, g1)
+    np.testing.assert_allclose(g2, 2 * g1, rtol=1e-6, atol=1e-12 * np.abs(g1).max())
+    assert np.array_equal(B.f, f_state) and B.steps == N
+    with pytest.raises(ValueError):
+        B.adjoint_backward(Fb[:-1])
+    B.tau = 0.61
+    with pytest.raises(WindtunnelError) as e:
+        B.adjoint_backward(Fb)
+    assert e.value.code == -4
+    B.adjoint_release()
+
+    C = setup(LBM(*DIMS), DIMS, W, True, True)
+    w = torch.tensor(np.where(W >= 1, 0, W), dtype=torch.float32, requires_grad=True)
+    F = gray_forces(w, C, N, 6)
+    J = (F * torch.from_numpy(Fb)).sum()
+    J.backward()
+    np.testing.assert_allclose(F.detach().numpy(), fa, rtol=0, atol=0)
+    np.testing.assert_allclose(w.grad.numpy(), ga.astype(np.float32), rtol=1e-6, atol=0)

@@ -213,6 +213,34 @@ def test_pool_slabs_exact(block_shift):
     np.testing.assert_allclose(tot, arr(fx["forces"]), rtol=1e-11, atol=1e-14)
 
 
+@pytest.mark.parametrize("layout", ["ab", "esoteric"])
+def test_worker_protocol_through_step_slab(layout):
+    """The lbmWorker.js step loop (js/lbmWorker.js:59-79) driven through the
+    solver-object calls: per worker stepSlab(z0,z1) then its share of
+    _boundaries, parity flip after all workers; fields equal the reference's
+    pooled run and the per-step force is the worker-ordered sum."""
+    fx = load("lbm_pool3")
+    L = LBM(*fx["dims"], arith="f64", layout=layout)
+    set_params(L, fx["worker_params"])
+    L.reset()
+    replay_voxelize(L, fx["voxelize"])
+    nz, NW = fx["dims"][2], fx["nWorkers"]
+    lo, hi = 1, nz - 1
+    got = np.zeros((fx["nsteps"], 3))
+    for s in range(fx["nsteps"]):
+        for w in range(NW):
+            z0 = lo + (hi - lo) * w // NW
+            z1 = lo + (hi - lo) * (w + 1) // NW
+            bz0 = 0 if w == 0 else z0
+            bz1 = nz if w == NW - 1 else z1
+            got[s] += L.step_slab(z0, z1)
+            L.boundaries(bz0, bz1, w == 0, w == NW - 1)
+        L.parity = L.parity ^ 1
+    for k in FIELDS:
+        assert sha(L.fields()[k]) == fx["pooled"][k], k
+    np.testing.assert_allclose(got, arr(fx["forces"]), rtol=1e-11, atol=1e-14)
+
+
 def test_fp16_storage_tracks_fp64():
     """binary16 deviations with float32 collision against the FP64 oracle."""
     fx = load("lbm_trt_les")

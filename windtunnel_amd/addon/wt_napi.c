@@ -130,6 +130,30 @@ static void finalize(napi_env env, void *data, void *hint)
     wt_destroy((wt_lattice *)data);
 }
 
+/* Lattice geometry for argument checks: cells, and bytes per field value
+ * and per population value as wt_get_fields / wt_get_pops write them. */
+static int lattice_info(napi_env env, wt_lattice *h, size_t *cells, size_t *fbytes)
+{
+    wt_state_t st;
+    if (wt_get_state(h, &st) != 0) {
+        napi_throw_error(env, NULL, wt_last_error());
+        return 0;
+    }
+    *cells = (size_t)st.nx * (size_t)st.ny * (size_t)st.nz;
+    *fbytes = st.storage == WT_STORE_F64 ? 8 : 4;
+    return 1;
+}
+
+/* true when a typed array of `have` bytes holds at least `need` bytes; throws otherwise */
+static int enough(napi_env env, size_t have, size_t need, const char *what)
+{
+    if (have >= need) return 1;
+    char msg[160];
+    snprintf(msg, sizeof msg, "%s: typed array of %zu bytes, %zu needed", what, have, need);
+    napi_throw_range_error(env, NULL, msg);
+    return 0;
+}
+
 /* create(nx, ny, nz, storage, arith, layout) -> handle */
 static napi_value js_create(napi_env env, napi_callback_info info)
 {
@@ -211,10 +235,19 @@ static napi_value js_voxelize(napi_env env, napi_callback_info info)
     const int *box = tdata(env, a[1], &nb, &ok);
     if (!ok) return NULL;
     if (!box || nb < 24) { napi_throw_type_error(env, NULL, "box must be an Int32Array(6)"); return NULL; }
-    const uint8_t *mask = tdata(env, a[2], NULL, &ok);
+    size_t mb = 0, wb = 0;
+    const uint8_t *mask = tdata(env, a[2], &mb, &ok);
     if (!ok) return NULL;
-    const float *wv = tdata(env, a[3], NULL, &ok);
+    const float *wv = tdata(env, a[3], &wb, &ok);
     if (!ok) return NULL;
+    if (box[3] >= box[0] && box[4] >= box[1] && box[5] >= box[2]) {
+        const size_t cnt = (size_t)(box[3] - box[0] + 1) * (size_t)(box[4] - box[1] + 1) * (size_t)(box[5] - box[2] + 1);
+        if (mask && !enough(env, mb, cnt, "voxelize mask")) return NULL;
+        size_t moving = 0;
+        if (mask)
+            for (size_t k = 0; k < cnt; k++) moving += mask[k] == 2;
+        if (moving && !enough(env, wv ? wb : 0, moving * 12, "voxelize wall velocities")) return NULL;
+    }
     wt_body_stats s;
     WT(env, wt_voxelize(h, box, mask, wv, boolish(env, a[4]), num(env, a[5], 0), &s));
     return stats_object(env, &s);
@@ -229,8 +262,10 @@ static napi_value js_voxelize_grid(napi_env env, napi_callback_info info)
     wt_lattice *h = handle(env, a[0]);
     if (!h) return NULL;
     int ok;
-    const uint8_t *grid = tdata(env, a[1], NULL, &ok);
+    size_t gb = 0;
+    const uint8_t *grid = tdata(env, a[1], &gb, &ok);
     if (!ok) return NULL;
+    if (!grid) gb = 0;
     wt_voxmeta vm;
     memset(&vm, 0, sizeof vm);
     vm.nx = (int)prop_num(env, a[2], "nx", 0);
@@ -245,12 +280,18 @@ static napi_value js_voxelize_grid(napi_env env, napi_callback_info info)
             if (napi_get_element(env, mn, k, &e) == napi_ok) vm.min[k] = num(env, e, 0);
         }
     }
-    const double *M = tdata(env, a[3], NULL, &ok);
+    size_t Mb = 0, pb = 0, bb = 0;
+    const double *M = tdata(env, a[3], &Mb, &ok);
     if (!ok) return NULL;
-    const double *pos = tdata(env, a[4], NULL, &ok);
+    const double *pos = tdata(env, a[4], &pb, &ok);
     if (!ok) return NULL;
-    const int *box = tdata(env, a[6], NULL, &ok);
+    const int *box = tdata(env, a[6], &bb, &ok);
     if (!ok) return NULL;
+    if (!enough(env, gb, (size_t)(vm.nx > 0 ? vm.nx : 0) * (size_t)(vm.ny > 0 ? vm.ny : 0) *
+                             (size_t)(vm.nz > 0 ? vm.nz : 0), "voxel grid") ||
+        !enough(env, M ? Mb : 0, 72, "rotation M") || !enough(env, pos ? pb : 0, 24, "position") ||
+        !enough(env, box ? bb : 0, 24, "box"))
+        return NULL;
     wt_body_stats s;
     WT(env, wt_voxelize_grid(h, grid, &vm, M, pos, num(env, a[5], 1), box, boolish(env, a[7]), num(env, a[8], 0), &s));
     return stats_object(env, &s);
@@ -290,6 +331,104 @@ static napi_value js_step_slabs(napi_env env, napi_callback_info info)
     return NULL;
 }
 
+/* stepSlab(h, z0, z1, force Float64Array(3) | null) */
+static napi_value js_step_slab(napi_env env, napi_callback_info info)
+{
+    napi_value a[4];
+    args(env, info, 4, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    size_t nb = 0;
+    double *f = tdata(env, a[3], &nb, &ok);
+    if (!ok) return NULL;
+    if (f && !enough(env, nb, 24, "stepSlab force")) return NULL;
+    WT(env, wt_step_slab(h, (int)num(env, a[1], 1), (int)num(env, a[2], 1), f));
+    return NULL;
+}
+
+/* boundaries(h, z0, z1, doLo, doHi) */
+static napi_value js_boundaries(napi_env env, napi_callback_info info)
+{
+    napi_value a[5];
+    args(env, info, 5, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    WT(env, wt_boundaries(h, (int)num(env, a[1], 0), (int)num(env, a[2], 0), boolish(env, a[3]), boolish(env, a[4])));
+    return NULL;
+}
+
+/* setGray(h, w Float32Array(n) | null) */
+static napi_value js_set_gray(napi_env env, napi_callback_info info)
+{
+    napi_value a[2];
+    args(env, info, 2, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    size_t nb;
+    const float *w = tdata(env, a[1], &nb, &ok);
+    if (!ok) return NULL;
+    size_t cells, fb;
+    if (!lattice_info(env, h, &cells, &fb)) return NULL;
+    if (w && !enough(env, nb, cells * 4, "setGray weights")) return NULL;
+    WT(env, wt_set_gray(h, w));
+    return NULL;
+}
+
+/* adjointForward(h, n, interval, forces Float64Array(3n) | null) */
+static napi_value js_adjoint_forward(napi_env env, napi_callback_info info)
+{
+    napi_value a[4];
+    args(env, info, 4, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    size_t nb;
+    const int n = (int)num(env, a[1], 1);
+    double *f = tdata(env, a[3], &nb, &ok);
+    if (!ok) return NULL;
+    if (f && nb < (size_t)n * 24) { napi_throw_range_error(env, NULL, "forces array too short"); return NULL; }
+    WT(env, wt_adjoint_forward(h, n, (int)num(env, a[2], 0), f));
+    return NULL;
+}
+
+/* adjointBackward(h, nsteps, Fbar Float64Array(3*nsteps), dJdw Float64Array(cells)) -> reverse-sweep ms */
+static napi_value js_adjoint_backward(napi_env env, napi_callback_info info)
+{
+    napi_value a[4];
+    args(env, info, 4, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok1, ok2;
+    size_t nb1, nb2;
+    const int n = (int)num(env, a[1], 0);
+    const double *Fb = tdata(env, a[2], &nb1, &ok1);
+    double *g = tdata(env, a[3], &nb2, &ok2);
+    if (!ok1 || !ok2) return NULL;
+    size_t cells, fb;
+    if (!lattice_info(env, h, &cells, &fb)) return NULL;
+    if (!Fb || !g) { napi_throw_type_error(env, NULL, "adjointBackward needs Fbar and dJdw arrays"); return NULL; }
+    if (n < 1 || !enough(env, nb1, (size_t)n * 24, "adjointBackward Fbar") ||
+        !enough(env, nb2, cells * 8, "adjointBackward dJdw"))
+        return NULL;
+    double ms = 0;
+    WT(env, wt_adjoint_backward(h, n, Fb, g, &ms));
+    napi_value v;
+    napi_create_double(env, ms, &v);
+    return v;
+}
+
+static napi_value js_adjoint_release(napi_env env, napi_callback_info info)
+{
+    napi_value a[1];
+    args(env, info, 1, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    WT(env, wt_adjoint_release(h));
+    return NULL;
+}
+
 static napi_value js_set_parity(napi_env env, napi_callback_info info)
 {
     napi_value a[2];
@@ -312,6 +451,7 @@ static napi_value js_get_state(napi_env env, napi_callback_info info)
     set_num(env, o, "parity", (double)s.parity);
     set_num(env, o, "steps", (double)s.steps);
     set_num(env, o, "blockCells", (double)s.block_cells);
+    set_num(env, o, "nx", s.nx); set_num(env, o, "ny", s.ny); set_num(env, o, "nz", s.nz);
     CALL(env, napi_create_array_with_length(env, 3, &f));
     for (uint32_t k = 0; k < 3; k++) {
         napi_value e;
@@ -331,9 +471,13 @@ static napi_value js_get_fields(napi_env env, napi_callback_info info)
     if (!h) return NULL;
     int ok;
     void *p[4];
+    size_t cells, fb;
+    if (!lattice_info(env, h, &cells, &fb)) return NULL;
     for (int k = 0; k < 4; k++) {
-        p[k] = tdata(env, a[1 + k], NULL, &ok);
+        size_t nb = 0;
+        p[k] = tdata(env, a[1 + k], &nb, &ok);
         if (!ok) return NULL;
+        if (p[k] && !enough(env, nb, cells * fb, "getFields output")) return NULL;
     }
     WT(env, wt_get_fields(h, p[0], p[1], p[2], p[3]));
     return NULL;
@@ -346,8 +490,11 @@ static napi_value js_get_pops(napi_env env, napi_callback_info info)
     wt_lattice *h = handle(env, a[0]);
     if (!h) return NULL;
     int ok;
-    void *out = tdata(env, a[2], NULL, &ok);
+    size_t nb = 0, cells, fb;
+    if (!lattice_info(env, h, &cells, &fb)) return NULL;
+    void *out = tdata(env, a[2], &nb, &ok);
     if (!ok) return NULL;
+    if (!enough(env, out ? nb : 0, 19 * cells * fb, "getPops output")) return NULL;
     WT(env, wt_get_pops(h, (int)num(env, a[1], 0), out));
     return NULL;
 }
@@ -359,10 +506,14 @@ static napi_value js_get_solid(napi_env env, napi_callback_info info)
     wt_lattice *h = handle(env, a[0]);
     if (!h) return NULL;
     int ok;
-    uint8_t *s = tdata(env, a[1], NULL, &ok);
+    size_t sb = 0, rb = 0, cells, fb;
+    if (!lattice_info(env, h, &cells, &fb)) return NULL;
+    uint8_t *s = tdata(env, a[1], &sb, &ok);
     if (!ok) return NULL;
-    uint8_t *nr = tdata(env, a[2], NULL, &ok);
+    uint8_t *nr = tdata(env, a[2], &rb, &ok);
     if (!ok) return NULL;
+    if ((s && !enough(env, sb, cells, "getSolid solid")) || (nr && !enough(env, rb, cells, "getSolid near")))
+        return NULL;
     WT(env, wt_get_solid(h, s, nr));
     return NULL;
 }
@@ -473,6 +624,8 @@ static napi_value init(napi_env env, napi_value exports)
         {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs},
         {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
         {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps},
+        {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},
+        {"adjointRelease", js_adjoint_release},
         {"voxelBounds", js_voxel_bounds}, {"voxelPlan", js_voxel_plan}, {"voxelBuild", js_voxel_build}};
     for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
         napi_value f;

@@ -255,14 +255,65 @@ struct Buf {
     ~Buf() { if (p) (void)hipFree(p); }
 };
 
-template <class S, class T>
-int adjoint_t(wt_lattice *h, int N, const double *Fbar, double *dJdw, double *forces, int K, double *ms)
+// A recorded forward pass: sqrt(N) checkpoints of the post-collision state
+// (AB layout) and the forward forces, kept on the lattice until released
+// or replaced, so the reverse sweep can run any number of times.
+struct AdjRecord {
+    int N = 0, K = 0, nseg = 0;
+    size_t st = 0;             // bytes of one stored state
+    int storage = 0, arith = 0;
+    uint64_t epoch = 0;        // lattice configuration epoch at record time
+    void *ck = nullptr;        // nseg states: ck + seg*st = s_{seg*K - 1} (s_0 for seg 0)
+    std::vector<double> forces;
+    ~AdjRecord() { if (ck) (void)hipFree(ck); }
+};
+
+int record_forward(wt_lattice *h, int N, int K, double *forces)
 {
-    const size_t st = 19 * (size_t)h->ns * sizeof(S);
-    const int nseg = (N + K - 1) / K;
-    std::vector<Buf> ck((size_t)nseg), rep((size_t)K + 1);
+    auto *R = new AdjRecord();
+    R->N = N; R->K = K; R->nseg = (N + K - 1) / K;
+    R->st = 19 * (size_t)h->ns * h->pop_elem;
+    R->storage = h->storage; R->arith = h->arith; R->epoch = h->epoch;
+    R->forces.assign((size_t)N * 3, 0.0);
+    if (hipMalloc(&R->ck, R->st * (size_t)R->nseg) != hipSuccess) {
+        R->ck = nullptr;
+        delete R;
+        return wt_fail(WT_E_NOMEM, "hipMalloc of %d adjoint checkpoints (%zu bytes each) failed", (N + K - 1) / K,
+                       19 * (size_t)h->ns * h->pop_elem);
+    }
+    auto ck = [&](int seg) { return (char *)R->ck + (size_t)seg * R->st; };
+    for (int k = 0; k < N; k++) {
+        const int seg = (k + 1) / K;
+        if (k == 0) WT_HIP(hipMemcpyAsync(ck(0), h->d_buf[h->parity], R->st, hipMemcpyDeviceToDevice, h->stream));
+        if ((k + 1) % K == 0 && seg >= 1 && seg < R->nseg)
+            WT_HIP(hipMemcpyAsync(ck(seg), h->d_buf[h->parity], R->st, hipMemcpyDeviceToDevice, h->stream));
+        if (int rc = lbm_launch_step(h, 0, h->nz, k == N - 1, h->d_forces + 3 * (k % h->forces_cap))) {
+            delete R;
+            return rc;
+        }
+        h->parity ^= 1;
+        h->steps++;
+        if (k % h->forces_cap == h->forces_cap - 1 || k == N - 1) {
+            const int base = k - (k % h->forces_cap);
+            WT_HIP(hipMemcpyAsync(R->forces.data() + 3 * base, h->d_forces, sizeof(double) * 3 * (k - base + 1),
+                                  hipMemcpyDeviceToHost, h->stream));
+            WT_HIP(hipStreamSynchronize(h->stream));
+        }
+    }
+    if (forces) std::copy(R->forces.begin(), R->forces.end(), forces);
+    for (int k = 0; k < 3; k++) h->force[k] = R->forces[3 * (N - 1) + k];
+    adj_release(h);
+    h->adj = R;
+    return 0;
+}
+
+template <class S, class T>
+int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJdw, double *ms)
+{
+    const int N = R.N, K = R.K, nseg = R.nseg;
+    const size_t st = R.st;
+    std::vector<Buf> rep((size_t)K + 1);
     Buf mu0, mu1, dg;
-    for (auto &b : ck) WT_HIP(hipMalloc(&b.p, st));
     for (auto &b : rep) WT_HIP(hipMalloc(&b.p, st));
     WT_HIP(hipMalloc(&mu0.p, st));
     WT_HIP(hipMalloc(&mu1.p, st));
@@ -271,37 +322,20 @@ int adjoint_t(wt_lattice *h, int N, const double *Fbar, double *dJdw, double *fo
     WT_HIP(hipMemsetAsync(mu1.p, 0, st, h->stream));
     WT_HIP(hipMemsetAsync(dg.p, 0, sizeof(double) * (size_t)h->n, h->stream));
 
-    // forward: checkpoint s_{seg*K - 1} (s_0 for the first segment), then step
-    std::vector<double> fw((size_t)N * 3);
-    for (int k = 0; k < N; k++) {
-        const int seg = (k + 1) / K;
-        if (k == 0) WT_HIP(hipMemcpyAsync(ck[0].p, h->d_buf[h->parity], st, hipMemcpyDeviceToDevice, h->stream));
-        if ((k + 1) % K == 0 && seg >= 1 && seg < nseg)
-            WT_HIP(hipMemcpyAsync(ck[seg].p, h->d_buf[h->parity], st, hipMemcpyDeviceToDevice, h->stream));
-        if (int rc = lbm_launch_step(h, 0, h->nz, k == N - 1, h->d_forces + 3 * (k % h->forces_cap))) return rc;
-        h->parity ^= 1;
-        h->steps++;
-        if (k % h->forces_cap == h->forces_cap - 1 || k == N - 1) {
-            const int base = k - (k % h->forces_cap);
-            WT_HIP(hipMemcpyAsync(fw.data() + 3 * base, h->d_forces, sizeof(double) * 3 * (k - base + 1),
-                                  hipMemcpyDeviceToHost, h->stream));
-            WT_HIP(hipStreamSynchronize(h->stream));
-        }
-    }
-    if (forces) std::copy(fw.begin(), fw.end(), forces);
-    for (int k = 0; k < 3; k++) h->force[k] = fw[3 * (N - 1) + k];
-
     WT_HIP(hipEventRecord(h->ev0, h->stream));
     AdjArgs<S> a{};
     a.solid = h->d_solid; a.sbits = h->d_sbits; a.nbits = h->d_nbits; a.wgt = h->d_wgt; a.dJdw = (double *)dg.p;
     a.ns = h->ns; a.lay = wt::Lay{h->ns, h->bsh}; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx;
     a.tau = h->p.tau; a.K = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
     void *mu_k = mu0.p, *mu_n = mu1.p;
+    // the replays below drive the forward kernel on explicit buffers; the
+    // lattice's own state and counters are left untouched
+    const int parity = h->parity;
     for (int seg = nseg - 1; seg >= 0; seg--) {
         // replay store: rep[r] = s_{base + r}
         const int base = seg == 0 ? 0 : seg * K - 1;
         const int k0 = seg * K, k1 = std::min(N, k0 + K);
-        WT_HIP(hipMemcpyAsync(rep[0].p, ck[seg].p, st, hipMemcpyDeviceToDevice, h->stream));
+        WT_HIP(hipMemcpyAsync(rep[0].p, (char *)R.ck + (size_t)seg * st, st, hipMemcpyDeviceToDevice, h->stream));
         for (int r = 0; base + r < k1 - 1; r++) {
             h->ovF = rep[r].p;
             h->ovG = rep[r + 1].p;
@@ -321,6 +355,7 @@ int adjoint_t(wt_lattice *h, int N, const double *Fbar, double *dJdw, double *fo
             std::swap(mu_k, mu_n);
         }
     }
+    h->parity = parity;
     WT_HIP(hipEventRecord(h->ev1, h->stream));
     WT_HIP(hipMemcpyAsync(dJdw, dg.p, sizeof(double) * (size_t)h->n, hipMemcpyDeviceToHost, h->stream));
     WT_HIP(hipStreamSynchronize(h->stream));
@@ -330,13 +365,29 @@ int adjoint_t(wt_lattice *h, int N, const double *Fbar, double *dJdw, double *fo
     return 0;
 }
 
+int check_adjoint_lattice(const wt_lattice *h)
+{
+    if (!h) return wt_fail(WT_E_ARG, "null lattice handle");
+    if (h->layout != WT_LAYOUT_AB) return wt_fail(WT_E_ARG, "the adjoint needs the AB layout");
+    if (h->p.movingWalls) return wt_fail(WT_E_ARG, "the adjoint does not support moving walls");
+    if (h->storage == WT_STORE_F16) return wt_fail(WT_E_ARG, "the adjoint needs float or double storage");
+    return 0;
+}
+
 }  // namespace
+
+void adj_release(wt_lattice *h)
+{
+    delete (AdjRecord *)h->adj;
+    h->adj = nullptr;
+}
 
 extern "C" {
 
 int wt_set_gray(wt_lattice *h, const float *w)
 {
     if (!h) return wt_fail(WT_E_ARG, "null lattice handle");
+    h->epoch++;
     if (!w) {
         if (h->d_wgt) (void)hipFree(h->d_wgt);
         h->d_wgt = nullptr;
@@ -379,19 +430,46 @@ int wt_set_gray(wt_lattice *h, const float *w)
     return 0;
 }
 
+int wt_adjoint_forward(wt_lattice *h, int nsteps, int interval, double *forces)
+{
+    if (int rc = check_adjoint_lattice(h)) return rc;
+    if (nsteps < 1) return wt_fail(WT_E_ARG, "nsteps must be positive, got %d", nsteps);
+    int K = interval > 0 ? interval : (int)std::ceil(std::sqrt((double)nsteps));
+    K = std::max(1, std::min(K, nsteps));
+    return record_forward(h, nsteps, K, forces);
+}
+
+int wt_adjoint_backward(wt_lattice *h, int nsteps, const double *Fbar, double *dJdw, double *reverse_ms)
+{
+    if (int rc = check_adjoint_lattice(h)) return rc;
+    if (!Fbar || !dJdw) return wt_fail(WT_E_ARG, "null adjoint buffer");
+    const auto *R = (const AdjRecord *)h->adj;
+    if (!R) return wt_fail(WT_E_STATE, "no recorded forward pass: call wt_adjoint_forward first");
+    if (nsteps != R->N)
+        return wt_fail(WT_E_ARG, "Fbar covers %d steps, the recorded forward pass %d", nsteps, R->N);
+    if (R->epoch != h->epoch || R->storage != h->storage || R->arith != h->arith)
+        return wt_fail(WT_E_STATE, "the lattice configuration changed since the recorded forward pass");
+    if (h->storage == WT_STORE_F64) return reverse_t<double, double>(h, *R, Fbar, dJdw, reverse_ms);
+    if (h->arith == WT_ARITH_F64) return reverse_t<float, double>(h, *R, Fbar, dJdw, reverse_ms);
+    return reverse_t<float, float>(h, *R, Fbar, dJdw, reverse_ms);
+}
+
+int wt_adjoint_release(wt_lattice *h)
+{
+    if (!h) return wt_fail(WT_E_ARG, "null lattice handle");
+    adj_release(h);
+    return 0;
+}
+
 int wt_adjoint(wt_lattice *h, int nsteps, const double *Fbar, double *dJdw, double *forces, int interval,
                double *reverse_ms)
 {
-    if (!h) return wt_fail(WT_E_ARG, "null lattice handle");
+    if (int rc = check_adjoint_lattice(h)) return rc;
     if (nsteps < 1 || !Fbar || !dJdw) return wt_fail(WT_E_ARG, "bad adjoint request");
-    if (h->layout != WT_LAYOUT_AB) return wt_fail(WT_E_ARG, "the adjoint needs the AB layout");
-    if (h->p.movingWalls) return wt_fail(WT_E_ARG, "the adjoint does not support moving walls");
-    if (h->storage == WT_STORE_F16) return wt_fail(WT_E_ARG, "the adjoint needs float or double storage");
-    int K = interval > 0 ? interval : (int)std::ceil(std::sqrt((double)nsteps));
-    K = std::max(1, std::min(K, nsteps));
-    if (h->storage == WT_STORE_F64) return adjoint_t<double, double>(h, nsteps, Fbar, dJdw, forces, K, reverse_ms);
-    if (h->arith == WT_ARITH_F64) return adjoint_t<float, double>(h, nsteps, Fbar, dJdw, forces, K, reverse_ms);
-    return adjoint_t<float, float>(h, nsteps, Fbar, dJdw, forces, K, reverse_ms);
+    if (int rc = wt_adjoint_forward(h, nsteps, interval, forces)) return rc;
+    const int rc = wt_adjoint_backward(h, nsteps, Fbar, dJdw, reverse_ms);
+    adj_release(h);
+    return rc;
 }
 
 }  // extern "C"

@@ -69,6 +69,7 @@ int alloc(void **p, size_t bytes)
 
 void release(wt_lattice *h)
 {
+    adj_release(h);
     for (void *p : {(void *)h->d_buf[0], (void *)h->d_solid, (void *)h->d_sbits,
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
                     (void *)h->d_forces})
@@ -201,6 +202,7 @@ int wt_set_params(wt_lattice *h, const wt_params *p)
     WT_CHECK_H(h);
     if (!p) return wt_fail(WT_E_ARG, "null params");
     h->p = *p;
+    h->epoch++;
     return 0;
 }
 
@@ -227,6 +229,7 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
 {
     WT_CHECK_H(h);
     if (!box) return wt_fail(WT_E_ARG, "null box");
+    h->epoch++;
     const int nx = h->nx, ny = h->ny, nz = h->nz;
     const int x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
     const bool empty = x1 < x0 || y1 < y0 || z1 < z0;
@@ -334,6 +337,7 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
     WT_CHECK_H(h);
     if (!grid || !vm || !M || !pos || !box) return wt_fail(WT_E_ARG, "null argument");
     if (vm->nx < 1 || vm->ny < 1 || vm->nz < 1) return wt_fail(WT_E_ARG, "empty voxel grid");
+    h->epoch++;
     const int64_t cnt = std::max(0, box[3] - box[0] + 1) * (int64_t)std::max(0, box[4] - box[1] + 1) *
                         std::max(0, box[5] - box[2] + 1);
     std::vector<uint8_t> mask((size_t)cnt);
@@ -402,6 +406,35 @@ static int run_steps(wt_lattice *h, int nsteps, int nslabs, double *forces)
 
 int wt_step(wt_lattice *h, int nsteps, double *forces) { return run_steps(h, nsteps, 1, forces); }
 
+int wt_step_slab(wt_lattice *h, int z0, int z1, double force[3])
+{
+    WT_CHECK_H(h);
+    if (z0 < 1 || z1 > h->nz - 1 || z0 > z1)
+        return wt_fail(WT_E_ARG, "slab [%d,%d) is not within the interior rows [1,%d)", z0, z1, h->nz - 1);
+    double f[3] = {0, 0, 0};
+    if (z1 > z0) {
+        if (int rc = lbm_launch_step(h, z0, z1, true, h->d_forces)) return rc;
+        WT_HIP(hipMemcpyAsync(f, h->d_forces, sizeof f, hipMemcpyDeviceToHost, h->stream));
+        WT_HIP(hipStreamSynchronize(h->stream));
+    }
+    if (force) std::memcpy(force, f, sizeof f);
+    return 0;
+}
+
+int wt_boundaries(wt_lattice *h, int z0, int z1, int doLo, int doHi)
+{
+    WT_CHECK_H(h);
+    if (z0 < 0 || z1 > h->nz || z0 > z1) return wt_fail(WT_E_ARG, "rows [%d,%d) out of range", z0, z1);
+    // the planes of interior rows were written by wt_step_slab with those
+    // rows; the end planes belong to whoever is told to take them
+    if (doLo)
+        if (int rc = lbm_launch_step(h, 0, 1, true, nullptr)) return rc;
+    if (doHi)
+        if (int rc = lbm_launch_step(h, h->nz - 1, h->nz, true, nullptr)) return rc;
+    WT_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 int wt_step_slabs(wt_lattice *h, int nsteps, int nslabs, double *forces)
 {
     return run_steps(h, nsteps, nslabs, forces);
@@ -442,6 +475,8 @@ int wt_get_state(const wt_lattice *h, wt_state_t *out)
     if (!out) return wt_fail(WT_E_ARG, "null output");
     out->parity = h->parity; out->steps = h->steps;
     for (int k = 0; k < 3; k++) out->force[k] = h->force[k];
+    out->nx = h->nx; out->ny = h->ny; out->nz = h->nz;
+    out->storage = h->storage; out->arith = h->arith; out->layout = h->layout;
     out->block_cells = h->bsh ? 1 << h->bsh : 0;
     out->reserved = 0;
     return 0;

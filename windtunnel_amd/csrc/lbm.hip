@@ -282,8 +282,21 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO>
-__global__ void __launch_bounds__(1024) k_step(StepArgs<S, MAC> a)
+// Resident waves per SIMD the step kernel is register-budgeted for, by
+// storage type (measured on MI355X at 512x256x256): binary16 moves half the
+// bytes per load, so it needs 8 waves (64 VGPRs) in flight to cover HBM
+// latency; 4-byte storage runs best unconstrained (one 1024-thread block
+// per CU, 4 waves per SIMD). Build switches WT_F16_WAVES / WT_WAVES.
+#ifndef WT_F16_WAVES
+#define WT_F16_WAVES 8
+#endif
+#ifndef WT_WAVES
+#define WT_WAVES 4
+#endif
+template <class S> constexpr int step_waves() { return std::is_same<S, __half>::value ? WT_F16_WAVES : WT_WAVES; }
+
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO, int WAVES>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES))) k_step(StepArgs<S, MAC> a)
 {
     using P = Pop<S>;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -712,7 +725,7 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     } else {
         a.py0 = 1; a.py1 = 0; a.pz0 = 1; a.pz1 = 0;
     }
-    hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO>), dim3(gx, h->ny, bz1 - bz0), dim3(bx), 0,
+    hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO, step_waves<S>()>), dim3(gx, h->ny, bz1 - bz0), dim3(bx), 0,
                        h->stream, a);
     WT_HIP(hipGetLastError());
     if (d_force) {

@@ -77,6 +77,8 @@ struct wt_lattice {
     float *d_wgt = nullptr;       // gray bounce-back weights, or null
     void *ovF = nullptr, *ovG = nullptr;  // explicit step buffers (adjoint replay), AB only
     int replay = 0;                       // replayed steps leave the macroscopic fields alone
+    void *adj = nullptr;                  // recorded adjoint forward pass (adjoint.hip), or null
+    uint64_t epoch = 0;                   // bumped by every change of parameters, body or weights
     void *d_mac = nullptr;        // rho, ux, uy, uz planes of n values
     double *d_partial = nullptr;  // per-block force partials
     int64_t partial_cap = 0;
@@ -120,6 +122,9 @@ int lbm_launch_step(wt_lattice *h, int bz0, int bz1, bool macro, double *force_o
 int lbm_get_pops(wt_lattice *h, int which, void *out);
 int lbm_grid_mask(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, const double M[9],
                   const double pos[3], double scale, const int box[6], uint8_t *mask);
+
+// adjoint record (adjoint.hip): frees the recorded forward pass, if any
+void adj_release(wt_lattice *h);
 
 // host reference formulas shared by host and kernels
 void eq_deviation(double r, double vx, double vy, double vz, double out[19]);

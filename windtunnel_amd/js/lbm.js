@@ -69,6 +69,7 @@ class LBM {
     this._fields = null;
     this._solid = null;
     this._f3 = new Float64Array(3);
+    this._sf = new Float64Array(3);
     this.reset();
   }
 
@@ -250,9 +251,70 @@ class LBM {
     return this.solid[xi + this.nx * (yi + this.ny * zi)] !== 0;
   }
 
-  /** Per-slab stepping is a whole-lattice device launch here: see pool.js. */
-  stepSlab() {
-    throw new Error('stepSlab: the GPU lattice steps whole z-slab sets per launch; use LBMPool.Pool or stepMany');
+  /**
+   * Collide + stream the z rows [z0,z1) (js/lbm.js:293-465) on the device.
+   * The x/y boundary planes and the outlet copy of those rows come with the
+   * same launch, so _boundaries() only has the z end planes left. Leaves the
+   * slab's force in this._sf; no parity flip, as in the reference.
+   */
+  stepSlab(z0, z1) {
+    this._sync();
+    native().stepSlab(this._h, z0, z1, this._sf);
+    this._dirty();
+  }
+
+  /** _boundaries(g, z0, z1, doLo, doHi) (js/lbm.js:467-512); g is the device's next buffer. */
+  _boundaries(g, z0, z1, doLo, doHi) {
+    const a = z0 === undefined ? 0 : z0, b = z1 === undefined ? this.nz : z1;
+    const lo = z0 === undefined ? true : !!doLo, hi = z1 === undefined ? true : !!doHi;
+    this._sync();
+    native().boundaries(this._h, a, b, lo, hi);
+    this._dirty();
+  }
+
+  idx(x, y, z) { return x + this.nx * (y + this.ny * z); }
+
+  /**
+   * Host views of the lattice (js/lbm.js:135-140): the populations and the
+   * worker control block live on the device, so only the downloaded
+   * fields, masks and wall velocities are returned, as snapshots.
+   */
+  buffers() {
+    const F = this._macro(), M = this._masks();
+    return { ux: F.ux.buffer, uy: F.uy.buffer, uz: F.uz.buffer, rho: F.rho.buffer,
+      solid: M.solid.buffer, near: M.near.buffer };
+  }
+
+  // ---- gray bounce-back and its discrete adjoint (no reference counterpart)
+
+  /** Gray weights, one per cell (0 plain fluid, (0,1) gray); null clears. */
+  setGray(w) {
+    native().setGray(this._h, w ? Float32Array.from(w) : null);
+  }
+
+  /** n steps as stepMany(n) does, recording sqrt(n) checkpoints; returns the forces [3n]. */
+  adjointForward(n, interval) {
+    const f = new Float64Array(3 * n);
+    this._sync();
+    native().adjointForward(this._h, n, interval || 0, f);
+    this.force.set(f.subarray(3 * n - 3));
+    this.steps += n;
+    this._adjSteps = n;
+    this._dirty();
+    return f;
+  }
+
+  /** dJ/dw per cell for J = sum_k Fbar[3k..3k+2] . F_k over the recorded forward pass. */
+  adjointBackward(Fbar) {
+    const n = this._adjSteps || 0;
+    const g = new Float64Array(this.n);
+    const ms = native().adjointBackward(this._h, n, Float64Array.from(Fbar), g);
+    return { dJdw: g, ms };
+  }
+
+  adjointRelease() {
+    native().adjointRelease(this._h);
+    this._adjSteps = 0;
   }
 }
 

@@ -47,6 +47,8 @@ class BodyStats(C.Structure):
 
 class State(C.Structure):
     _fields_ = [("parity", C.c_int64), ("steps", C.c_int64), ("force", C.c_double * 3),
+                ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("storage", C.c_int32), ("arith", C.c_int32), ("layout", C.c_int32),
                 ("block_cells", C.c_int32), ("reserved", C.c_int32)]
 
 
@@ -80,7 +82,12 @@ SIGNATURES = {
     "wt_set_timing": (_I, [_P, _I]),
     "wt_get_timing": (_I, [_P, _P, _P]),
     "wt_set_gray": (_I, [_P, _P]),
+    "wt_step_slab": (_I, [_P, _I, _I, _P]),
+    "wt_boundaries": (_I, [_P, _I, _I, _I, _I]),
     "wt_adjoint": (_I, [_P, _I, _P, _P, _P, _I, _P]),
+    "wt_adjoint_forward": (_I, [_P, _I, _I, _P]),
+    "wt_adjoint_backward": (_I, [_P, _I, _P, _P, _P]),
+    "wt_adjoint_release": (_I, [_P]),
     "wt_voxel_bounds": (_I, [_P, _L, _P]),
     "wt_voxel_plan": (_I, [_P, _L, _I, _P]),
     "wt_voxel_build": (_I, [_P, _L, _I, _P, _P]),
@@ -339,6 +346,24 @@ class LBM:
         self._fields = None
         return f
 
+    def step_slab(self, z0: int, z1: int) -> np.ndarray:
+        """stepSlab(z0, z1): rows [z0,z1) with their x/y planes, no parity
+        flip; returns the slab's force (the reference's _sf)."""
+        f = np.zeros(3)
+        _check(lib().wt_step_slab(self._h, int(z0), int(z1), _ptr(f)))
+        self._fields = None
+        return f
+
+    def boundaries(self, z0=None, z1=None, doLo=None, doHi=None) -> None:
+        """_boundaries(g, z0, z1, doLo, doHi) after step_slab."""
+        whole = z0 is None
+        a = 0 if z0 is None else int(z0)
+        b = self.nz if z1 is None else int(z1)
+        lo = True if whole else bool(doLo)
+        hi = True if z1 is None else bool(doHi)
+        _check(lib().wt_boundaries(self._h, a, b, int(lo), int(hi)))
+        self._fields = None
+
     def time_steps(self, n: int) -> float:
         """Device milliseconds for n steps, from HIP events on the lattice stream."""
         ms = C.c_double()
@@ -377,6 +402,31 @@ class LBM:
         _check(lib().wt_adjoint(self._h, int(nsteps), _ptr(Fbar), _ptr(g), _ptr(f), int(interval), C.byref(ms)))
         self._fields = None
         return f, g, ms.value
+
+    def adjoint_forward(self, nsteps: int, interval: int = 0) -> np.ndarray:
+        """Run nsteps like step() and record checkpoints for adjoint_backward.
+        Returns the forces [nsteps, 3]."""
+        f = np.zeros((int(nsteps), 3))
+        _check(lib().wt_adjoint_forward(self._h, int(nsteps), int(interval), _ptr(f)))
+        self._fields = None
+        self._adj_steps = int(nsteps)
+        return f
+
+    def adjoint_backward(self, Fbar):
+        """dJ/dw for J = sum_k Fbar[k].F_k over the recorded forward pass.
+        Returns (dJ/dw [n], reverse-sweep device ms)."""
+        n = getattr(self, "_adj_steps", 0)
+        Fbar = np.ascontiguousarray(Fbar, dtype=np.float64)
+        if Fbar.size != 3 * n:
+            raise ValueError(f"Fbar must hold {n} x 3 values (the recorded forward's forces)")
+        g = np.zeros(self.n)
+        ms = C.c_double()
+        _check(lib().wt_adjoint_backward(self._h, n, _ptr(Fbar), _ptr(g), C.byref(ms)))
+        return g, ms.value
+
+    def adjoint_release(self) -> None:
+        _check(lib().wt_adjoint_release(self._h))
+        self._adj_steps = 0
 
     # ---- host accessors over the downloaded fields (js/lbm.js:515-545) ------
     def sample(self, x, y, z, out=None):
