@@ -83,10 +83,9 @@ def place_mesh_body(L, scale: float, res: int = 200) -> dict:
     vox = Voxel.build(body_mesh(), res)
     pos = np.array([0.3 * L.nx, L.ny / 2, L.nz / 2])
     b = np.array(vox["bbox"])
-    lo = np.floor(pos + scale * b[:3]).astype(int) - 2
-    hi = np.ceil(pos + scale * b[3:]).astype(int) + 2
-    box = [int(lo[0]), int(lo[1]), int(lo[2]), int(hi[0]), int(hi[1]), int(hi[2])]
-    L.voxelize_grid(vox, np.eye(3).reshape(-1), pos, scale, L.clamp_box(box))
+    lo, hi = pos + scale * b[:3] - 2, pos + scale * b[3:] + 2
+    box = dict(x0=lo[0], y0=lo[1], z0=lo[2], x1=hi[0], y1=hi[1], z1=hi[2])
+    L.voxelize_grid(vox, np.eye(3).reshape(-1), pos, scale, box)
     return dict(voxelize_s=time.perf_counter() - t0, grid=[vox["nx"], vox["ny"], vox["nz"]],
                 tris=vox["stats"]["tris"], openness=vox["stats"]["openness"])
 

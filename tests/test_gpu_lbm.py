@@ -373,15 +373,15 @@ def sphere_lattice(dims, **kw):
     return L
 
 
-@pytest.mark.parametrize("arith,storage", [("f64", "f32"), ("f32", "f32"), ("f32", "f16")])
+@pytest.mark.parametrize("arith,storage", [("f64", "f32"), ("f64", "f64"), ("f32", "f32"), ("f32", "f16")])
 def test_layouts_agree_bitwise(arith, storage, monkeypatch):
-    """128x40x36 sphere at the production defaults (64-cell blocks for 4-byte
-    storage): planar, blocked and in-place lattices compute the same
-    populations, fields and forces bit for bit, and in binary64 arithmetic
-    they equal the oracle."""
+    """128x40x36 sphere at the production defaults (64-cell blocks for 4- and
+    8-byte storage, 128 for binary16): planar, blocked and in-place lattices
+    compute the same populations, fields and forces bit for bit, and in
+    binary64 arithmetic they equal the oracle (float or double storage)."""
     dims = (128, 40, 36)
     runs = {}
-    for shift in (0, 6, 4):
+    for shift in (0, 7, 6, 4):
         for layout in ("ab", "esoteric"):
             monkeypatch.setenv("WT_BLOCK_SHIFT", str(shift))
             L = sphere_lattice(dims, arith=arith, storage=storage, layout=layout)
@@ -396,9 +396,10 @@ def test_layouts_agree_bitwise(arith, storage, monkeypatch):
             assert np.array_equal(F[k], base[1][k]), (key, k)
         np.testing.assert_allclose(f, base[0], rtol=1e-12, atol=1e-15, err_msg=str(key))
     if arith == "f64":
-        monkeypatch.setenv("WT_BLOCK_SHIFT", "6")
+        monkeypatch.delenv("WT_BLOCK_SHIFT")
         L = sphere_lattice(dims, arith=arith, storage=storage)
-        O = OracleLBM(*dims)
+        assert L.block_cells == 64
+        O = OracleLBM(*dims, prec=8 if storage == "f64" else 4)
         O.set_params(tau=0.52, les=True, csm=0.1, trt=True, u0=0.08, **{"lambda": 3 / 16})
         O.reset()
         O.voxelize_mask(*sphere_mask(dims))

@@ -141,6 +141,17 @@ template <class S> struct AdjArgs {
     int nx, ny, nz, nwx;
     double Fb[3];
     double tau, K, lambda;
+    // Straight-line path of the cells set in `hot` (interior, not gray, no
+    // solid or gray neighbour). mu is never written at boundary-plane or
+    // solid cells, so it stays 0 there and the plain gather already drops
+    // the links the general path skips:
+    // element of slot q at the row-0 cell shifted by the streaming y/z part
+    // (muo: mu at m + c_q; spo: s_{k-1} at m - c_q; sto: home), plus the row
+    // term (row >> lsh) * lstride and the lane term of the x shift, as in
+    // the forward step (rows are whole storage blocks).
+    const uint32_t *hot;
+    int64_t muo[Q], spo[Q], sto[Q];
+    int lsh, lstride, lmask;
 };
 
 __device__ __forceinline__ bool bitset(const uint32_t *bits, int nwx, int ny, int x, int y, int z)
@@ -154,78 +165,142 @@ template <class S> __device__ __forceinline__ bool computed(const AdjArgs<S> &a,
            !bitset(a.sbits, a.nwx, a.ny, x, y, z);
 }
 
-template <class S> __device__ __forceinline__ double weight_eff(const AdjArgs<S> &a, int64_t j, int x, int y, int z)
+// Cells the straight-line adjoint path may take (see AdjArgs::hot).
+__global__ void __launch_bounds__(256) k_adj_hot(const uint32_t *sbits, const uint32_t *nbits, const float *wgt,
+                                                 uint32_t *hot, int nx, int ny, int nz, int nwx)
 {
-    if (bitset(a.sbits, a.nwx, a.ny, x, y, z)) return 1.0;
-    return a.wgt ? (double)a.wgt[j] : 0.0;
+    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nwords = (int64_t)nwx * ny * nz;
+    if (w >= nwords) return;
+    const int wx = (int)(w % nwx);
+    const int64_t row = w / nwx;
+    const int y = (int)(row % ny), z = (int)(row / ny);
+    uint32_t hb = 0;
+    if (y >= 1 && y <= ny - 2 && z >= 1 && z <= nz - 2) {
+        const uint32_t cold = sbits[w] | nbits[w];
+        for (int b = 0; b < 32; b++) {
+            const int x = wx * 32 + b;
+            if (x < 1 || x > nx - 2 || ((cold >> b) & 1u)) continue;
+            if (wgt && wgt[x + (int64_t)nx * (y + (int64_t)ny * z)] > 0.0f) continue;
+            hb |= 1u << b;
+        }
+    }
+    hot[w] = hb;
 }
 
-// lambda_k at cell (x,y,z) = m; `comp` says whether m itself is computed
-template <class S, class T>
-__device__ __forceinline__ void lambda_at(const AdjArgs<S> &a, int64_t m, int x, int y, int z, bool comp, bool nr,
-                                          T (&lam)[Q])
+template <class S, class T> __device__ __forceinline__ T ld_el(const S *base, int64_t uoff, int64_t rb, uint32_t lane)
 {
-    const T wm = comp ? (a.wgt ? (T)a.wgt[m] : T(0)) : T(0);
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
-        T v = 0;
-        if (computed(a, X, Y, Z)) v = (T(1) - wm) * (T)a.mu[pix(q, m + off_of(q, a.nx, a.ny), a.lay)];
-        if (comp && nr) {
-            const int64_t j = m + off_of(q, a.nx, a.ny);
-            const double wj = weight_eff(a, j, X, Y, Z);
-            if (wj > 0) {
-                const int qb = OPP[q];
-                const bool flink = a.solid[j] == wt::SOLID_OBJ || (a.solid[j] == 0 && a.wgt && a.wgt[j] > 0.0f);
-                const double cF = flink ? EX[qb] * a.Fb[0] + EY[qb] * a.Fb[1] + EZ[qb] * a.Fb[2] : 0.0;
-                v += (T)wj * ((T)a.mu[pix(qb, m, a.lay)] - T(2) * (T)cF);
-            }
-        }
-        lam[q] = v;
-    }
+    return (T)*(const S *)((const char *)(base + (uoff + rb)) + lane);
 }
 
 template <class S, class T, bool LES, bool TRT>
-__global__ void __launch_bounds__(256) k_adjoint(AdjArgs<S> a)
+__global__ void __launch_bounds__(1024) k_adjoint(AdjArgs<S> a)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
     if (x >= a.nx) return;
+    if (bitset(a.hot, a.nwx, a.ny, x, y, z)) {
+        // no gray weight in reach and every neighbour computed: lambda_k is
+        // the plain gather of mu_k, f_{k-1} the plain pull of s_{k-1}
+        if (!a.Sp) return;
+        const int64_t row = (int64_t)a.nx * (y + (int64_t)a.ny * z);
+        const int64_t rb = (row >> a.lsh) * a.lstride;
+        auto lane = [&](int xs) {
+            return (uint32_t)(((xs >> a.lsh) * a.lstride + (xs & a.lmask)) * (int)sizeof(S));
+        };
+        const uint32_t o[3] = {lane(x - 1), lane(x), lane(x + 1)};
+        T lam[Q], f[Q], mu[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            lam[q] = ld_el<S, T>(a.mu, a.muo[q], rb, o[1 + EX[q]]);
+            f[q] = ld_el<S, T>(a.Sp, a.spo[q], rb, o[1 - EX[q]]);
+        }
+        if (x == a.nx - 2) {
+            // the outlet copies this cell: fold in lambda of x = nx-1, whose
+            // links leaving the lattice (e_x = +1) carry nothing
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+                if (EX[q] != 1) lam[q] += ld_el<S, T>(a.mu, a.muo[q], rb, o[2 + EX[q]]);
+        }
+        collide_adj<T, LES, TRT>(f, lam, (T)a.tau, (T)a.K, (T)a.lambda, mu);
+#pragma unroll
+        for (int q = 0; q < Q; q++) *(S *)((char *)(a.mun + (a.sto[q] + rb)) + o[1]) = (S)mu[q];
+        return;
+    }
     if (!computed(a, x, y, z)) return;  // outlet lambdas are folded in by the x = nx-2 cells
+
+    // Near-wall, gray and boundary-adjacent cells. Every load is issued up
+    // front; links whose far cell is not computed read mu = 0 there (mu is
+    // only ever written at computed cells) or are masked by a select.
     const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
     const bool nr = bitset(a.nbits, a.nwx, a.ny, x, y, z);
+    const float wi = a.wgt ? a.wgt[i] : 0.0f;
+    T mun[Q], muh[Q];  // mu_k at i + c_q, and at i itself
+    int sj[Q];         // solid state of i + c_q
+    float wj[Q];       // gray weight of i + c_q
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int64_t n1 = i + off_of(q, a.nx, a.ny);
+        mun[q] = (T)a.mu[pix(q, n1, a.lay)];
+        muh[q] = (T)a.mu[pix(q, i, a.lay)];
+        sj[q] = a.solid[n1];
+        wj[q] = a.wgt ? a.wgt[n1] : 0.0f;
+    }
+    // lambda_k(i): gray-aware streaming, plus bounce-back links of a near cell
     T lam[Q];
-    lambda_at<S, T>(a, i, x, y, z, true, nr, lam);
-
-    if (a.wgt && a.wgt[i] > 0.0f) {
+    const T wm = (T)wi;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        T v = (T(1) - wm) * mun[q];
+        const double w = sj[q] ? 1.0 : (double)wj[q];
+        if (nr && w > 0) {
+            const int qb = OPP[q];
+            const bool flink = sj[q] == wt::SOLID_OBJ || (sj[q] == 0 && wj[q] > 0.0f);
+            const double cF = flink ? EX[qb] * a.Fb[0] + EY[qb] * a.Fb[1] + EZ[qb] * a.Fb[2] : 0.0;
+            v += (T)w * (muh[qb] - T(2) * (T)cF);
+        }
+        lam[q] = v;
+    }
+    if (wi > 0.0f) {
+        // dJ/dw_i over the links leaving i into computed cells
+        double sk[Q], fdn[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            sk[q] = (double)a.Sk[pix(q, i, a.lay)];
+            fdn[q] = (double)a.Sk[pix(OPP[q], i + off_of(q, a.nx, a.ny), a.lay)];
+        }
         double g = 0;
+#pragma unroll
         for (int q = 0; q < Q; q++) {
             const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
-            if (!computed(a, X, Y, Z)) continue;
-            const int64_t n1 = i + off_of(q, a.nx, a.ny);
-            const double fd = (double)a.Sk[pix(OPP[q], n1, a.lay)];
-            g += (double)a.mu[pix(q, n1, a.lay)] * (fd - (double)a.Sk[pix(q, i, a.lay)]) -
-                 2 * fd * (EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2]);
+            const bool comp = X >= 1 && X <= a.nx - 2 && Y >= 1 && Y <= a.ny - 2 && Z >= 1 && Z <= a.nz - 2 && !sj[q];
+            if (comp)
+                g += (double)mun[q] * (fdn[q] - sk[q]) - 2 * fdn[q] * (EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2]);
         }
         a.dJdw[i] += g;
     }
     if (!a.Sp) return;
     if (x == a.nx - 2) {
-        T lo[Q];
-        lambda_at<S, T>(a, i + 1, x + 1, y, z, false, false, lo);
+        // the outlet copies this cell: fold in lambda of x = nx-1
 #pragma unroll
-        for (int q = 0; q < Q; q++) lam[q] += lo[q];
+        for (int q = 0; q < Q; q++)
+            if (EX[q] != 1) lam[q] += (T)a.mu[pix(q, i + 1 + off_of(q, a.nx, a.ny), a.lay)];
     }
-    // pre-collision populations of step k-1 (gray-aware pull)
+    // pre-collision populations of step k-1 (gray-aware pull from j = i - c_q = i + c_qbar)
+    T spj[Q], sph[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        spj[q] = (T)a.Sp[pix(q, i - off_of(q, a.nx, a.ny), a.lay)];
+        sph[q] = (T)a.Sp[pix(q, i, a.lay)];
+    }
     T f[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        const int64_t j = i - off_of(q, a.nx, a.ny);
-        if (!nr) { f[q] = (T)a.Sp[pix(q, j, a.lay)]; continue; }
-        const double w = weight_eff(a, j, x - EX[q], y - EY[q], z - EZ[q]);
-        if (w == 1) f[q] = (T)a.Sp[pix(OPP[q], i, a.lay)];
-        else if (w > 0) f[q] = (T(1) - (T)w) * (T)a.Sp[pix(q, j, a.lay)] + (T)w * (T)a.Sp[pix(OPP[q], i, a.lay)];
-        else f[q] = (T)a.Sp[pix(q, j, a.lay)];
+        const int qb = OPP[q];
+        const double w = !nr ? 0.0 : (sj[qb] ? 1.0 : (double)wj[qb]);
+        if (w == 1) f[q] = sph[qb];
+        else if (w > 0) f[q] = (T(1) - (T)w) * spj[q] + (T)w * sph[qb];
+        else f[q] = spj[q];
     }
     T mu[Q];
     collide_adj<T, LES, TRT>(f, lam, (T)a.tau, (T)a.K, (T)a.lambda, mu);
@@ -236,7 +311,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<S> a)
 template <class S, class T>
 int launch_adj(wt_lattice *h, const AdjArgs<S> &a)
 {
-    const int bx = 256, gx = (h->nx + bx - 1) / bx;
+    const int bx = wt::row_block(h->nx), gx = (h->nx + bx - 1) / bx;
     const dim3 grid(gx, h->ny, h->nz);
     const bool les = h->p.les, trt = h->p.trt;
     if (les && trt) hipLaunchKernelGGL((k_adjoint<S, T, true, true>), grid, dim3(bx), 0, h->stream, a);
@@ -318,6 +393,12 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     WT_HIP(hipMalloc(&mu0.p, st));
     WT_HIP(hipMalloc(&mu1.p, st));
     WT_HIP(hipMalloc(&dg.p, sizeof(double) * (size_t)h->n));
+    const int64_t nwords = (int64_t)h->nwx * h->ny * h->nz;
+    Buf hot;
+    WT_HIP(hipMalloc(&hot.p, sizeof(uint32_t) * (size_t)nwords));
+    hipLaunchKernelGGL(k_adj_hot, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, h->stream, h->d_sbits,
+                       h->d_nbits, h->d_wgt, (uint32_t *)hot.p, h->nx, h->ny, h->nz, h->nwx);
+    WT_HIP(hipGetLastError());
     WT_HIP(hipMemsetAsync(mu0.p, 0, st, h->stream));
     WT_HIP(hipMemsetAsync(mu1.p, 0, st, h->stream));
     WT_HIP(hipMemsetAsync(dg.p, 0, sizeof(double) * (size_t)h->n, h->stream));
@@ -327,6 +408,18 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     a.solid = h->d_solid; a.sbits = h->d_sbits; a.nbits = h->d_nbits; a.wgt = h->d_wgt; a.dJdw = (double *)dg.p;
     a.ns = h->ns; a.lay = wt::Lay{h->ns, h->bsh}; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx;
     a.tau = h->p.tau; a.K = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
+    a.hot = (const uint32_t *)hot.p;
+    a.lsh = h->bsh;
+    a.lstride = h->bsh ? Q << h->bsh : 1;
+    a.lmask = h->bsh ? (1 << h->bsh) - 1 : 0;
+    for (int q = 0; q < Q; q++) {
+        // slot q of the row-0 cell shifted by dyz cells (a multiple of nx, so of the block)
+        const int64_t dyz = (int64_t)h->nx * (EY[q] + (int64_t)h->ny * EZ[q]);
+        const int64_t slot = h->bsh ? (int64_t)q << h->bsh : q * h->ns;
+        a.muo[q] = slot + (dyz >> h->bsh) * a.lstride;
+        a.spo[q] = slot + ((-dyz) >> h->bsh) * a.lstride;
+        a.sto[q] = slot;
+    }
     void *mu_k = mu0.p, *mu_n = mu1.p;
     // the replays below drive the forward kernel on explicit buffers; the
     // lattice's own state and counters are left untouched

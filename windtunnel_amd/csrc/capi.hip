@@ -135,10 +135,11 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
     int64_t bpad = 0;
     if (const char *e = std::getenv("WT_BUFFER_PAD")) bpad = std::atoll(e);
     if (bpad < 0) bpad = 0;
-    // Population layout, measured on MI355X: 64-cell blocks for 4- and
-    // 8-byte values, planar planes for binary16 (WT_BLOCK_SHIFT = log2 of
-    // the block overrides it, 0 = planar: a tuning and test knob).
-    h->bsh = storage == WT_STORE_F16 ? 0 : 6;
+    // Population layout, measured on MI355X at 512x256x256: 64-cell blocks
+    // for 4- and 8-byte values, 128-cell blocks (256 B per direction) for
+    // binary16 (WT_BLOCK_SHIFT = log2 of the block overrides it, 0 = planar:
+    // a tuning and test knob).
+    h->bsh = storage == WT_STORE_F16 ? 7 : 6;
     if (const char *e = std::getenv("WT_BLOCK_SHIFT")) {
         const int v = std::atoi(e);
         h->bsh = v <= 0 ? 0 : std::max(2, std::min(10, v));

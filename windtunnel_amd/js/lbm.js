@@ -272,19 +272,6 @@ class LBM {
     this._dirty();
   }
 
-  idx(x, y, z) { return x + this.nx * (y + this.ny * z); }
-
-  /**
-   * Host views of the lattice (js/lbm.js:135-140): the populations and the
-   * worker control block live on the device, so only the downloaded
-   * fields, masks and wall velocities are returned, as snapshots.
-   */
-  buffers() {
-    const F = this._macro(), M = this._masks();
-    return { ux: F.ux.buffer, uy: F.uy.buffer, uz: F.uz.buffer, rho: F.rho.buffer,
-      solid: M.solid.buffer, near: M.near.buffer };
-  }
-
   // ---- gray bounce-back and its discrete adjoint (no reference counterpart)
 
   /** Gray weights, one per cell (0 plain fluid, (0,1) gray); null clears. */

@@ -143,3 +143,10 @@ def coefficient_stats(forces, u0: float, area: float, settle: int = SETTLE):
     st = ForceStats(settle)
     st.accumulate_series(forces, u0, area)
     return st.run_stats()
+
+
+def pressure_coefficient(rho, u0: float):
+    """C_p = ((rho - 1) / 3) / (0.5 u0^2) of a density field (js/main.js:950),
+    without the display's neighbour smoothing (js/main.js:944-949)."""
+    r = np.asarray(rho, dtype=np.float64)
+    return ((r - 1) / 3) / (0.5 * u0 * u0)
