@@ -415,7 +415,7 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     for (int q = 0; q < Q; q++) {
         // slot q of the row-0 cell shifted by dyz cells (a multiple of nx, so of the block)
         const int64_t dyz = (int64_t)h->nx * (EY[q] + (int64_t)h->ny * EZ[q]);
-        const int64_t slot = h->bsh ? (int64_t)q << h->bsh : q * h->ns;
+        const int64_t slot = h->bsh ? (int64_t)wt::SLOT[q] << h->bsh : wt::SLOT[q] * h->ns;
         a.muo[q] = slot + (dyz >> h->bsh) * a.lstride;
         a.spo[q] = slot + ((-dyz) >> h->bsh) * a.lstride;
         a.sto[q] = slot;

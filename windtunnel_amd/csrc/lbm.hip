@@ -54,6 +54,17 @@ template <class E> __device__ __forceinline__ void st_pop(E *p, int64_t k, E v)
 #endif
 }
 
+// binary16 through its bit pattern (the nontemporal builtins take integer
+// and floating types only)
+__device__ __forceinline__ __half ld_pop(const __half *p, int64_t k)
+{
+    return __ushort_as_half(ld_pop((const unsigned short *)p, k));
+}
+__device__ __forceinline__ void st_pop(__half *p, int64_t k, __half v)
+{
+    st_pop((unsigned short *)p, k, __half_as_ushort(v));
+}
+
 template <> struct Pop<float> {
     template <class T> __device__ static T ld(const float *p, int64_t k) { return (T)ld_pop(p, k); }
     template <class T> __device__ static void st(float *p, int64_t k, T v) { st_pop(p, k, (float)v); }
@@ -151,6 +162,12 @@ __device__ __forceinline__ bool bit(const uint32_t *bits, int nwx, int ny, int x
 {
     return (bits[(x >> 5) + (int64_t)nwx * (y + (int64_t)ny * z)] >> (x & 31)) & 1u;
 }
+
+// Build switch: WT_FAST_COLLIDE=0 runs float arithmetic through the
+// reference-order collision below instead of collide_fast.
+#ifndef WT_FAST_COLLIDE
+#define WT_FAST_COLLIDE 1
+#endif
 
 // Moments, deviation equilibrium, LES and TRT/BGK relaxation of one cell
 // (js/lbm.js:377-460) in the reference's expression order.
@@ -335,7 +352,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
 #pragma unroll
             for (int q = 0; q < Q; q++) f[q] = ld_at<S, T>(a.lp[q] + rb, o[1 + ld_shift<ESO>(q)]);
             T g[Q], r, vx, vy, vz;
-            if constexpr (std::is_same<T, float>::value)
+            if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
                 collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
                                        vx, vy, vz);
             else
@@ -422,7 +439,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
                 }
             }
             T g[Q], r, vx, vy, vz;
-            if constexpr (std::is_same<T, float>::value)
+            if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
                 collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
                                        vx, vy, vz);
             else
@@ -700,7 +717,8 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
         const int ph = h->parity, nph = ph ^ 1;
         // element offset of slot `slot` of the row-0 cell shifted by dyz cells (a multiple of nx)
         auto uoff = [&](int slot, int64_t dyz) -> int64_t {
-            return (h->bsh ? (int64_t)slot << h->bsh : slot * h->ns) + (dyz >> h->bsh) * a.lstride;
+            const int64_t s = SLOT[slot];
+            return (h->bsh ? s << h->bsh : s * h->ns) + (dyz >> h->bsh) * a.lstride;
         };
         for (int q = 0; q < Q; q++) {
             const int64_t dyz = (int64_t)h->nx * (EY[q] + (int64_t)h->ny * EZ[q]);

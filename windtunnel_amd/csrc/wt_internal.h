@@ -34,10 +34,24 @@ struct Lay {
     int64_t ns;  // cells per plane
     int sh;      // log2 of the cells per block, or 0 for planar
 };
+// Storage position of direction q within a block (or plane order). With
+// WT_SLOT_ORDER 1 the directions a cell pulls from the same neighbouring
+// row (same e_y, e_z) sit next to each other, so a wave's pulls from one
+// row form one contiguous run: (0,0) 0 1 2 | (1,0) 3 7 10 | (-1,0) 4 8 9 |
+// (0,1) 5 11 14 | (0,-1) 6 12 13 | 15 | 16 | 17 | 18. Order 0 is q itself.
+#ifndef WT_SLOT_ORDER
+#define WT_SLOT_ORDER 1
+#endif
+#if WT_SLOT_ORDER
+constexpr int SLOT[Q] = {0, 1, 2, 3, 6, 9, 12, 4, 7, 8, 5, 10, 13, 14, 11, 15, 16, 17, 18};
+#else
+constexpr int SLOT[Q] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18};
+#endif
 __host__ __device__ __forceinline__ int64_t pix(int q, int64_t cell, Lay L)
 {
-    if (L.sh == 0) return q * L.ns + cell;
-    return (((cell >> L.sh) * Q + q) << L.sh) + (cell & ((int64_t(1) << L.sh) - 1));
+    const int s = SLOT[q];
+    if (L.sh == 0) return s * L.ns + cell;
+    return (((cell >> L.sh) * Q + s) << L.sh) + (cell & ((int64_t(1) << L.sh) - 1));
 }
 
 // binary16 storage keeps (f_q - w_q) * F16_SCALE: a power of two, so the
