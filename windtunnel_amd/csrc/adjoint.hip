@@ -369,6 +369,10 @@ int record_forward(wt_lattice *h, int N, int K, double *forces)
         h->parity ^= 1;
         h->steps++;
         if (k % h->forces_cap == h->forces_cap - 1 || k == N - 1) {
+            if (int rc = lbm_flush_force(h)) {
+                delete R;
+                return rc;
+            }
             const int base = k - (k % h->forces_cap);
             WT_HIP(hipMemcpyAsync(R->forces.data() + 3 * base, h->d_forces, sizeof(double) * 3 * (k - base + 1),
                                   hipMemcpyDeviceToHost, h->stream));

@@ -130,10 +130,16 @@ template <class S, class MAC> struct StepArgs {
     const float *wgt;  // gray bounce-back weights (AB layout), or null
     MAC *rho, *ux, *uy, *uz;
     double *partial;
+    // force reduction of the previous launch, done by workgroup (0,0,0):
+    // prev_count partials at prev_partial summed into prev_out (null: none)
+    const double *prev_partial;
+    double *prev_out;
+    int64_t prev_count;
     int64_t n, ns;              // ns: cells per population plane (whole blocks)
     Lay lay;                    // population layout
     int py0, py1, pz0, pz1;     // rows (y,z) whose blocks write force partials
     int nx, ny, nz, nwx, z0, mov, phase, rho_near;
+    int revy, revz;  // sweep the rows with y / z descending
     double tau, sq18cs2, lambda, u0;
     float tauf, omf, sq18f, lamf;  // the same in float: tau, 1/tau, 18 sqrt(2) Cs^2, Lambda
     double eqv[Q];  // boundary equilibrium deviation at (rho=1, u0, 0, 0)
@@ -292,11 +298,27 @@ __device__ __forceinline__ void collide_fast(const float (&f)[Q], float tau0, fl
     }
 }
 
+#ifndef WT_DEFER_FORCE
+#define WT_DEFER_FORCE 1
+#endif
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
+}
+
+// Fixed-order sum of `count` force partials into out[0..2] by one whole
+// wavefront: lane l adds partials l, l+64, ...; then a shuffle tree. Every
+// path that produces a step's force uses it, so forces are deterministic.
+__device__ __forceinline__ void reduce_partials(const double *p, int64_t count, double *out)
+{
+    const int lane = threadIdx.x & 63;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int64_t b = lane; b < count; b += 64) { s0 += p[3 * b]; s1 += p[3 * b + 1]; s2 += p[3 * b + 2]; }
+    s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2);
+    if (lane == 0) { out[0] = s0; out[1] = s1; out[2] = s2; }
 }
 
 // Resident waves per SIMD the step kernel is register-budgeted for, by
@@ -316,8 +338,11 @@ template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO,
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES))) k_step(StepArgs<S, MAC> a)
 {
     using P = Pop<S>;
+    if (a.prev_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 64)
+        reduce_partials(a.prev_partial, a.prev_count, a.prev_out);
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y, z = blockIdx.z + a.z0;
+    const int y = a.revy ? a.ny - 1 - (int)blockIdx.y : (int)blockIdx.y;
+    const int z = a.z0 + (a.revz ? (int)(gridDim.z - 1 - blockIdx.z) : (int)blockIdx.z);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int ph = a.phase, nph = a.phase ^ 1;
     double fx = 0, fy = 0, fz = 0;
@@ -476,24 +501,10 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
     }
 }
 
-// Fixed-order sum of the block partials into one step's force.
-__global__ void __launch_bounds__(1024) k_force_sum(const double *partial, int64_t count, double *out)
+// A pending force reduction on its own (one wavefront).
+__global__ void __launch_bounds__(64) k_force_sum(const double *partial, int64_t count, double *out)
 {
-    double s[3] = {0, 0, 0};
-    for (int64_t b = threadIdx.x; b < count; b += 1024) {
-        s[0] += partial[3 * b]; s[1] += partial[3 * b + 1]; s[2] += partial[3 * b + 2];
-    }
-    __shared__ double red[16][3];
-    for (int k = 0; k < 3; k++) s[k] = wave_sum(s[k]);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) for (int k = 0; k < 3; k++) red[wv][k] = s[k];
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) {
-            double t = 0;
-            for (int w = 0; w < 16; w++) t += red[w][k];
-            out[k] = t;
-        }
+    reduce_partials(partial, count, out);
 }
 
 // ------------------------------------------------------- reset / voxelize
@@ -701,10 +712,23 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     a.wgt = h->d_wgt;
     MAC *m = (MAC *)h->d_mac;
     a.rho = m; a.ux = m + h->n; a.uy = m + 2 * h->n; a.uz = m + 3 * h->n;
-    a.partial = h->d_partial;
+    // two partial buffers: this launch writes one while it reduces the
+    // previous launch's partials from the other
+    h->pslot ^= 1;
+    a.partial = h->d_partial + (size_t)h->pslot * 3 * h->partial_cap;
+    a.prev_partial = h->pend_partial;
+    a.prev_out = h->pend_out;
+    a.prev_count = h->pend_count;
     a.n = h->n; a.ns = h->ns; a.lay = Lay{h->ns, h->bsh}; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx; a.z0 = bz0;
     a.mov = h->p.movingWalls;
     a.rho_near = !h->replay;
+    // Sweep order (h->sweep): bit 0 reverses z, bit 1 reverses y on every
+    // other step, so the rows a step wrote last are the first the next step
+    // reads while they are still in the 256 MB memory-side cache. The
+    // result does not depend on the order.
+    const int odd = (int)(h->steps & 1);
+    a.revz = (h->sweep & 1) ? odd : 0;
+    a.revy = (h->sweep & 2) ? odd : 0;
     a.tau = h->p.tau; a.sq18cs2 = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
     a.tauf = (float)a.tau; a.omf = (float)(1 / a.tau); a.sq18f = (float)a.sq18cs2; a.lamf = (float)a.lambda;
     a.u0 = h->p.u0;
@@ -746,11 +770,17 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO, step_waves<S>()>), dim3(gx, h->ny, bz1 - bz0), dim3(bx), 0,
                        h->stream, a);
     WT_HIP(hipGetLastError());
-    if (d_force) {
-        hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(1024), 0, h->stream, h->d_partial, count, d_force);
-        WT_HIP(hipGetLastError());
-    }
+    // this launch's force is reduced by the next launch, or by lbm_flush_force
+    // (build switch WT_DEFER_FORCE=0: by its own launch right away)
+    h->pend_partial = a.partial;
+    h->pend_out = d_force;
+    h->pend_count = d_force ? count : 0;
+    if (!d_force) h->pend_partial = nullptr;
+#if !WT_DEFER_FORCE
+    return lbm_flush_force(h);
+#else
     return 0;
+#endif
 }
 
 template <class S, class T, class MAC, bool ESO>
@@ -825,6 +855,17 @@ int lbm_launch_step(wt_lattice *h, int bz0, int bz1, bool macro, double *d_force
     if (h->storage == WT_STORE_F16)
         return step_layout<__half, float, float>(h, bz0, bz1, macro, d_force);
     return step_layout<double, double, double>(h, bz0, bz1, macro, d_force);
+}
+
+int lbm_flush_force(wt_lattice *h)
+{
+    if (!h->pend_out) return 0;
+    hipLaunchKernelGGL(k_force_sum, dim3(1), dim3(64), 0, h->stream, h->pend_partial, h->pend_count, h->pend_out);
+    h->pend_partial = nullptr;
+    h->pend_out = nullptr;
+    h->pend_count = 0;
+    WT_HIP(hipGetLastError());
+    return 0;
 }
 
 int lbm_get_pops(wt_lattice *h, int which, void *d_out)

@@ -79,6 +79,7 @@ struct wt_lattice {
     int64_t n = 0;
     int64_t ns = 0;     // cells per population plane, n rounded up to a whole block
     int bsh = 6;        // log2 of the cells per storage block, 0 = planar planes
+    int sweep = 0;      // row order of the step: bit 0 alternates z, bit 1 alternates y
     int storage = WT_STORE_F32, arith = WT_ARITH_F32, layout = WT_LAYOUT_AB;
     int pop_elem = 4;   // bytes per stored population
     int mac_elem = 4;   // bytes per macroscopic field value
@@ -94,8 +95,14 @@ struct wt_lattice {
     void *adj = nullptr;                  // recorded adjoint forward pass (adjoint.hip), or null
     uint64_t epoch = 0;                   // bumped by every change of parameters, body or weights
     void *d_mac = nullptr;        // rho, ux, uy, uz planes of n values
-    double *d_partial = nullptr;  // per-block force partials
+    double *d_partial = nullptr;  // per-block force partials, two buffers of partial_cap triples
     int64_t partial_cap = 0;
+    int pslot = 0;                // buffer the last step launch wrote
+    // force reduction still owed for the last step launch (done by the next
+    // launch, or by lbm_flush_force): partials, their count, destination
+    const double *pend_partial = nullptr;
+    double *pend_out = nullptr;
+    int64_t pend_count = 0;
     double *d_forces = nullptr;   // per-step force totals of the current batch
     int forces_cap = 0;
     hipStream_t stream = nullptr;
@@ -132,7 +139,11 @@ int wt_fail(int code, const char *fmt, ...);
 int lbm_reset(wt_lattice *h);
 int lbm_apply_voxelize(wt_lattice *h, const uint8_t *ops);
 int lbm_rebuild_masks(wt_lattice *h);
+// Launches one step over z rows [bz0,bz1). Its force (when force_out is
+// given) is reduced into force_out by the NEXT step launch or by
+// lbm_flush_force, which must run before force_out is read.
 int lbm_launch_step(wt_lattice *h, int bz0, int bz1, bool macro, double *force_out);
+int lbm_flush_force(wt_lattice *h);
 int lbm_get_pops(wt_lattice *h, int which, void *out);
 int lbm_grid_mask(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, const double M[9],
                   const double pos[3], double scale, const int box[6], uint8_t *mask);
