@@ -19,10 +19,12 @@
 // are recomputed by the forward kernel into a replay store, so the reverse
 // sweep costs one recompute step plus one adjoint step per forward step.
 
+#include "collide_adj.h"
 #include "wt_internal.h"
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -39,92 +41,18 @@ __host__ __device__ constexpr int64_t off_of(int q, int nx, int ny)
     return EX[q] + (int64_t)nx * (EY[q] + (int64_t)ny * EZ[q]);
 }
 
-template <class T>
-__device__ __forceinline__ T wq(int q) { return q == 0 ? T(1.0 / 3) : (q <= 6 ? T(1.0 / 18) : T(1.0 / 36)); }
+using wt_adj::collide_adj;
+using wt_adj::collide_adj_fast;
 
-// mu = (d collide / d f)^T lam for the collision of js/lbm.js:377-460 in its
-// generic form e_q = W_q (A + 3 r cu + 4.5 r cu^2); LES and TRT included.
+// float arithmetic takes the rearranged form, double the generic one
 template <class T, bool LES, bool TRT>
-__device__ __forceinline__ void collide_adj(const T (&f)[Q], const T (&lam)[Q], T tau0, T K, T lamb, T (&mu)[Q])
+__device__ __forceinline__ void adj_collide(const T (&f)[Q], const T (&lam)[Q], double tau, double K, double lambda,
+                                            T (&mu)[Q])
 {
-    T dr = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-    for (int q = 0; q < Q; q++) { dr += f[q]; jx += (T)EX[q] * f[q]; jy += (T)EY[q] * f[q]; jz += (T)EZ[q] * f[q]; }
-    const T r = T(1) + dr, inv = T(1) / r;
-    const T vx = jx * inv, vy = jy * inv, vz = jz * inv;
-    const T usq = vx * vx + vy * vy + vz * vz;
-    const T A = dr - T(1.5) * r * usq;
-    T cu[Q], nn[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        cu[q] = (T)EX[q] * vx + (T)EY[q] * vy + (T)EZ[q] * vz;
-        nn[q] = f[q] - wq<T>(q) * (A + T(3) * r * cu[q] + T(4.5) * r * cu[q] * cu[q]);
-    }
-    T te = tau0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0, mag = 0, Sq = 0;
-    if (LES) {
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            pxx += (T)(EX[q] * EX[q]) * nn[q]; pyy += (T)(EY[q] * EY[q]) * nn[q]; pzz += (T)(EZ[q] * EZ[q]) * nn[q];
-            pxy += (T)(EX[q] * EY[q]) * nn[q]; pxz += (T)(EX[q] * EZ[q]) * nn[q]; pyz += (T)(EY[q] * EZ[q]) * nn[q];
-        }
-        mag = sqrt(T(2) * (pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz)));
-        Sq = sqrt(tau0 * tau0 + K * mag * inv);
-        te = T(0.5) * (tau0 + Sq);
-    }
-    const T om = T(1) / te;
-    const T omm = TRT ? T(1) / (T(0.5) + lamb / (te - T(0.5))) : om;
-
-    T nb[Q], omb = 0, ommb = 0;
-#pragma unroll
-    for (int q = 0; q < Q; q++) mu[q] = lam[q];
-    nb[0] = -om * lam[0];
-    omb -= nn[0] * lam[0];
-#pragma unroll
-    for (int a = 1; a < Q; a += 2) {
-        const int b = a + 1;
-        const T sp = T(0.5) * (nn[a] + nn[b]), sm = T(0.5) * (nn[a] - nn[b]);
-        const T spb = -om * (lam[a] + lam[b]), smb = -omm * (lam[a] - lam[b]);
-        nb[a] = T(0.5) * (spb + smb);
-        nb[b] = T(0.5) * (spb - smb);
-        omb -= sp * (lam[a] + lam[b]);
-        ommb -= sm * (lam[a] - lam[b]);
-    }
-    T teb = 0;
-    if (TRT) teb += ommb * omm * omm * lamb / ((te - T(0.5)) * (te - T(0.5)));
-    else omb += ommb;
-    T invb = 0;
-    if (LES) {
-        teb += -omb / (te * te);
-        const T magb = teb * K * inv / (T(4) * Sq);
-        invb += teb * K * mag / (T(4) * Sq);
-        const T Xb = mag > T(0) ? magb / mag : T(0);
-        const T pxxb = T(2) * Xb * pxx, pyyb = T(2) * Xb * pyy, pzzb = T(2) * Xb * pzz;
-        const T pxyb = T(4) * Xb * pxy, pxzb = T(4) * Xb * pxz, pyzb = T(4) * Xb * pyz;
-#pragma unroll
-        for (int q = 0; q < Q; q++)
-            nb[q] += pxxb * (T)(EX[q] * EX[q]) + pyyb * (T)(EY[q] * EY[q]) + pzzb * (T)(EZ[q] * EZ[q]) +
-                     pxyb * (T)(EX[q] * EY[q]) + pxzb * (T)(EX[q] * EZ[q]) + pyzb * (T)(EY[q] * EZ[q]);
-    }
-    T Ab = 0, rb = 0, vxb = 0, vyb = 0, vzb = 0;
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        mu[q] += nb[q];
-        const T eb = -nb[q] * wq<T>(q);
-        Ab += eb;
-        rb += eb * (T(3) * cu[q] + T(4.5) * cu[q] * cu[q]);
-        const T cub = eb * (T(3) * r + T(9) * r * cu[q]);
-        vxb += (T)EX[q] * cub; vyb += (T)EY[q] * cub; vzb += (T)EZ[q] * cub;
-    }
-    T drb = Ab;
-    rb += -T(1.5) * usq * Ab;
-    const T usqb = -T(1.5) * r * Ab;
-    vxb += T(2) * vx * usqb; vyb += T(2) * vy * usqb; vzb += T(2) * vz * usqb;
-    const T jxb = vxb * inv, jyb = vyb * inv, jzb = vzb * inv;
-    invb += vxb * jx + vyb * jy + vzb * jz;
-    rb += -inv * inv * invb;
-    drb += rb;
-#pragma unroll
-    for (int q = 0; q < Q; q++) mu[q] += drb + (T)EX[q] * jxb + (T)EY[q] * jyb + (T)EZ[q] * jzb;
+    if constexpr (std::is_same<T, float>::value)
+        collide_adj_fast<LES, TRT>(f, lam, (float)tau, (float)K, (float)lambda, mu);
+    else
+        collide_adj<T, LES, TRT>(f, lam, (T)tau, (T)K, (T)lambda, mu);
 }
 
 template <class S> struct AdjArgs {
@@ -150,6 +78,7 @@ template <class S> struct AdjArgs {
     // term (row >> lsh) * lstride and the lane term of the x shift, as in
     // the forward step (rows are whole storage blocks).
     const uint32_t *hot;
+    const uint32_t *rows;  // dispatch order of the rows (row id y + ny z per workgroup row)
     int64_t muo[Q], spo[Q], sto[Q];
     int lsh, lstride, lmask;
 };
@@ -193,13 +122,108 @@ template <class S, class T> __device__ __forceinline__ T ld_el(const S *base, in
     return (T)*(const S *)((const char *)(base + (uoff + rb)) + lane);
 }
 
+// Build switch: WT_ADJ_WAVES asks for that many resident waves per SIMD
+// (a register budget) for the adjoint kernel.
+#ifdef WT_ADJ_WAVES
+#define WT_ADJ_ATTR __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WT_ADJ_WAVES)))
+#else
+#define WT_ADJ_ATTR __launch_bounds__(1024)
+#endif
+
+// One reverse step: one workgroup per lattice row (as the forward step),
+// hot cells on the straight path, the other computed cells (near-wall and
+// gray cells) through cold_cell.
 template <class S, class T, bool LES, bool TRT>
-__global__ void __launch_bounds__(1024) k_adjoint(AdjArgs<S> a)
+__global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S> a)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y, z = blockIdx.z;
+    // rows holding cold cells come first in the dispatch order, so their
+    // long dependent load chains overlap the whole sweep
+    const uint32_t row = a.rows[blockIdx.y + (int64_t)gridDim.y * blockIdx.z];
+    const int y = (int)(row % (uint32_t)a.ny), z = (int)(row / (uint32_t)a.ny);
     if (x >= a.nx) return;
-    if (bitset(a.hot, a.nwx, a.ny, x, y, z)) {
+    if (!bitset(a.hot, a.nwx, a.ny, x, y, z)) {
+        if (!computed(a, x, y, z)) return;  // outlet lambdas are folded in by the x = nx-2 cells
+        // Cold cell (near-wall, gray). Every load is issued up front; links
+        // whose far cell is not computed read mu = 0 there (mu is only ever
+        // written at computed cells) or are masked by a select.
+        const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
+        const bool nr = bitset(a.nbits, a.nwx, a.ny, x, y, z);
+        const float wi = a.wgt ? a.wgt[i] : 0.0f;
+        T mun[Q], muh[Q];  // mu_k at i + c_q, and at i itself
+        int sj[Q];         // solid state of i + c_q
+        float wj[Q];       // gray weight of i + c_q
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int64_t n1 = i + off_of(q, a.nx, a.ny);
+            mun[q] = (T)a.mu[pix(q, n1, a.lay)];
+            muh[q] = (T)a.mu[pix(q, i, a.lay)];
+            sj[q] = a.solid[n1];
+            wj[q] = a.wgt ? a.wgt[n1] : 0.0f;
+        }
+        // lambda_k(i): gray-aware streaming, plus bounce-back links of a near cell
+        T lam[Q];
+        const T wm = (T)wi;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            T v = (T(1) - wm) * mun[q];
+            const double w = sj[q] ? 1.0 : (double)wj[q];
+            if (nr && w > 0) {
+                const int qb = OPP[q];
+                const bool flink = sj[q] == wt::SOLID_OBJ || (sj[q] == 0 && wj[q] > 0.0f);
+                const double cF = flink ? EX[qb] * a.Fb[0] + EY[qb] * a.Fb[1] + EZ[qb] * a.Fb[2] : 0.0;
+                v += (T)w * (muh[qb] - T(2) * (T)cF);
+            }
+            lam[q] = v;
+        }
+        if (wi > 0.0f) {
+            // dJ/dw_i over the links leaving i into computed cells
+            double sk[Q], fdn[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                sk[q] = (double)a.Sk[pix(q, i, a.lay)];
+                fdn[q] = (double)a.Sk[pix(OPP[q], i + off_of(q, a.nx, a.ny), a.lay)];
+            }
+            double g = 0;
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
+                const bool comp = X >= 1 && X <= a.nx - 2 && Y >= 1 && Y <= a.ny - 2 && Z >= 1 && Z <= a.nz - 2 && !sj[q];
+                if (comp)
+                    g += (double)mun[q] * (fdn[q] - sk[q]) - 2 * fdn[q] * (EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2]);
+            }
+            a.dJdw[i] += g;
+        }
+        if (!a.Sp) return;
+        if (x == a.nx - 2) {
+            // the outlet copies this cell: fold in lambda of x = nx-1
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+                if (EX[q] != 1) lam[q] += (T)a.mu[pix(q, i + 1 + off_of(q, a.nx, a.ny), a.lay)];
+        }
+        // pre-collision populations of step k-1 (gray-aware pull from j = i - c_q = i + c_qbar)
+        T spj[Q], sph[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            spj[q] = (T)a.Sp[pix(q, i - off_of(q, a.nx, a.ny), a.lay)];
+            sph[q] = (T)a.Sp[pix(q, i, a.lay)];
+        }
+        T f[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int qb = OPP[q];
+            const double w = !nr ? 0.0 : (sj[qb] ? 1.0 : (double)wj[qb]);
+            if (w == 1) f[q] = sph[qb];
+            else if (w > 0) f[q] = (T(1) - (T)w) * spj[q] + (T)w * sph[qb];
+            else f[q] = spj[q];
+        }
+        T mu[Q];
+        adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
+#pragma unroll
+        for (int q = 0; q < Q; q++) a.mun[pix(q, i, a.lay)] = (S)mu[q];
+        return;
+    }
+    {
         // no gray weight in reach and every neighbour computed: lambda_k is
         // the plain gather of mu_k, f_{k-1} the plain pull of s_{k-1}
         if (!a.Sp) return;
@@ -222,91 +246,32 @@ __global__ void __launch_bounds__(1024) k_adjoint(AdjArgs<S> a)
             for (int q = 0; q < Q; q++)
                 if (EX[q] != 1) lam[q] += ld_el<S, T>(a.mu, a.muo[q], rb, o[2 + EX[q]]);
         }
-        collide_adj<T, LES, TRT>(f, lam, (T)a.tau, (T)a.K, (T)a.lambda, mu);
+        adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
 #pragma unroll
         for (int q = 0; q < Q; q++) *(S *)((char *)(a.mun + (a.sto[q] + rb)) + o[1]) = (S)mu[q];
-        return;
     }
-    if (!computed(a, x, y, z)) return;  // outlet lambdas are folded in by the x = nx-2 cells
-
-    // Near-wall, gray and boundary-adjacent cells. Every load is issued up
-    // front; links whose far cell is not computed read mu = 0 there (mu is
-    // only ever written at computed cells) or are masked by a select.
-    const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
-    const bool nr = bitset(a.nbits, a.nwx, a.ny, x, y, z);
-    const float wi = a.wgt ? a.wgt[i] : 0.0f;
-    T mun[Q], muh[Q];  // mu_k at i + c_q, and at i itself
-    int sj[Q];         // solid state of i + c_q
-    float wj[Q];       // gray weight of i + c_q
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const int64_t n1 = i + off_of(q, a.nx, a.ny);
-        mun[q] = (T)a.mu[pix(q, n1, a.lay)];
-        muh[q] = (T)a.mu[pix(q, i, a.lay)];
-        sj[q] = a.solid[n1];
-        wj[q] = a.wgt ? a.wgt[n1] : 0.0f;
-    }
-    // lambda_k(i): gray-aware streaming, plus bounce-back links of a near cell
-    T lam[Q];
-    const T wm = (T)wi;
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        T v = (T(1) - wm) * mun[q];
-        const double w = sj[q] ? 1.0 : (double)wj[q];
-        if (nr && w > 0) {
-            const int qb = OPP[q];
-            const bool flink = sj[q] == wt::SOLID_OBJ || (sj[q] == 0 && wj[q] > 0.0f);
-            const double cF = flink ? EX[qb] * a.Fb[0] + EY[qb] * a.Fb[1] + EZ[qb] * a.Fb[2] : 0.0;
-            v += (T)w * (muh[qb] - T(2) * (T)cF);
-        }
-        lam[q] = v;
-    }
-    if (wi > 0.0f) {
-        // dJ/dw_i over the links leaving i into computed cells
-        double sk[Q], fdn[Q];
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            sk[q] = (double)a.Sk[pix(q, i, a.lay)];
-            fdn[q] = (double)a.Sk[pix(OPP[q], i + off_of(q, a.nx, a.ny), a.lay)];
-        }
-        double g = 0;
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
-            const bool comp = X >= 1 && X <= a.nx - 2 && Y >= 1 && Y <= a.ny - 2 && Z >= 1 && Z <= a.nz - 2 && !sj[q];
-            if (comp)
-                g += (double)mun[q] * (fdn[q] - sk[q]) - 2 * fdn[q] * (EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2]);
-        }
-        a.dJdw[i] += g;
-    }
-    if (!a.Sp) return;
-    if (x == a.nx - 2) {
-        // the outlet copies this cell: fold in lambda of x = nx-1
-#pragma unroll
-        for (int q = 0; q < Q; q++)
-            if (EX[q] != 1) lam[q] += (T)a.mu[pix(q, i + 1 + off_of(q, a.nx, a.ny), a.lay)];
-    }
-    // pre-collision populations of step k-1 (gray-aware pull from j = i - c_q = i + c_qbar)
-    T spj[Q], sph[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        spj[q] = (T)a.Sp[pix(q, i - off_of(q, a.nx, a.ny), a.lay)];
-        sph[q] = (T)a.Sp[pix(q, i, a.lay)];
-    }
-    T f[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const int qb = OPP[q];
-        const double w = !nr ? 0.0 : (sj[qb] ? 1.0 : (double)wj[qb]);
-        if (w == 1) f[q] = sph[qb];
-        else if (w > 0) f[q] = (T(1) - (T)w) * spj[q] + (T)w * sph[qb];
-        else f[q] = spj[q];
-    }
-    T mu[Q];
-    collide_adj<T, LES, TRT>(f, lam, (T)a.tau, (T)a.K, (T)a.lambda, mu);
-#pragma unroll
-    for (int q = 0; q < Q; q++) a.mun[pix(q, i, a.lay)] = (S)mu[q];
 }
+
+// Computed cells that are not hot: interior fluid cells next to a solid or
+// gray cell, and gray cells (any order; each cell is independent).
+__global__ void __launch_bounds__(256) k_adj_cold_list(const uint32_t *sbits, const uint32_t *hot, uint32_t *cold,
+                                                       unsigned int *count, int nx, int ny, int nz, int nwx)
+{
+    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nwords = (int64_t)nwx * ny * nz;
+    if (w >= nwords) return;
+    const int wx = (int)(w % nwx);
+    const int64_t row = w / nwx;
+    const int y = (int)(row % ny), z = (int)(row / ny);
+    if (y < 1 || y > ny - 2 || z < 1 || z > nz - 2) return;
+    const uint32_t skip = sbits[w] | hot[w];
+    for (int b = 0; b < 32; b++) {
+        const int x = wx * 32 + b;
+        if (x < 1 || x > nx - 2 || ((skip >> b) & 1u)) continue;
+        cold[atomicAdd(count, 1u)] = (uint32_t)(x + (int64_t)nx * (y + (int64_t)ny * z));
+    }
+}
+
 
 template <class S, class T>
 int launch_adj(wt_lattice *h, const AdjArgs<S> &a)
@@ -403,6 +368,31 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     hipLaunchKernelGGL(k_adj_hot, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, h->stream, h->d_sbits,
                        h->d_nbits, h->d_wgt, (uint32_t *)hot.p, h->nx, h->ny, h->nz, h->nwx);
     WT_HIP(hipGetLastError());
+    Buf cold, ccount;
+    WT_HIP(hipMalloc(&cold.p, sizeof(uint32_t) * (size_t)h->n));
+    WT_HIP(hipMalloc(&ccount.p, sizeof(unsigned int)));
+    WT_HIP(hipMemsetAsync(ccount.p, 0, sizeof(unsigned int), h->stream));
+    hipLaunchKernelGGL(k_adj_cold_list, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, h->stream, h->d_sbits,
+                       (const uint32_t *)hot.p, (uint32_t *)cold.p, (unsigned int *)ccount.p, h->nx, h->ny, h->nz,
+                       h->nwx);
+    WT_HIP(hipGetLastError());
+    unsigned int ncold = 0;
+    WT_HIP(hipMemcpyAsync(&ncold, ccount.p, sizeof ncold, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipStreamSynchronize(h->stream));
+    // dispatch order of the rows: those holding cold cells first
+    std::vector<uint32_t> cl(ncold);
+    if (ncold) WT_HIP(hipMemcpy(cl.data(), cold.p, sizeof(uint32_t) * ncold, hipMemcpyDeviceToHost));
+    const int64_t nrows = (int64_t)h->ny * h->nz;
+    std::vector<uint8_t> is_cold((size_t)nrows, 0);
+    for (uint32_t c : cl) is_cold[c / (uint32_t)h->nx] = 1;
+    std::vector<uint32_t> order;
+    order.reserve((size_t)nrows);
+    for (int pass = 1; pass >= 0; pass--)
+        for (int64_t r = 0; r < nrows; r++)
+            if (is_cold[(size_t)r] == pass) order.push_back((uint32_t)r);
+    Buf rows;
+    WT_HIP(hipMalloc(&rows.p, sizeof(uint32_t) * (size_t)nrows));
+    WT_HIP(hipMemcpy(rows.p, order.data(), sizeof(uint32_t) * (size_t)nrows, hipMemcpyHostToDevice));
     WT_HIP(hipMemsetAsync(mu0.p, 0, st, h->stream));
     WT_HIP(hipMemsetAsync(mu1.p, 0, st, h->stream));
     WT_HIP(hipMemsetAsync(dg.p, 0, sizeof(double) * (size_t)h->n, h->stream));
@@ -413,6 +403,7 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     a.ns = h->ns; a.lay = wt::Lay{h->ns, h->bsh}; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx;
     a.tau = h->p.tau; a.K = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
     a.hot = (const uint32_t *)hot.p;
+    a.rows = (const uint32_t *)rows.p;
     a.lsh = h->bsh;
     a.lstride = h->bsh ? Q << h->bsh : 1;
     a.lmask = h->bsh ? (1 << h->bsh) - 1 : 0;
