@@ -137,7 +137,7 @@ def measured_traffic():
 
 
 # --------------------------------------------------------------- precision
-def _storage_runs(dims, scale, tau, u0, les, steps, avg, storages):
+def _storage_runs(dims, scale, tau, u0, les, steps, avg, storages, csm=0.1):
     """Run the mesh-body lattice in each (storage, arith); keep C_d statistics
     over the last `avg` steps (pair-averaged, 16-block SEM) and the fields."""
     from windtunnel_amd.lbm import LBM
@@ -145,7 +145,7 @@ def _storage_runs(dims, scale, tau, u0, les, steps, avg, storages):
     out = {}
     for storage, arith in storages:
         L = LBM(*dims, storage=storage, arith=arith)
-        L.set_params(tau=tau, les=les, csm=0.1, trt=True, u0=u0, **{"lambda": 3 / 16})
+        L.set_params(tau=tau, les=les, csm=csm, trt=True, u0=u0, **{"lambda": 3 / 16})
         L.reset()
         place_mesh_body(L, scale)
         F = L.step(steps)
@@ -182,17 +182,23 @@ def run_precision(c3_steps=4000):
       mean over 2000 steps carries its SEM (which understates the error at
       ~2 shedding periods per window).
     * c3: 512x256x256 TRT+LES, binary16 against float32 on the same GPU.
+    The LES legs use Cs = 0.14 (the solver's default, js/lbm.js:109): with
+    Cs = 0.1 the impulsively started 512x256x256 and 256x128x128 wakes
+    diverge within 1000 steps, in binary64 arithmetic (the reference's own)
+    as in float.
     """
     f64 = [("f64", "f64"), ("f32", "f32"), ("f16", "f32")]
     res = dict(laminar=dict(grid="128x64x64", tau=0.6, u0=0.05, les=False, steps=4000, cd_window=2000),
-               les_128=dict(grid="128x64x64", tau=0.505, u0=0.08, les=True, steps=4000, cd_window=2000, chaotic=True),
-               c3=dict(grid="512x256x256", tau=0.505, u0=0.08, les=True, steps=c3_steps, cd_window=c3_steps // 2,
-                       chaotic=True))
+               les_128=dict(grid="128x64x64", tau=0.505, u0=0.08, les=True, csm=0.14, steps=4000, cd_window=2000,
+                            chaotic=True),
+               c3=dict(grid="512x256x256", tau=0.505, u0=0.08, les=True, csm=0.14, steps=c3_steps,
+                       cd_window=c3_steps // 2, chaotic=True,
+                       note="Cs=0.1 diverges within 1000 steps here, in binary64 arithmetic as in float"))
     res["laminar"].update(_compare(_storage_runs((128, 64, 64), 0.5, 0.6, 0.05, False, 4000, 2000, f64), "f64"))
-    res["les_128"].update(_compare(_storage_runs((128, 64, 64), 0.5, 0.505, 0.08, True, 4000, 2000, f64), "f64"))
+    res["les_128"].update(_compare(_storage_runs((128, 64, 64), 0.5, 0.505, 0.08, True, 4000, 2000, f64, 0.14), "f64"))
     if c3_steps > 0:
         res["c3"].update(_compare(_storage_runs((512, 256, 256), 2, 0.505, 0.08, True, c3_steps, c3_steps // 2,
-                                                [("f32", "f32"), ("f16", "f32")]), "f32"))
+                                                [("f32", "f32"), ("f16", "f32")], 0.14), "f32"))
     return res
 
 
