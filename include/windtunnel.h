@@ -129,6 +129,30 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
                      const double pos[3], double scale, const int box[6], int ground,
                      double roadVel, wt_body_stats *out);
 
+/* One analytic primitive of a body (js/shapes.js:42-75 insidePrim, the
+ * catalogue at :77-169). c: centre in body units; s: half-extents (box),
+ * radii (ellipsoid), [radius, half-height] (cylinder and cone, axis local
+ * +y, cone apex at +y), [chord, -, span] (foil); m: primitive -> body
+ * rotation, row-major as eulerMat builds it (:29-40); naca: 4-digit
+ * [camber m, position p, thickness t] of a foil when has_naca (else the
+ * defaults 0, 0.4, 0.12); wheel: one of makeWheelVel's wheels. */
+enum { WT_PRIM_BOX = 0, WT_PRIM_ELL = 1, WT_PRIM_CYL = 2, WT_PRIM_CONE = 3, WT_PRIM_FOIL = 4 };
+typedef struct {
+    int32_t type, wheel, has_naca, reserved;
+    double c[3], s[3], m[9], naca[3];
+} wt_part;
+
+/* voxelize(body.makeTest(), box, ground, body.makeWheelVel(u0), roadVel)
+ * for an analytic body (js/shapes.js:365-470 with js/lbm.js:191-266): the
+ * union of the parts is tested on the GPU at every cell centre of the
+ * clamped box (world -> body by M^T (X - pos) / scale, then each part's
+ * m^T), in binary64 in the reference's operation order. wheel_u0 > 0 gives
+ * the cells of wheel parts the rolling velocity of makeWheelVel(wheel_u0);
+ * 0 gives no wall velocities. Then voxelize proceeds as wt_voxelize. */
+int wt_voxelize_parts(wt_lattice *h, const wt_part *parts, int nparts, const double M[9], const double pos[3],
+                      double scale, const int box[6], int ground, double roadVel, double wheel_u0,
+                      wt_body_stats *out);
+
 /* step() nsteps times (js/lbm.js:285-291). forces (may be NULL) receives the
  * 3 force components after each step, FP64, nsteps*3 doubles. */
 int wt_step(wt_lattice *h, int nsteps, double *forces);
@@ -172,6 +196,10 @@ int wt_get_pops(wt_lattice *h, int which, void *out);
 
 /* solid (0 fluid, 1 body, 2 road) and near-wall flags (js/lbm.js:82-83). */
 int wt_get_solid(wt_lattice *h, uint8_t *solid, uint8_t *near);
+
+/* Wall velocities uwx, uwy, uwz (js/lbm.js:90-93, set by voxelize): n
+ * floats each; any pointer may be NULL. */
+int wt_get_walls(const wt_lattice *h, float *uwx, float *uwy, float *uwz);
 
 /* Device time of nsteps steps, measured with HIP events on the lattice's
  * stream (no host sync inside the timed region). */

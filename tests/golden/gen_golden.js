@@ -302,6 +302,35 @@ if (want('lbm_c2')) {
   console.log(`  lbm_c2: ${((Date.now() - t0) / 1000).toFixed(1)} s, ${(tStep / NSTEPS).toFixed(2)} ms/step`);
 }
 
+if (want('lbm_bodies')) {
+  // the app's analytic bodies (js/shapes.js catalogue) placed as js/main.js:453
+  // places them, some pitched and yawed, rasterised through Body.makeTest and
+  // (ground vehicles) makeWheelVel with a rolling road (js/main.js:607-608)
+  const NX = 96, NY = 44, NZ = 44;
+  const cases = [];
+  const poses = { sedan: [0, 0.12], sports: [0.03, -0.2], truck: [0, 0], airplane: [0.1, 0.3], wing: [0.14, 0],
+    sphere: [0, 0], cube: [0.2, 0.4], cylinder: [0, 0.25], teardrop: [-0.1, 0.15] };
+  for (const key of Object.keys(poses)) {
+    const b = new Shapes.Body(key);
+    b.fit(NX, NY, NZ);
+    const ground = !!b.def.ground;
+    b.pos = [NX * 0.36, ground ? 1 : NY * 0.5, NZ * 0.5];
+    b.pitch = poses[key][0]; b.yaw = poses[key][1];
+    b._updateMat();
+    const L = new LBM(NX, NY, NZ);
+    L.u0 = 0.06;
+    L.reset();
+    const rec = voxelizeRecorded(L, b.makeTest(), b.box(), ground, ground ? b.makeWheelVel(L.u0) : null,
+      ground ? L.u0 : 0);
+    rec.key = key;
+    rec.body = { M: b.M, pos: b.pos, scale: b.scale, pitch: b.pitch, yaw: b.yaw,
+      parts: b.parts.map(p => ({ t: p.t, c: p.c, s: p.s, m: p.m, naca: p.naca || null, wheel: !!p.wheel })) };
+    rec.rolling_u0 = ground ? L.u0 : 0;
+    cases.push(rec);
+  }
+  write('lbm_bodies', { dims: [NX, NY, NZ], cases });
+}
+
 // test.html procedure on two of its bluff bodies (js body via Shapes, replayed as a mask)
 for (const shape of ['cube', 'sphere']) {
   const name = 'lbm_testpage_' + shape;

@@ -129,6 +129,23 @@ check('gray adjoint pair: backward is linear in Fbar and repeatable', () => {
   assert(threw, 'short Fbar must be refused');
 });
 
+check('voxelizeBody rasterises the catalogue bodies as the reference does (lbm_bodies)', () => {
+  const fx = load('lbm_bodies');
+  for (const c of fx.cases) {
+    const L = new LBM(...fx.dims, { arith: 'f64' });
+    L.u0 = 0.06;
+    L.reset();
+    const bb = c.box_in;
+    const body = { parts: c.body.parts, M: c.body.M, pos: c.body.pos, scale: c.body.scale, def: {},
+      _updateMat() {}, box() { return { x0: bb[0], y0: bb[1], z0: bb[2], x1: bb[3], y1: bb[4], z1: bb[5] }; } };
+    L.voxelizeBody(body, c.ground, c.rolling_u0);
+    const w = c.after;
+    assert(sha(L.solid) === w.solid && sha(L.near) === w.near, c.key + ' solid/near');
+    assert(sha(L.uwx) === w.uwx && sha(L.uwy) === w.uwy && sha(L.uwz) === w.uwz, c.key + ' wall velocities');
+    assert(L.frontalArea === c.result.frontalArea && L.rideHeight === c.result.rideHeight, c.key + ' stats');
+  }
+});
+
 check('Voxel.build is bit-exact (vox_box, vox_hitcap, vox_sphere)', () => {
   for (const name of ['vox_box', 'vox_hitcap', 'vox_sphere']) {
     const fx = load(name);

@@ -331,6 +331,110 @@ static napi_value js_step_slabs(napi_env env, napi_callback_info info)
     return NULL;
 }
 
+/* getWalls(h, uwx, uwy, uwz) -- Float32Arrays of n cells or null */
+static napi_value js_get_walls(napi_env env, napi_callback_info info)
+{
+    napi_value a[4];
+    args(env, info, 4, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    size_t cells, fb;
+    if (!lattice_info(env, h, &cells, &fb)) return NULL;
+    float *p[3];
+    for (int k = 0; k < 3; k++) {
+        int ok;
+        size_t nb = 0;
+        p[k] = tdata(env, a[1 + k], &nb, &ok);
+        if (!ok) return NULL;
+        if (p[k] && !enough(env, nb, cells * 4, "getWalls output")) return NULL;
+    }
+    WT(env, wt_get_walls(h, p[0], p[1], p[2]));
+    return NULL;
+}
+
+static int prim_type(napi_env env, napi_value part)
+{
+    static const char *names[] = {"box", "ell", "cyl", "cone", "foil"};
+    napi_value v;
+    char buf[16] = "";
+    size_t len = 0;
+    if (napi_get_named_property(env, part, "t", &v) != napi_ok ||
+        napi_get_value_string_utf8(env, v, buf, sizeof buf, &len) != napi_ok)
+        return -1;
+    for (int i = 0; i < 5; i++)
+        if (strcmp(buf, names[i]) == 0) return i;
+    return -1;
+}
+
+/* doubles of array property `k` of object o into out[0..n-1] (missing: keep) */
+static int prop_doubles(napi_env env, napi_value o, const char *k, double *out, uint32_t n)
+{
+    napi_value v;
+    bool has = false;
+    if (napi_has_named_property(env, o, k, &has) != napi_ok || !has) return 0;
+    napi_get_named_property(env, o, k, &v);
+    if (is_nullish(env, v)) return 0;
+    for (uint32_t i = 0; i < n; i++) {
+        napi_value e;
+        if (napi_get_element(env, v, i, &e) != napi_ok) return 0;
+        napi_valuetype t;
+        napi_typeof(env, e, &t);
+        if (t == napi_number) out[i] = num(env, e, 0);
+    }
+    return 1;
+}
+
+/* voxelizeParts(h, parts [{t, c, s, m, naca?, wheel?}], M Float64Array(9),
+ *               pos Float64Array(3), scale, box Int32Array(6), ground, roadVel, wheelU0) */
+static napi_value js_voxelize_parts(napi_env env, napi_callback_info info)
+{
+    napi_value a[9];
+    args(env, info, 9, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    uint32_t np = 0;
+    if (napi_get_array_length(env, a[1], &np) != napi_ok || np > 64) {
+        napi_throw_type_error(env, NULL, "parts must be an array of at most 64 primitives");
+        return NULL;
+    }
+    wt_part parts[64];
+    memset(parts, 0, sizeof parts);
+    for (uint32_t i = 0; i < np; i++) {
+        napi_value p;
+        napi_get_element(env, a[1], i, &p);
+        const int t = prim_type(env, p);
+        if (t < 0) { napi_throw_type_error(env, NULL, "part type must be box, ell, cyl, cone or foil"); return NULL; }
+        parts[i].type = t;
+        napi_value w;
+        bool has = false;
+        if (napi_has_named_property(env, p, "wheel", &has) == napi_ok && has) {
+            napi_get_named_property(env, p, "wheel", &w);
+            parts[i].wheel = boolish(env, w);
+        }
+        const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        memcpy(parts[i].m, I, sizeof I);
+        prop_doubles(env, p, "c", parts[i].c, 3);
+        prop_doubles(env, p, "s", parts[i].s, t == WT_PRIM_CYL || t == WT_PRIM_CONE ? 2 : 3);
+        prop_doubles(env, p, "m", parts[i].m, 9);
+        parts[i].has_naca = prop_doubles(env, p, "naca", parts[i].naca, 3);
+    }
+    int ok;
+    size_t Mb = 0, pb = 0, bb = 0;
+    const double *M = tdata(env, a[2], &Mb, &ok);
+    if (!ok) return NULL;
+    const double *pos = tdata(env, a[3], &pb, &ok);
+    if (!ok) return NULL;
+    const int *box = tdata(env, a[5], &bb, &ok);
+    if (!ok) return NULL;
+    if (!enough(env, M ? Mb : 0, 72, "rotation M") || !enough(env, pos ? pb : 0, 24, "position") ||
+        !enough(env, box ? bb : 0, 24, "box"))
+        return NULL;
+    wt_body_stats s;
+    WT(env, wt_voxelize_parts(h, parts, (int)np, M, pos, num(env, a[4], 1), box, boolish(env, a[6]),
+                              num(env, a[7], 0), num(env, a[8], 0), &s));
+    return stats_object(env, &s);
+}
+
 /* stepSlab(h, z0, z1, force Float64Array(3) | null) */
 static napi_value js_step_slab(napi_env env, napi_callback_info info)
 {
@@ -624,7 +728,7 @@ static napi_value init(napi_env env, napi_value exports)
         {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs},
         {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
         {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps},
-        {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},
+        {"voxelizeParts", js_voxelize_parts}, {"getWalls", js_get_walls}, {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},
         {"adjointRelease", js_adjoint_release},
         {"voxelBounds", js_voxel_bounds}, {"voxelPlan", js_voxel_plan}, {"voxelBuild", js_voxel_build}};
     for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {

@@ -538,6 +538,15 @@ int wt_get_solid(wt_lattice *h, uint8_t *solid, uint8_t *near)
     return 0;
 }
 
+int wt_get_walls(const wt_lattice *h, float *uwx, float *uwy, float *uwz)
+{
+    WT_CHECK_H(h);
+    float *dst[3] = {uwx, uwy, uwz};
+    for (int k = 0; k < 3; k++)
+        if (dst[k]) std::memcpy(dst[k], h->h_uw.data() + (size_t)k * h->n, sizeof(float) * (size_t)h->n);
+    return 0;
+}
+
 int wt_time_steps(wt_lattice *h, int nsteps, double *ms)
 {
     WT_CHECK_H(h);

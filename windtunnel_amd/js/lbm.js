@@ -68,6 +68,7 @@ class LBM {
     this.rideHeight = 0;
     this._fields = null;
     this._solid = null;
+    this._uw = null;
     this._f3 = new Float64Array(3);
     this._sf = new Float64Array(3);
     this.reset();
@@ -110,6 +111,17 @@ class LBM {
   get solid() { return this._masks().solid; }
   get near() { return this._masks().near; }
 
+  _walls() {
+    if (!this._uw) {
+      this._uw = { uwx: new Float32Array(this.n), uwy: new Float32Array(this.n), uwz: new Float32Array(this.n) };
+      native().getWalls(this._h, this._uw.uwx, this._uw.uwy, this._uw.uwz);
+    }
+    return this._uw;
+  }
+  get uwx() { return this._walls().uwx; }
+  get uwy() { return this._walls().uwy; }
+  get uwz() { return this._walls().uwz; }
+
   _pops(which) {
     const a = this.storage === 'f64' ? new Float64Array(Q * this.n) : new Float32Array(Q * this.n);
     native().getPops(this._h, which, a);
@@ -138,6 +150,7 @@ class LBM {
     this.solidCells = s.solidCells;
     this.rideHeight = s.rideHeight;
     this._solid = null;
+    this._uw = null;
     this._dirty();
   }
 
@@ -188,6 +201,25 @@ class LBM {
     this._sync();
     this._applyStats(native().voxelizeGrid(this._h, vox.grid, vox, Float64Array.from(M), Float64Array.from(pos),
       scale, this._clampBox(box), !!ground, roadVel || 0));
+  }
+
+  /**
+   * voxelize(body.makeTest(), body.box(), ground, rolling ? body.makeWheelVel(u0) : null,
+   * rolling ? u0 : 0) for a Shapes.Body (js/main.js:607-608, js/shapes.js:365-470), with the
+   * point tests and wheel velocities evaluated on the GPU. Voxel-backed (uploaded) bodies go
+   * through the grid lookup. `rollingU0` > 0 spins the wheels and moves the road.
+   */
+  voxelizeBody(body, ground, rollingU0) {
+    body._updateMat();
+    const u0 = rollingU0 > 0 ? rollingU0 : 0;
+    if (body.def && body.def.custom) {
+      this.voxelizeGrid(body.def.vox, body.M, body.pos, body.scale, body.box(), ground, u0);
+      return;
+    }
+    const parts = body.parts.map(p => ({ t: p.t, c: p.c, s: p.s, m: p.m, naca: p.naca || null, wheel: !!p.wheel }));
+    this._sync();
+    this._applyStats(native().voxelizeParts(this._h, parts, Float64Array.from(body.M), Float64Array.from(body.pos),
+      body.scale, this._clampBox(body.box()), !!ground, u0, u0));
   }
 
   /** Advance one step (js/lbm.js:285-291). */

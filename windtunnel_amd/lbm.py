@@ -52,6 +52,14 @@ class State(C.Structure):
                 ("block_cells", C.c_int32), ("reserved", C.c_int32)]
 
 
+class Part(C.Structure):
+    _fields_ = [("type", C.c_int32), ("wheel", C.c_int32), ("has_naca", C.c_int32), ("reserved", C.c_int32),
+                ("c", C.c_double * 3), ("s", C.c_double * 3), ("m", C.c_double * 9), ("naca", C.c_double * 3)]
+
+
+PRIM = {"box": 0, "ell": 1, "cyl": 2, "cone": 3, "foil": 4}
+
+
 class VoxMeta(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("ntri", C.c_int32),
                 ("cell", C.c_double), ("inv", C.c_double), ("min", C.c_double * 3),
@@ -82,6 +90,8 @@ SIGNATURES = {
     "wt_set_timing": (_I, [_P, _I]),
     "wt_get_timing": (_I, [_P, _P, _P]),
     "wt_set_gray": (_I, [_P, _P]),
+    "wt_voxelize_parts": (_I, [_P, _P, _I, _P, _P, _D, _P, _I, _D, _D, _P]),
+    "wt_get_walls": (_I, [_P, _P, _P, _P]),
     "wt_step_slab": (_I, [_P, _I, _I, _P]),
     "wt_boundaries": (_I, [_P, _I, _I, _I, _I]),
     "wt_adjoint": (_I, [_P, _I, _P, _P, _P, _I, _P]),
@@ -314,6 +324,42 @@ class LBM:
                     else:
                         mask.append(1)
         self.voxelize_mask(b, np.array(mask, dtype=np.uint8), np.array(wv, dtype=np.float32), ground, roadVel)
+
+    def voxelize_parts(self, parts, M, pos, scale, box, ground=False, roadVel=0.0, wheel_u0=0.0):
+        """Analytic body (Body.makeTest / makeWheelVel, js/shapes.js:365-470) on
+        the GPU. parts: dicts {t: box|ell|cyl|cone|foil, c, s, m (3x3 row-major
+        primitive -> body rotation), naca (optional), wheel (optional)}."""
+        arr = (Part * max(1, len(parts)))()
+        for i, p in enumerate(parts):
+            q = arr[i]
+            q.type = PRIM[p["t"]]
+            q.wheel = int(bool(p.get("wheel", False)))
+            s = list(p["s"]) + [0.0] * (3 - len(p["s"]))
+            for k in range(3):
+                q.c[k] = float(p["c"][k])
+                q.s[k] = float(s[k])
+            m = np.asarray(p.get("m", np.eye(3)), dtype=np.float64).reshape(-1)
+            for k in range(9):
+                q.m[k] = float(m[k])
+            if p.get("naca") is not None:
+                q.has_naca = 1
+                for k in range(3):
+                    q.naca[k] = float(p["naca"][k])
+        b = np.ascontiguousarray(self.clamp_box(box) if isinstance(box, dict) else box, dtype=np.int32)
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        st = BodyStats()
+        _check(lib().wt_voxelize_parts(self._h, C.cast(arr, C.c_void_p), len(parts), _ptr(M), _ptr(pos), float(scale),
+                                       _ptr(b), int(bool(ground)), float(roadVel or 0.0), float(wheel_u0 or 0.0),
+                                       C.byref(st)))
+        self._apply_stats(st)
+        self._fields = None
+
+    def walls(self):
+        """Wall velocities (uwx, uwy, uwz) per cell (js/lbm.js:90-93)."""
+        w = np.empty((3, self.n), dtype=np.float32)
+        _check(lib().wt_get_walls(self._h, _ptr(w[0]), _ptr(w[1]), _ptr(w[2])))
+        return w
 
     def voxelize_grid(self, vox: dict, M, pos, scale, box, ground=False, roadVel=0.0):
         """Grid-backed body (js/shapes.js:369-388) evaluated on the GPU."""
