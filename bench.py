@@ -335,6 +335,71 @@ def run_cpu(wl, threads):
     return n / best / 1e6
 
 
+# ------------------------------------------------------------------- scale
+def run_scale(N=3000, K=500):
+    """640x320x320 TRT+LES in binary16 storage around the seeded leaky
+    icosphere (level 7, R = 56 cells, 0.5 % of the triangles deleted),
+    voxelized on the GPU at one lattice cell per voxel; N recorded forward
+    steps, then the checkpointed adjoint of the mean C_d over the last K
+    steps with respect to a gray skin (weight 0.5 on the first fluid layer
+    around the body)."""
+    import torch
+
+    from windtunnel_amd.lbm import LBM, Voxel
+    from windtunnel_amd.synth import noisy_sphere
+    dims = (640, 320, 320)
+    res = dict(grid="640x320x320", storage="f16", steps=N, tau=0.505, u0=0.08, Lambda=3 / 16, Cs=0.14)
+    P = noisy_sphere(level=7, R=56, holes=0.005, seed=5)
+    t0 = time.perf_counter()
+    vox = Voxel.build(P, 128)
+    res["voxel"] = dict(tris=vox["stats"]["tris"], solid=vox["stats"]["solid"], openness=vox["stats"]["openness"],
+                        dims=[vox["nx"], vox["ny"], vox["nz"]], seconds=round(time.perf_counter() - t0, 3))
+    fx = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "fixtures", "vox_leaky_l7.json")
+    if os.path.exists(fx):
+        import hashlib
+        with open(fx) as fh:
+            want = json.load(fh)
+        res["voxel"]["matches_voxel_js"] = bool(
+            hashlib.sha256(np.ascontiguousarray(vox["grid"]).tobytes()).hexdigest() == want["grid_sha256"]
+            and vox["stats"]["openness"] == want["stats"]["openness"])
+    L = LBM(*dims, storage="f16")
+    L.set_params(tau=0.505, les=True, csm=0.14, trt=True, u0=0.08, **{"lambda": 3 / 16})
+    L.reset()
+    pos = np.array([0.3 * dims[0], dims[1] / 2, dims[2] / 2])
+    b = np.array(vox["bbox"])
+    lo, hi = pos + b[:3] - 2, pos + b[3:] + 2
+    L.voxelize_grid(vox, np.eye(3).reshape(-1), pos, 1.0,
+                    dict(x0=lo[0], y0=lo[1], z0=lo[2], x1=hi[0], y1=hi[1], z1=hi[2]))
+    res.update(solid_cells=L.solidCells, frontal_area=L.frontalArea)
+    solid, near = L.solid_near()
+    w = np.where((near == 1) & (solid == 0), 0.5, 0.0).astype(np.float32)
+    del solid, near
+    res["gray_cells"] = int(np.count_nonzero(w))
+    L.set_gray(w)
+    del w
+    step_ms = L.time_steps(100) / 100
+    res.update(kernel_ms=round(step_ms, 4), mlups=round(L.n / (step_ms * 1e-3) / 1e6, 1))
+    L.reset()
+    q = 0.5 * 0.08 ** 2 * L.frontalArea
+    Fbar = np.zeros((N, 3))
+    Fbar[N - K:, 0] = 1.0 / (K * q)
+    t0 = time.perf_counter()
+    F = L.adjoint_forward(N)
+    fwd = time.perf_counter() - t0
+    free, total = torch.cuda.mem_get_info()
+    g, rev_ms = L.adjoint_backward(Fbar)
+    L.adjoint_release()
+    gray = np.flatnonzero(g)
+    res.update(recorded_forward_ms_per_step=round(fwd * 1e3 / N, 4), reverse_ms_per_step=round(rev_ms / N, 4),
+               cost_ratio=round((rev_ms / N) / step_ms, 3), finite=bool(np.all(np.isfinite(F)) and np.all(np.isfinite(g))),
+               cd=float(np.mean(F[N - K:, 0]) / q), grad_nonzero_cells=int(gray.size),
+               grad_abs_max=float(np.abs(g).max()),
+               device_gib_total=round(total / 2 ** 30, 1),
+               device_gib_in_use_after_forward=round((total - free) / 2 ** 30, 1))
+    L.close()
+    return res
+
+
 def _cpu_mask(dims, scale):
     """Solid mask of the mesh body as the GPU lattice holds it (downloaded)."""
     from windtunnel_amd.lbm import LBM
@@ -358,6 +423,7 @@ def main():
     ap.add_argument("--no-f16", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--fd-cells", type=int, default=64, help="central-difference cells of the adjoint leg (0: none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy three times (HBM counter calibration)")
@@ -413,6 +479,10 @@ def main():
     if not a.no_adjoint:
         log("adjoint 192x96x96, 3000 steps")
         out["adjoint"] = run_adjoint(a.fd_cells)
+    if not a.no_scale:
+        log("scale 640x320x320 f16, level-7 leaky icosphere, 3000 steps + adjoint")
+        out["scale"] = run_scale()
+        log(f"scale: {json.dumps(out['scale'])}")
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline, {threads} threads")

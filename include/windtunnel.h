@@ -224,12 +224,16 @@ int wt_set_gray(wt_lattice *h, const float *w);
 
 /* Discrete adjoint (no reference counterpart): runs nsteps forward steps
  * from the current state, as wt_step does (forces may be NULL), then the
- * reverse sweep with sqrt(N) checkpointing (interval <= 0 picks
- * ceil(sqrt(nsteps))) and returns dJ/dw per cell for the objective
- * J = sum_k Fbar[3k..3k+2] . F_k; cells without a gray weight get 0.
- * reverse_ms (may be NULL) receives the device time of the reverse sweep
- * (recomputation plus adjoint steps). AB layout, float or double storage,
- * no moving walls. */
+ * reverse sweep with sqrt(N) checkpointing and returns dJ/dw per cell for
+ * the objective J = sum_k Fbar[3k..3k+2] . F_k; cells without a gray weight
+ * get 0. interval <= 0 picks ceil(sqrt(nsteps)) steps per checkpoint, or
+ * more when the checkpoints would take over half the free device memory.
+ * The reverse sweep keeps a replay store of up to interval + 1 states; when
+ * that does not fit, it replays each segment in shorter windows (more
+ * recomputation, the same result bit for bit). reverse_ms (may be NULL)
+ * receives the device time of the reverse sweep (recomputation plus adjoint
+ * steps). AB layout, no moving walls. Any storage: binary16 states keep the
+ * forward's own offsets, and the adjoint fields are float. */
 int wt_adjoint(wt_lattice *h, int nsteps, const double *Fbar, double *dJdw, double *forces, int interval,
                double *reverse_ms);
 

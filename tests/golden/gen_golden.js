@@ -410,6 +410,9 @@ voxCase('vox_c2body', synth.ellipsoidBox(), 96, false);
 voxCase('vox_c2body_r200', synth.ellipsoidBox(), 200, false);
 voxCase('vox_leaky', synth.noisySphere({ level: 4, R: 20, holes: 0.02, seed: 5 }), 64, false);
 voxCase('vox_leaky_l6', synth.noisySphere({ level: 6, R: 56, holes: 0.005, seed: 5 }), 120, false);
+// the 640x320x320 scale case: level-7 icosphere (327 680 triangles before the
+// 0.5 % deletion), R = 56 cells, voxelized at one lattice cell per voxel
+voxCase('vox_leaky_l7', synth.noisySphere({ level: 7, R: 56, holes: 0.005, seed: 5 }), 128, false);
 {
   // a small soup dense enough along one ray to hit the 512-crossing cap
   const tris = [];

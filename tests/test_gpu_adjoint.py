@@ -133,3 +133,44 @@ def test_forward_backward_pair_and_autograd():
     J.backward()
     np.testing.assert_allclose(F.detach().numpy(), fa, rtol=0, atol=0)
     np.testing.assert_allclose(w.grad.numpy(), ga.astype(np.float32), rtol=1e-6, atol=0)
+
+
+DIMS_B = (128, 16, 16)  # nx a multiple of 128: blocked storage for every precision
+W_B = gray_field(DIMS_B, (40, 8, 8), 2.5, 4.5)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_windowed_reverse_sweep_is_bit_identical(storage, monkeypatch):
+    """A replay store shorter than the checkpoint segment (WT_ADJ_WINDOW)
+    recomputes more but must give the same gradient bit for bit."""
+    N = 47
+    Fb = fbar(N)
+    print(f"[adjoint window] storage={storage}", flush=True)
+    L = setup(LBM(*DIMS_B, storage=storage), DIMS_B, W_B, True, True)
+    L.adjoint_forward(N, interval=10)
+    g_full, _ = L.adjoint_backward(Fb)
+    for cap in (3, 1):
+        monkeypatch.setenv("WT_ADJ_WINDOW", str(cap))
+        g_win, _ = L.adjoint_backward(Fb)
+        assert np.array_equal(g_win, g_full), cap
+    monkeypatch.delenv("WT_ADJ_WINDOW")
+    assert np.abs(g_full).max() > 0
+    L.adjoint_release()
+
+
+def test_fp16_adjoint_tracks_fp64():
+    """binary16 states with float adjoint fields against the FP64 oracle
+    adjoint (LES + TRT, blocked storage)."""
+    N = 60
+    O = setup(OracleLBM(*DIMS_B, prec=8), DIMS_B, W_B, True, True)
+    _, go = O.adjoint(N, fbar(N))
+    gray = (W_B > 0) & (W_B < 1)
+    errs = {}
+    for storage in ("f32", "f16"):
+        G = setup(LBM(*DIMS_B, storage=storage), DIMS_B, W_B, True, True)
+        _, gg, _ = G.adjoint(N, fbar(N))
+        errs[storage] = np.linalg.norm(gg[gray] - go[gray]) / np.linalg.norm(go[gray])
+        G.close()
+    print(f"[adjoint f16] rel-L2 vs FP64: {errs}", flush=True)
+    assert errs["f32"] < 1e-3
+    assert errs["f16"] < 3e-3

@@ -19,6 +19,7 @@ GEN = {
     "vox_c2body_r200": lambda: synth.ellipsoid_box(),
     "vox_leaky": lambda: synth.noisy_sphere(level=4, R=20, holes=0.02, seed=5),
     "vox_leaky_l6": lambda: synth.noisy_sphere(level=6, R=56, holes=0.005, seed=5),
+    "vox_leaky_l7": lambda: synth.noisy_sphere(level=7, R=56, holes=0.005, seed=5),
 }
 
 
@@ -31,7 +32,7 @@ def mesh(name, fx):
 
 
 @pytest.mark.parametrize("name", ["vox_box", "vox_sphere", "vox_c2body", "vox_c2body_r200", "vox_leaky",
-                                  "vox_leaky_l6", "vox_hitcap"])
+                                  "vox_leaky_l6", "vox_leaky_l7", "vox_hitcap"])
 def test_build_bit_exact(name):
     fx = load(name)
     P = mesh(name, fx)

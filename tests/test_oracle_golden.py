@@ -149,13 +149,14 @@ def _mesh(fx, name):
         "vox_c2body_r200": lambda: synth.ellipsoid_box(),
         "vox_leaky": lambda: synth.noisy_sphere(level=4, R=20, holes=0.02, seed=5),
         "vox_leaky_l6": lambda: synth.noisy_sphere(level=6, R=56, holes=0.005, seed=5),
+        "vox_leaky_l7": lambda: synth.noisy_sphere(level=7, R=56, holes=0.005, seed=5),
     }
     P = gens[name]()
     assert synth.sha256(P) == fx["mesh_sha256"], "synthetic mesh bytes differ from the fixture's"
     return P
 
 
-VOX = ["vox_box", "vox_sphere", "vox_c2body", "vox_c2body_r200", "vox_leaky", "vox_leaky_l6", "vox_hitcap"]
+VOX = ["vox_box", "vox_sphere", "vox_c2body", "vox_c2body_r200", "vox_leaky", "vox_leaky_l6", "vox_leaky_l7", "vox_hitcap"]
 
 
 @pytest.mark.parametrize("name", VOX)

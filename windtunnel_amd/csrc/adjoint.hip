@@ -18,12 +18,20 @@
 // (19): 1.5x the bytes of a forward step. States inside a checkpoint segment
 // are recomputed by the forward kernel into a replay store, so the reverse
 // sweep costs one recompute step plus one adjoint step per forward step.
+//
+// Binary16 storage: the states (checkpoints, replays) keep the forward's own
+// binary16 offsets, so every replay is bit-identical to the recorded forward;
+// the adjoint fields mu are float, and the rounding of a store is taken as
+// the identity. When the replay store of a whole segment does not fit in
+// device memory, each segment is swept in windows: the window's states are
+// recomputed from the segment's checkpoint, and only those are kept.
 
 #include "collide_adj.h"
 #include "wt_internal.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -55,11 +63,19 @@ __device__ __forceinline__ void adj_collide(const T (&f)[Q], const T (&lam)[Q], 
         collide_adj<T, LES, TRT>(f, lam, (T)tau, (T)K, (T)lambda, mu);
 }
 
-template <class S> struct AdjArgs {
-    const S *mu;     // mu_k
+// Stored state value (the population offset f - w) as T.
+template <class T, class E> __device__ __forceinline__ T sval(const E *p, int64_t k) { return (T)p[k]; }
+template <class T> __device__ __forceinline__ T sval(const __half *p, int64_t k)
+{
+    return (T)(__half2float(p[k]) * wt::F16_INV_SCALE);
+}
+
+// S: storage type of the forward states; M: type of the adjoint fields.
+template <class S, class M> struct AdjArgs {
+    const M *mu;     // mu_k
     const S *Sk;     // s_k (gradient at gray cells)
     const S *Sp;     // s_{k-1} (pre-collision gather of step k-1), or null at k = 0
-    S *mun;          // mu_{k-1}
+    M *mun;          // mu_{k-1}
     const uint8_t *solid;
     const uint32_t *sbits, *nbits;
     const float *wgt;
@@ -88,7 +104,7 @@ __device__ __forceinline__ bool bitset(const uint32_t *bits, int nwx, int ny, in
     return (bits[(x >> 5) + (int64_t)nwx * (y + (int64_t)ny * z)] >> (x & 31)) & 1u;
 }
 
-template <class S> __device__ __forceinline__ bool computed(const AdjArgs<S> &a, int x, int y, int z)
+template <class S, class M> __device__ __forceinline__ bool computed(const AdjArgs<S, M> &a, int x, int y, int z)
 {
     return x >= 1 && x <= a.nx - 2 && y >= 1 && y <= a.ny - 2 && z >= 1 && z <= a.nz - 2 &&
            !bitset(a.sbits, a.nwx, a.ny, x, y, z);
@@ -117,9 +133,9 @@ __global__ void __launch_bounds__(256) k_adj_hot(const uint32_t *sbits, const ui
     hot[w] = hb;
 }
 
-template <class S, class T> __device__ __forceinline__ T ld_el(const S *base, int64_t uoff, int64_t rb, uint32_t lane)
+template <class T, class E> __device__ __forceinline__ T ld_el(const E *base, int64_t uoff, int64_t rb, uint32_t lane)
 {
-    return (T)*(const S *)((const char *)(base + (uoff + rb)) + lane);
+    return sval<T>((const E *)((const char *)(base + (uoff + rb)) + lane), 0);
 }
 
 // Build switch: WT_ADJ_WAVES asks for that many resident waves per SIMD
@@ -133,8 +149,8 @@ template <class S, class T> __device__ __forceinline__ T ld_el(const S *base, in
 // One reverse step: one workgroup per lattice row (as the forward step),
 // hot cells on the straight path, the other computed cells (near-wall and
 // gray cells) through cold_cell.
-template <class S, class T, bool LES, bool TRT>
-__global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S> a)
+template <class S, class M, class T, bool LES, bool TRT>
+__global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S, M> a)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     // rows holding cold cells come first in the dispatch order, so their
@@ -181,8 +197,8 @@ __global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S> a)
             double sk[Q], fdn[Q];
 #pragma unroll
             for (int q = 0; q < Q; q++) {
-                sk[q] = (double)a.Sk[pix(q, i, a.lay)];
-                fdn[q] = (double)a.Sk[pix(OPP[q], i + off_of(q, a.nx, a.ny), a.lay)];
+                sk[q] = sval<double>(a.Sk, pix(q, i, a.lay));
+                fdn[q] = sval<double>(a.Sk, pix(OPP[q], i + off_of(q, a.nx, a.ny), a.lay));
             }
             double g = 0;
 #pragma unroll
@@ -205,8 +221,8 @@ __global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S> a)
         T spj[Q], sph[Q];
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            spj[q] = (T)a.Sp[pix(q, i - off_of(q, a.nx, a.ny), a.lay)];
-            sph[q] = (T)a.Sp[pix(q, i, a.lay)];
+            spj[q] = sval<T>(a.Sp, pix(q, i - off_of(q, a.nx, a.ny), a.lay));
+            sph[q] = sval<T>(a.Sp, pix(q, i, a.lay));
         }
         T f[Q];
 #pragma unroll
@@ -220,7 +236,7 @@ __global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S> a)
         T mu[Q];
         adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
 #pragma unroll
-        for (int q = 0; q < Q; q++) a.mun[pix(q, i, a.lay)] = (S)mu[q];
+        for (int q = 0; q < Q; q++) a.mun[pix(q, i, a.lay)] = (M)mu[q];
         return;
     }
     {
@@ -229,26 +245,27 @@ __global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S> a)
         if (!a.Sp) return;
         const int64_t row = (int64_t)a.nx * (y + (int64_t)a.ny * z);
         const int64_t rb = (row >> a.lsh) * a.lstride;
-        auto lane = [&](int xs) {
-            return (uint32_t)(((xs >> a.lsh) * a.lstride + (xs & a.lmask)) * (int)sizeof(S));
-        };
-        const uint32_t o[3] = {lane(x - 1), lane(x), lane(x + 1)};
+        auto lane = [&](int xs) { return (uint32_t)((xs >> a.lsh) * a.lstride + (xs & a.lmask)); };
+        const uint32_t e[3] = {lane(x - 1), lane(x), lane(x + 1)};
+        // byte offsets of the lanes in the adjoint (M) and state (S) arrays
+        const uint32_t o[3] = {e[0] * (uint32_t)sizeof(M), e[1] * (uint32_t)sizeof(M), e[2] * (uint32_t)sizeof(M)};
+        const uint32_t os[3] = {e[0] * (uint32_t)sizeof(S), e[1] * (uint32_t)sizeof(S), e[2] * (uint32_t)sizeof(S)};
         T lam[Q], f[Q], mu[Q];
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            lam[q] = ld_el<S, T>(a.mu, a.muo[q], rb, o[1 + EX[q]]);
-            f[q] = ld_el<S, T>(a.Sp, a.spo[q], rb, o[1 - EX[q]]);
+            lam[q] = ld_el<T>(a.mu, a.muo[q], rb, o[1 + EX[q]]);
+            f[q] = ld_el<T>(a.Sp, a.spo[q], rb, os[1 - EX[q]]);
         }
         if (x == a.nx - 2) {
             // the outlet copies this cell: fold in lambda of x = nx-1, whose
             // links leaving the lattice (e_x = +1) carry nothing
 #pragma unroll
             for (int q = 0; q < Q; q++)
-                if (EX[q] != 1) lam[q] += ld_el<S, T>(a.mu, a.muo[q], rb, o[2 + EX[q]]);
+                if (EX[q] != 1) lam[q] += ld_el<T>(a.mu, a.muo[q], rb, o[2 + EX[q]]);
         }
         adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
 #pragma unroll
-        for (int q = 0; q < Q; q++) *(S *)((char *)(a.mun + (a.sto[q] + rb)) + o[1]) = (S)mu[q];
+        for (int q = 0; q < Q; q++) *(M *)((char *)(a.mun + (a.sto[q] + rb)) + o[1]) = (M)mu[q];
     }
 }
 
@@ -273,16 +290,16 @@ __global__ void __launch_bounds__(256) k_adj_cold_list(const uint32_t *sbits, co
 }
 
 
-template <class S, class T>
-int launch_adj(wt_lattice *h, const AdjArgs<S> &a)
+template <class S, class M, class T>
+int launch_adj(wt_lattice *h, const AdjArgs<S, M> &a)
 {
     const int bx = wt::row_block(h->nx), gx = (h->nx + bx - 1) / bx;
     const dim3 grid(gx, h->ny, h->nz);
     const bool les = h->p.les, trt = h->p.trt;
-    if (les && trt) hipLaunchKernelGGL((k_adjoint<S, T, true, true>), grid, dim3(bx), 0, h->stream, a);
-    else if (les) hipLaunchKernelGGL((k_adjoint<S, T, true, false>), grid, dim3(bx), 0, h->stream, a);
-    else if (trt) hipLaunchKernelGGL((k_adjoint<S, T, false, true>), grid, dim3(bx), 0, h->stream, a);
-    else hipLaunchKernelGGL((k_adjoint<S, T, false, false>), grid, dim3(bx), 0, h->stream, a);
+    if (les && trt) hipLaunchKernelGGL((k_adjoint<S, M, T, true, true>), grid, dim3(bx), 0, h->stream, a);
+    else if (les) hipLaunchKernelGGL((k_adjoint<S, M, T, true, false>), grid, dim3(bx), 0, h->stream, a);
+    else if (trt) hipLaunchKernelGGL((k_adjoint<S, M, T, false, true>), grid, dim3(bx), 0, h->stream, a);
+    else hipLaunchKernelGGL((k_adjoint<S, M, T, false, false>), grid, dim3(bx), 0, h->stream, a);
     WT_HIP(hipGetLastError());
     return 0;
 }
@@ -351,16 +368,35 @@ int record_forward(wt_lattice *h, int N, int K, double *forces)
     return 0;
 }
 
-template <class S, class T>
+// Steps per replay window: the whole segment (K) when its K + 1 states fit
+// in free device memory next to a reserve, else as many as fit. The
+// environment variable WT_ADJ_WINDOW caps it (tests of the windowed sweep).
+int replay_window(int K, size_t st, int *W)
+{
+    size_t freeb = 0, total = 0;
+    WT_HIP(hipMemGetInfo(&freeb, &total));
+    const size_t reserve = std::max<size_t>((size_t)1 << 30, total / 64);
+    const size_t fit = freeb > reserve ? (freeb - reserve) / st : 0;
+    int w = (int)std::min<size_t>((size_t)K, fit > 0 ? fit - 1 : 0);
+    if (const char *e = std::getenv("WT_ADJ_WINDOW")) {
+        const int cap = std::atoi(e);
+        if (cap >= 1) w = std::min(w, cap);
+    }
+    if (w < 1)
+        return wt_fail(WT_E_NOMEM, "no room for the adjoint replay store: %zu bytes free, %zu per state", freeb, st);
+    *W = w;
+    return 0;
+}
+
+template <class S, class M, class T>
 int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJdw, double *ms)
 {
     const int N = R.N, K = R.K, nseg = R.nseg;
-    const size_t st = R.st;
-    std::vector<Buf> rep((size_t)K + 1);
+    const size_t st = R.st;                                  // one stored state
+    const size_t sm = 19 * (size_t)h->ns * sizeof(M);        // one adjoint field
     Buf mu0, mu1, dg;
-    for (auto &b : rep) WT_HIP(hipMalloc(&b.p, st));
-    WT_HIP(hipMalloc(&mu0.p, st));
-    WT_HIP(hipMalloc(&mu1.p, st));
+    WT_HIP(hipMalloc(&mu0.p, sm));
+    WT_HIP(hipMalloc(&mu1.p, sm));
     WT_HIP(hipMalloc(&dg.p, sizeof(double) * (size_t)h->n));
     const int64_t nwords = (int64_t)h->nwx * h->ny * h->nz;
     Buf hot;
@@ -393,12 +429,18 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     Buf rows;
     WT_HIP(hipMalloc(&rows.p, sizeof(uint32_t) * (size_t)nrows));
     WT_HIP(hipMemcpy(rows.p, order.data(), sizeof(uint32_t) * (size_t)nrows, hipMemcpyHostToDevice));
-    WT_HIP(hipMemsetAsync(mu0.p, 0, st, h->stream));
-    WT_HIP(hipMemsetAsync(mu1.p, 0, st, h->stream));
+    int W = 0;
+    if (int rc = replay_window(K, st, &W)) return rc;
+    std::vector<Buf> rep((size_t)W + 1);
+    for (auto &b : rep) WT_HIP(hipMalloc(&b.p, st));
+    std::vector<void *> rp((size_t)W + 1);
+    for (int r = 0; r <= W; r++) rp[(size_t)r] = rep[(size_t)r].p;
+    WT_HIP(hipMemsetAsync(mu0.p, 0, sm, h->stream));
+    WT_HIP(hipMemsetAsync(mu1.p, 0, sm, h->stream));
     WT_HIP(hipMemsetAsync(dg.p, 0, sizeof(double) * (size_t)h->n, h->stream));
 
     WT_HIP(hipEventRecord(h->ev0, h->stream));
-    AdjArgs<S> a{};
+    AdjArgs<S, M> a{};
     a.solid = h->d_solid; a.sbits = h->d_sbits; a.nbits = h->d_nbits; a.wgt = h->d_wgt; a.dJdw = (double *)dg.p;
     a.ns = h->ns; a.lay = wt::Lay{h->ns, h->bsh}; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx;
     a.tau = h->p.tau; a.K = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
@@ -419,28 +461,41 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     // the replays below drive the forward kernel on explicit buffers; the
     // lattice's own state and counters are left untouched
     const int parity = h->parity;
+    auto replay = [&](void *from, void *to) {
+        h->ovF = from;
+        h->ovG = to;
+        h->replay = 1;
+        const int rc = lbm_launch_step(h, 0, h->nz, false, nullptr);
+        h->ovF = h->ovG = nullptr;
+        h->replay = 0;
+        return rc;
+    };
     for (int seg = nseg - 1; seg >= 0; seg--) {
-        // replay store: rep[r] = s_{base + r}
+        // checkpoint of the segment: s_base; its reverse steps are [k0, k1)
         const int base = seg == 0 ? 0 : seg * K - 1;
         const int k0 = seg * K, k1 = std::min(N, k0 + K);
-        WT_HIP(hipMemcpyAsync(rep[0].p, (char *)R.ck + (size_t)seg * st, st, hipMemcpyDeviceToDevice, h->stream));
-        for (int r = 0; base + r < k1 - 1; r++) {
-            h->ovF = rep[r].p;
-            h->ovG = rep[r + 1].p;
-            h->replay = 1;
-            const int rc = lbm_launch_step(h, 0, h->nz, false, nullptr);
-            h->ovF = h->ovG = nullptr;
-            h->replay = 0;
-            if (rc) return rc;
-        }
-        for (int k = k1 - 1; k >= k0; k--) {
-            a.mu = (const S *)mu_k;
-            a.mun = (S *)mu_n;
-            a.Sk = (const S *)rep[k - base].p;
-            a.Sp = k > 0 ? (const S *)rep[k - 1 - base].p : nullptr;
-            for (int c = 0; c < 3; c++) a.Fb[c] = Fbar[3 * k + c];
-            if (int rc = launch_adj<S, T>(h, a)) return rc;
-            std::swap(mu_k, mu_n);
+        for (int b = k1; b > k0;) {
+            // window of reverse steps [a0, b): replay store rp[r] = s_{s0 + r}
+            // for r = 0 .. b-1-s0 (at most W + 1 states)
+            const int a0 = std::max(k0, b - W);
+            const int s0 = std::max(a0 - 1, 0);
+            WT_HIP(hipMemcpyAsync(rp[0], (char *)R.ck + (size_t)seg * st, st, hipMemcpyDeviceToDevice, h->stream));
+            for (int c = base; c < s0; c++) {
+                if (int rc = replay(rp[0], rp[1])) return rc;
+                std::swap(rp[0], rp[1]);
+            }
+            for (int r = 0; s0 + r < b - 1; r++)
+                if (int rc = replay(rp[(size_t)r], rp[(size_t)r + 1])) return rc;
+            for (int k = b - 1; k >= a0; k--) {
+                a.mu = (const M *)mu_k;
+                a.mun = (M *)mu_n;
+                a.Sk = (const S *)rp[(size_t)(k - s0)];
+                a.Sp = k > 0 ? (const S *)rp[(size_t)(k - 1 - s0)] : nullptr;
+                for (int c = 0; c < 3; c++) a.Fb[c] = Fbar[3 * k + c];
+                if (int rc = launch_adj<S, M, T>(h, a)) return rc;
+                std::swap(mu_k, mu_n);
+            }
+            b = a0;
         }
     }
     h->parity = parity;
@@ -458,7 +513,6 @@ int check_adjoint_lattice(const wt_lattice *h)
     if (!h) return wt_fail(WT_E_ARG, "null lattice handle");
     if (h->layout != WT_LAYOUT_AB) return wt_fail(WT_E_ARG, "the adjoint needs the AB layout");
     if (h->p.movingWalls) return wt_fail(WT_E_ARG, "the adjoint does not support moving walls");
-    if (h->storage == WT_STORE_F16) return wt_fail(WT_E_ARG, "the adjoint needs float or double storage");
     return 0;
 }
 
@@ -496,23 +550,6 @@ int wt_set_gray(wt_lattice *h, const float *w)
     if (!h->d_wgt && hipMalloc((void **)&h->d_wgt, sizeof(float) * (size_t)h->n) != hipSuccess)
         return wt_fail(WT_E_NOMEM, "hipMalloc of the weight field failed");
     WT_HIP(hipMemcpyAsync(h->d_wgt, v.data(), sizeof(float) * (size_t)h->n, hipMemcpyHostToDevice, h->stream));
-    // force-carrying rows now include every gray cell
-    int zlo = h->nz, zhi = -1, ylo = h->ny, yhi = -1;
-    for (int64_t i = 0; i < h->n; i++)
-        if (v[i] > 0) {
-            const int z = (int)(i / ((int64_t)h->nx * h->ny)), y = (int)((i / h->nx) % h->ny);
-            zlo = std::min(zlo, z); zhi = std::max(zhi, z);
-            ylo = std::min(ylo, y); yhi = std::max(yhi, y);
-        }
-    if (zhi >= 0) {
-        const int a = std::max(1, zlo - 1), b = std::min(h->nz - 2, zhi + 1);
-        const int c = std::max(1, ylo - 1), d = std::min(h->ny - 2, yhi + 1);
-        const bool had = h->fz0 >= 0;
-        h->fz0 = had ? std::min(h->fz0, a) : a;
-        h->fz1 = had ? std::max(h->fz1, b) : b;
-        h->fy0 = had ? std::min(h->fy0, c) : c;
-        h->fy1 = had ? std::max(h->fy1, d) : d;
-    }
     if (int rc = lbm_rebuild_masks(h)) return rc;
     WT_HIP(hipStreamSynchronize(h->stream));
     return 0;
@@ -523,6 +560,16 @@ int wt_adjoint_forward(wt_lattice *h, int nsteps, int interval, double *forces)
     if (int rc = check_adjoint_lattice(h)) return rc;
     if (nsteps < 1) return wt_fail(WT_E_ARG, "nsteps must be positive, got %d", nsteps);
     int K = interval > 0 ? interval : (int)std::ceil(std::sqrt((double)nsteps));
+    if (interval <= 0) {
+        // the checkpoints take at most half the free device memory; the
+        // reverse sweep sizes its replay store to what is left
+        size_t freeb = 0, total = 0;
+        WT_HIP(hipMemGetInfo(&freeb, &total));
+        const size_t st = 19 * (size_t)h->ns * h->pop_elem;
+        const int64_t fit = (int64_t)(freeb / 2 / st);
+        if (fit < 1) return wt_fail(WT_E_NOMEM, "no room for one adjoint checkpoint (%zu bytes free)", freeb);
+        K = std::max<int64_t>(K, (nsteps + fit - 1) / fit);
+    }
     K = std::max(1, std::min(K, nsteps));
     return record_forward(h, nsteps, K, forces);
 }
@@ -537,9 +584,10 @@ int wt_adjoint_backward(wt_lattice *h, int nsteps, const double *Fbar, double *d
         return wt_fail(WT_E_ARG, "Fbar covers %d steps, the recorded forward pass %d", nsteps, R->N);
     if (R->epoch != h->epoch || R->storage != h->storage || R->arith != h->arith)
         return wt_fail(WT_E_STATE, "the lattice configuration changed since the recorded forward pass");
-    if (h->storage == WT_STORE_F64) return reverse_t<double, double>(h, *R, Fbar, dJdw, reverse_ms);
-    if (h->arith == WT_ARITH_F64) return reverse_t<float, double>(h, *R, Fbar, dJdw, reverse_ms);
-    return reverse_t<float, float>(h, *R, Fbar, dJdw, reverse_ms);
+    if (h->storage == WT_STORE_F64) return reverse_t<double, double, double>(h, *R, Fbar, dJdw, reverse_ms);
+    if (h->storage == WT_STORE_F16) return reverse_t<__half, float, float>(h, *R, Fbar, dJdw, reverse_ms);
+    if (h->arith == WT_ARITH_F64) return reverse_t<float, float, double>(h, *R, Fbar, dJdw, reverse_ms);
+    return reverse_t<float, float, float>(h, *R, Fbar, dJdw, reverse_ms);
 }
 
 int wt_adjoint_release(wt_lattice *h)

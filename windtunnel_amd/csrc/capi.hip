@@ -72,11 +72,14 @@ void release(wt_lattice *h)
     adj_release(h);
     for (void *p : {(void *)h->d_buf[0], (void *)h->d_solid, (void *)h->d_sbits,
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
-                    (void *)h->d_forces})
+                    (void *)h->d_forces, (void *)h->d_near})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -163,8 +166,6 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
     // both AB buffers live in one allocation, the second starting bpad
     // elements (rounded to 64) after the end of the first
     const size_t bstride = 19 * ns + (size_t)(bpad + 63) / 64 * 64;
-    const int bx = wt::row_block(nx);
-    const int64_t blocks = (int64_t)((nx + bx - 1) / bx) * ny * nz;
     int rc = 0;
     if (!rc) rc = alloc(&h->d_buf[0], (layout == WT_LAYOUT_AB ? bstride + 19 * ns : 19 * ns) * h->pop_elem);
     if (!rc && layout == WT_LAYOUT_AB) h->d_buf[1] = (char *)h->d_buf[0] + bstride * h->pop_elem;
@@ -177,11 +178,19 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
     if (!rc) rc = alloc((void **)&h->d_nbits, sizeof(uint32_t) * h->nwx * (size_t)ny * nz);
     if (!rc) rc = alloc((void **)&h->d_uw, sizeof(float) * 3 * n);
     if (!rc) rc = alloc(&h->d_mac, 4 * n * h->mac_elem);
-    h->partial_cap = blocks;
-    if (!rc) rc = alloc((void **)&h->d_partial, sizeof(double) * 6 * (size_t)blocks);
+    h->partial_cap = wt::NEAR_BLOCKS;
+    if (!rc) rc = alloc((void **)&h->d_partial, sizeof(double) * 6 * (size_t)h->partial_cap);
     h->forces_cap = 4096;
     if (!rc) rc = alloc((void **)&h->d_forces, sizeof(double) * 3 * (size_t)h->forces_cap);
     if (!rc && hipStreamCreate(&h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "hipStreamCreate failed");
+    {
+        const char *e = std::getenv("WT_NEAR_STREAM");
+        if (!rc && !(e && std::atoi(e) == 0) &&
+            (hipStreamCreate(&h->side_stream) != hipSuccess ||
+             hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+             hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess))
+            rc = wt_fail(WT_E_DEVICE, "creating the near-wall stream failed");
+    }
     if (!rc && (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess))
         rc = wt_fail(WT_E_DEVICE, "hipEventCreate failed");
     if (rc) { release(h); delete h; return rc; }
@@ -272,7 +281,6 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     std::vector<uint8_t> col((size_t)ny * nz, 0), plan((size_t)nx * nz, 0);
     std::vector<int16_t> low((size_t)nx * nz, 32767);
     int64_t cells = 0, k = 0, kw = 0;
-    int zlo = nz, zhi = -1, ylo = ny, yhi = -1;
     if (!empty) {
         for (int z = z0; z <= z1; z++)
             for (int y = y0; y <= y1; y++) {
@@ -293,8 +301,6 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
                     plan[pk] = 1;
                     if (y < low[pk]) low[pk] = (int16_t)y;
                     cells++;
-                    zlo = std::min(zlo, z); zhi = std::max(zhi, z);
-                    ylo = std::min(ylo, y); yhi = std::max(yhi, y);
                 }
             }
     }
@@ -310,14 +316,6 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     std::sort(gaps.begin(), gaps.end());
     h->body.rideHeight = gaps.empty() ? 0 : gaps[gaps.size() >> 1];
     h->p.movingWalls = mov;
-    // z rows whose fluid cells can carry body links
-    if (zhi >= 0) {
-        h->fz0 = std::max(1, zlo - 1); h->fz1 = std::min(nz - 2, zhi + 1);
-        h->fy0 = std::max(1, ylo - 1); h->fy1 = std::min(ny - 2, yhi + 1);
-    } else {
-        h->fz0 = h->fz1 = -1;
-        h->fy0 = 1; h->fy1 = 0;
-    }
 
     uint8_t *d_ops = nullptr;
     if (int rc = alloc((void **)&d_ops, (size_t)n)) return rc;

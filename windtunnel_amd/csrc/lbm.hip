@@ -137,7 +137,9 @@ template <class S, class MAC> struct StepArgs {
     int64_t prev_count;
     int64_t n, ns;              // ns: cells per population plane (whole blocks)
     Lay lay;                    // population layout
-    int py0, py1, pz0, pz1;     // rows (y,z) whose blocks write force partials
+    // near-wall cells of the launch: near[near0 .. near0 + near_cnt) (k_near)
+    const uint32_t *near;
+    int64_t near0, near_cnt;
     int nx, ny, nz, nwx, z0, mov, phase, rho_near;
     int revy, revz;  // sweep the rows with y / z descending
     double tau, sq18cs2, lambda, u0;
@@ -321,11 +323,12 @@ __device__ __forceinline__ void reduce_partials(const double *p, int64_t count, 
     if (lane == 0) { out[0] = s0; out[1] = s1; out[2] = s2; }
 }
 
-// Resident waves per SIMD the step kernel is register-budgeted for, by
-// storage type (measured on MI355X at 512x256x256): binary16 moves half the
-// bytes per load, so it needs 8 waves (64 VGPRs) in flight to cover HBM
-// latency; 4-byte storage runs best unconstrained (one 1024-thread block
-// per CU, 4 waves per SIMD). Build switches WT_F16_WAVES / WT_WAVES.
+// Resident waves per SIMD of the step kernel, by storage type (measured on
+// MI355X at 512x256x256): binary16 moves half the bytes per load, so it
+// needs 8 waves (at most 64 VGPRs) in flight to cover HBM latency; 4- and
+// 8-byte storage run best with one 1024-thread block per CU (4 waves per
+// SIMD; two blocks per CU cost ~7% at fp32). Build switches WT_F16_WAVES /
+// WT_WAVES.
 #ifndef WT_F16_WAVES
 #define WT_F16_WAVES 8
 #endif
@@ -338,14 +341,16 @@ template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO,
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES))) k_step(StepArgs<S, MAC> a)
 {
     using P = Pop<S>;
+    // Four waves per SIMD at most: claiming v127 gives every wave a
+    // 128-register file, so a second 1024-thread block does not fit a CU.
+    if constexpr (WAVES <= 4) asm volatile("" ::: "v127");
     if (a.prev_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 64)
         reduce_partials(a.prev_partial, a.prev_count, a.prev_out);
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = a.revy ? a.ny - 1 - (int)blockIdx.y : (int)blockIdx.y;
     const int z = a.z0 + (a.revz ? (int)(gridDim.z - 1 - blockIdx.z) : (int)blockIdx.z);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int ph = a.phase, nph = a.phase ^ 1;
-    double fx = 0, fy = 0, fz = 0;
+    const int nph = a.phase ^ 1;
 
     // store of one value of cell m (AB: home slot in g; esoteric: slot in the new representation)
     auto put = [&](int q, int64_t m, int cx, int cy, int cz, T v) {
@@ -425,78 +430,111 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
                     a.uy[i + 1] = a.uy[i]; a.uz[i + 1] = a.uz[i];
                 }
             }
+        }
+        // near-wall cells (interior fluid with a solid or gray upstream
+        // neighbour) are stepped by k_near
+    }
+}
+
+// One near-wall cell (an interior fluid cell with a solid or gray upstream
+// neighbour): halfway bounce-back, moving wall, momentum exchange and gray
+// links, then the collision of the straight path.
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO>
+__device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, double &fx, double &fy, double &fz)
+{
+    using P = Pop<S>;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
+    const int ph = a.phase, nph = a.phase ^ 1;
+    const bool out = x == nx - 2;  // also writes the outlet copy at x = nx-1
+    const bool out_sol = out && bit(a.sbits, a.nwx, ny, x + 1, y, z);
+    auto put = [&](int q, int64_t m, int cx, int cy, int cz, T v) {
+        if (ESO) {
+            if (slot_inside(q, cx, cy, cz, nx, ny, nz)) P::template st<T>(a.G, slot_of(q, m, nph, a.lay, nx, ny), v);
         } else {
-            // near-wall cell: halfway bounce-back, moving wall, momentum
-            // exchange (js/lbm.js:334-368)
-            T f[Q];
-            {
+            P::template st<T>(a.G, pix(q, m, a.lay), v);
+        }
+    };
+
+    // near-wall cell: halfway bounce-back, moving wall, momentum
+    // exchange (js/lbm.js:334-368). Both candidates of every link (the
+    // pulled value and the bounce-back value) are loaded up front, together
+    // with the upstream solid states, so the cell costs one memory round
+    // trip; the selects below then pick per link.
+    T f[Q];
+    {
+        int sj[Q];
+        T fp[Q], fb[Q];
+        float wj[Q];
 #pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    const int64_t j = i - cell_off(q, nx, ny);
-                    const int sj = a.solid[j];
-                    if (sj) {
-                        const int64_t k = ESO ? slot_of(OPP[q], i, ph, a.lay, nx, ny) : pix(OPP[q], i, a.lay);
-                        const T fd = P::template ld<T>(a.F, k);
-                        T corr = 0;
-                        if (a.mov) {
-                            const T cu = (T)EX[q] * (T)a.uwx[j] + (T)EY[q] * (T)a.uwy[j] + (T)EZ[q] * (T)a.uwz[j];
-                            if (cu != T(0)) {
-                                const T wq = q == 0 ? T(1.0 / 3) : (q <= 6 ? T(1.0 / 18) : T(1.0 / 36));
-                                corr = T(6) * wq * (T)a.rho[i] * cu;
-                            }
-                        }
-                        f[q] = fd + corr;
-                        if (sj == SOLID_OBJ) {
-                            const double d = (double)(T(2) * fd + corr);
-                            fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
-                        }
-                    } else if (!ESO && a.wgt && a.wgt[j] > 0.0f) {
-                        // gray upstream cell: a fraction w_j of the link bounces back
-                        const T wj = (T)a.wgt[j];
-                        const T fd = P::template ld<T>(a.F, pix(OPP[q], i, a.lay));
-                        f[q] = (T(1) - wj) * P::template ld<T>(a.F, pix(q, j, a.lay)) + wj * fd;
-                        const double d = (double)(wj * (T(2) * fd));
-                        fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
-                    } else {
-                        const int64_t k = ESO ? slot_of(q, j, ph, a.lay, nx, ny) : pix(q, j, a.lay);
-                        f[q] = P::template ld<T>(a.F, k);
-                    }
+        for (int q = 0; q < Q; q++) {
+            const int64_t j = i - cell_off(q, nx, ny);
+            sj[q] = a.solid[j];
+            fp[q] = P::template ld<T>(a.F, ESO ? slot_of(q, j, ph, a.lay, nx, ny) : pix(q, j, a.lay));
+            fb[q] = P::template ld<T>(a.F, ESO ? slot_of(OPP[q], i, ph, a.lay, nx, ny) : pix(OPP[q], i, a.lay));
+            wj[q] = (!ESO && a.wgt) ? a.wgt[j] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            T corr = 0;
+            if (a.mov && sj[q]) {
+                const int64_t j = i - cell_off(q, nx, ny);
+                const T cu = (T)EX[q] * (T)a.uwx[j] + (T)EY[q] * (T)a.uwy[j] + (T)EZ[q] * (T)a.uwz[j];
+                if (cu != T(0)) {
+                    const T wq = q == 0 ? T(1.0 / 3) : (q <= 6 ? T(1.0 / 18) : T(1.0 / 36));
+                    corr = T(6) * wq * (T)a.rho[i] * cu;
                 }
             }
-            T g[Q], r, vx, vy, vz;
-            if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
-                collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
-                                       vx, vy, vz);
-            else
-                collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                put(q, i, x, y, z, g[q]);
-                if (out && !(ESO && out_sol)) put(q, i + 1, x + 1, y, z, g[q]);
-            }
-            if (MACRO) {
-                a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
-                if (out) { a.rho[i + 1] = (MAC)r; a.ux[i + 1] = (MAC)vx; a.uy[i + 1] = (MAC)vy; a.uz[i + 1] = (MAC)vz; }
-            } else if (a.rho_near) {
-                a.rho[i] = (MAC)r;  // next step's moving-wall correction reads it
-            }
+            // solid upstream: bounce-back (plus the wall term); gray upstream:
+            // a fraction w_j of the link bounces back; else the plain pull
+            const bool gray = !sj[q] && wj[q] > 0.0f;
+            const T w = (T)wj[q];
+            const T mix = (T(1) - w) * fp[q] + w * fb[q];
+            f[q] = sj[q] ? fb[q] + corr : (gray ? mix : fp[q]);
+            const double d = sj[q] == SOLID_OBJ ? (double)(T(2) * fb[q] + corr)
+                                                : (gray ? (double)(w * (T(2) * fb[q])) : 0.0);
+            fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
         }
     }
+    T g[Q], r, vx, vy, vz;
+    if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
+        collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
+                               vx, vy, vz);
+    else
+        collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        put(q, i, x, y, z, g[q]);
+        if (out && !(ESO && out_sol)) put(q, i + 1, x + 1, y, z, g[q]);
+    }
+    if (MACRO) {
+        a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
+        if (out) { a.rho[i + 1] = (MAC)r; a.ux[i + 1] = (MAC)vx; a.uy[i + 1] = (MAC)vy; a.uz[i + 1] = (MAC)vz; }
+    } else if (a.rho_near) {
+        a.rho[i] = (MAC)r;  // next step's moving-wall correction reads it
+    }
+}
 
-    // Per-block FP64 force partial for blocks that can hold body links:
-    // wave shuffles, then the waves' sums in wave order.
-    if (y < a.py0 || y > a.py1 || z < a.pz0 || z > a.pz1) return;
-    const int64_t bid = blockIdx.x + (int64_t)gridDim.x * ((y - a.py0) + (int64_t)(a.py1 - a.py0 + 1) * (z - a.pz0));
-    __shared__ double red[16][3];
+// The near-wall cells of one step from the cell list (cell order), one
+// thread per listed cell, grid-stride. Their long dependent load chains run
+// here, apart from the streaming kernel, and each block writes its FP64
+// force partial: wave shuffles, then the waves' sums in wave order.
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO>
+__global__ void __launch_bounds__(NEAR_THREADS) k_near(StepArgs<S, MAC> a)
+{
+    double fx = 0, fy = 0, fz = 0;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < a.near_cnt;
+         k += (int64_t)gridDim.x * blockDim.x)
+        near_cell<S, T, MAC, MACRO, LES, TRT, ESO>(a, (int64_t)a.near[a.near0 + k], fx, fy, fz);
+    __shared__ double red[NEAR_THREADS / 64][3];
     fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) { red[wv][0] = fx; red[wv][1] = fy; red[wv][2] = fz; }
     __syncthreads();
     if (threadIdx.x == 0) {
         double sx = 0, sy = 0, sz = 0;
-        const int nw = blockDim.x >> 6;
-        for (int w = 0; w < nw; w++) { sx += red[w][0]; sy += red[w][1]; sz += red[w][2]; }
-        double *o = a.partial + bid * 3;
+        for (int w = 0; w < NEAR_THREADS / 64; w++) { sx += red[w][0]; sy += red[w][1]; sz += red[w][2]; }
+        double *o = a.partial + (int64_t)blockIdx.x * 3;
         o[0] = sx; o[1] = sy; o[2] = sz;
     }
 }
@@ -758,18 +796,34 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     }
     const int bx = row_block(h->nx);
     const int gx = (h->nx + bx - 1) / bx;
-    // blocks of the z rows that can hold body links write partials
-    const int lo = h->fz0 > bz0 ? h->fz0 : bz0, hi = h->fz1 < bz1 - 1 ? h->fz1 : bz1 - 1;
-    int64_t count = 0;
-    if (h->fz0 >= 0 && lo <= hi) {
-        a.py0 = h->fy0; a.py1 = h->fy1; a.pz0 = lo; a.pz1 = hi;
-        count = (int64_t)gx * (h->fy1 - h->fy0 + 1) * (hi - lo + 1);
-    } else {
-        a.py0 = 1; a.py1 = 0; a.pz0 = 1; a.pz1 = 0;
+    // the near-wall cells of rows z in [bz0, bz1); each k_near block writes one force partial
+    a.near = h->d_near;
+    a.near0 = h->near_z[(size_t)bz0];
+    a.near_cnt = h->near_z[(size_t)bz1] - a.near0;
+    const int64_t count = std::min<int64_t>((a.near_cnt + NEAR_THREADS - 1) / NEAR_THREADS, NEAR_BLOCKS);
+    // The two kernels touch disjoint cells (and, in place, disjoint slots),
+    // so the near-wall kernel runs beside the streaming kernel on the side
+    // stream: forked after everything before this step, joined before
+    // anything after it.
+    const bool side = count > 0 && h->side_stream;
+    if (side) {
+        WT_HIP(hipEventRecord(h->ev_fork, h->stream));
+        WT_HIP(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
+        hipLaunchKernelGGL((k_near<S, T, MAC, MACRO, LES, TRT, ESO>), dim3((unsigned)count), dim3(NEAR_THREADS), 0,
+                           h->side_stream, a);
+        WT_HIP(hipGetLastError());
+        WT_HIP(hipEventRecord(h->ev_join, h->side_stream));
     }
     hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO, step_waves<S>()>), dim3(gx, h->ny, bz1 - bz0), dim3(bx), 0,
                        h->stream, a);
     WT_HIP(hipGetLastError());
+    if (side) {
+        WT_HIP(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+    } else if (count > 0) {
+        hipLaunchKernelGGL((k_near<S, T, MAC, MACRO, LES, TRT, ESO>), dim3((unsigned)count), dim3(NEAR_THREADS), 0,
+                           h->stream, a);
+        WT_HIP(hipGetLastError());
+    }
     // this launch's force is reduced by the next launch, or by lbm_flush_force
     // (build switch WT_DEFER_FORCE=0: by its own launch right away)
     h->pend_partial = a.partial;
@@ -843,6 +897,39 @@ int lbm_rebuild_masks(wt_lattice *h)
     hipLaunchKernelGGL(k_masks, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, h->stream, h->d_solid,
                        h->d_wgt, h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx);
     WT_HIP(hipGetLastError());
+    // list of the near-wall cells in cell order, and where each z plane's start
+    std::vector<uint32_t> bits((size_t)nwords);
+    WT_HIP(hipMemcpyAsync(bits.data(), h->d_nbits, sizeof(uint32_t) * (size_t)nwords, hipMemcpyDeviceToHost,
+                          h->stream));
+    WT_HIP(hipStreamSynchronize(h->stream));
+    std::vector<uint32_t> list;
+    h->near_z.assign((size_t)h->nz + 1, 0);
+    for (int z = 0; z < h->nz; z++) {
+        h->near_z[(size_t)z] = (int64_t)list.size();
+        for (int y = 0; y < h->ny; y++)
+            for (int wx = 0; wx < h->nwx; wx++) {
+                uint32_t w = bits[(size_t)wx + (size_t)h->nwx * (y + (size_t)h->ny * z)];
+                while (w) {
+                    const int b = __builtin_ctz(w);
+                    w &= w - 1;
+                    list.push_back((uint32_t)(wx * 32 + b + (int64_t)h->nx * (y + (int64_t)h->ny * z)));
+                }
+            }
+    }
+    h->near_z[(size_t)h->nz] = (int64_t)list.size();
+    if ((int64_t)list.size() > h->near_cap) {
+        if (h->d_near) WT_HIP(hipFree(h->d_near));
+        h->d_near = nullptr;
+        h->near_cap = 0;
+        if (hipMalloc((void **)&h->d_near, sizeof(uint32_t) * list.size()) != hipSuccess) {
+            h->d_near = nullptr;
+            h->near_z.assign((size_t)h->nz + 1, 0);
+            return wt_fail(WT_E_NOMEM, "hipMalloc of the near-wall list (%zu cells) failed", list.size());
+        }
+        h->near_cap = (int64_t)list.size();
+    }
+    if (!list.empty())
+        WT_HIP(hipMemcpy(h->d_near, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice));
     return 0;
 }
 

@@ -66,6 +66,11 @@ constexpr float F16_INV_SCALE = 1.0f / 32768.0f;
 #ifndef WT_ROW_BLOCK_MAX
 #define WT_ROW_BLOCK_MAX 1024
 #endif
+// Near-wall kernel: threads per block, and at most this many blocks (and so
+// force partials) per launch; the blocks stride over longer lists.
+constexpr int NEAR_THREADS = 256;
+constexpr int NEAR_BLOCKS = 4096;
+
 inline int row_block(int nx) { int b = (nx + 63) / 64 * 64; return b > WT_ROW_BLOCK_MAX ? WT_ROW_BLOCK_MAX : b; }
 
 struct Body {
@@ -106,6 +111,11 @@ struct wt_lattice {
     double *d_forces = nullptr;   // per-step force totals of the current batch
     int forces_cap = 0;
     hipStream_t stream = nullptr;
+    // the near-wall kernel of a step runs on side_stream beside the
+    // streaming kernel: forked from and joined back into `stream` per step
+    // (WT_NEAR_STREAM=0 at wt_create runs it after the streaming kernel)
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // per-launch timing of the step kernel (wt_set_timing)
     int timing = 0;
@@ -119,8 +129,12 @@ struct wt_lattice {
     double force[3] = {0, 0, 0};
     int ground = 0;
     wt::Body body;
-    // inclusive z and y ranges of rows holding force-carrying (near-body) cells; fz0 = -1: none
-    int fz0 = -1, fz1 = -1, fy0 = 1, fy1 = 0;
+    // near-wall cells (interior fluid cells with a solid or gray upstream
+    // neighbour) in cell order; near_z[z] is the index of the first one in
+    // plane z (nz + 1 entries), so a z range is one slice of the list
+    uint32_t *d_near = nullptr;
+    int64_t near_cap = 0;
+    std::vector<int64_t> near_z;
 
     std::vector<uint8_t> h_solid;  // host mirror of the solid array
     std::vector<float> h_uw;       // host mirror of the wall velocities (3n)
