@@ -77,9 +77,6 @@ void release(wt_lattice *h)
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
-    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
-    if (h->side_stream) (void)hipStreamDestroy(h->side_stream);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -178,19 +175,15 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
     if (!rc) rc = alloc((void **)&h->d_nbits, sizeof(uint32_t) * h->nwx * (size_t)ny * nz);
     if (!rc) rc = alloc((void **)&h->d_uw, sizeof(float) * 3 * n);
     if (!rc) rc = alloc(&h->d_mac, 4 * n * h->mac_elem);
-    h->partial_cap = wt::NEAR_BLOCKS;
+    // force partials: one per near-wall block, at most (nz + 1) planes of
+    // row blocks (every listed cell is a lattice cell)
+    const int bx = wt::row_block(nx);
+    h->partial_cap = (int64_t)((nx + bx - 1) / bx) * ny * ((int64_t)nz + 1);
     if (!rc) rc = alloc((void **)&h->d_partial, sizeof(double) * 6 * (size_t)h->partial_cap);
     h->forces_cap = 4096;
     if (!rc) rc = alloc((void **)&h->d_forces, sizeof(double) * 3 * (size_t)h->forces_cap);
     if (!rc && hipStreamCreate(&h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "hipStreamCreate failed");
-    {
-        const char *e = std::getenv("WT_NEAR_STREAM");
-        if (!rc && !(e && std::atoi(e) == 0) &&
-            (hipStreamCreate(&h->side_stream) != hipSuccess ||
-             hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
-             hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess))
-            rc = wt_fail(WT_E_DEVICE, "creating the near-wall stream failed");
-    }
+
     if (!rc && (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess))
         rc = wt_fail(WT_E_DEVICE, "hipEventCreate failed");
     if (rc) { release(h); delete h; return rc; }

@@ -137,9 +137,11 @@ template <class S, class MAC> struct StepArgs {
     int64_t prev_count;
     int64_t n, ns;              // ns: cells per population plane (whole blocks)
     Lay lay;                    // population layout
-    // near-wall cells of the launch: near[near0 .. near0 + near_cnt) (k_near)
+    // near-wall cells of the launch: near[near0 .. near0 + near_cnt), taken
+    // by the blocks of the first near_planes z planes of the grid
     const uint32_t *near;
     int64_t near0, near_cnt;
+    int near_planes;
     int nx, ny, nz, nwx, z0, mov, phase, rho_near;
     int revy, revz;  // sweep the rows with y / z descending
     double tau, sq18cs2, lambda, u0;
@@ -337,6 +339,141 @@ __device__ __forceinline__ void reduce_partials(const double *p, int64_t count, 
 #endif
 template <class S> constexpr int step_waves() { return std::is_same<S, __half>::value ? WT_F16_WAVES : WT_WAVES; }
 
+// One near-wall cell (an interior fluid cell with a solid or gray upstream
+// neighbour): halfway bounce-back, moving wall, momentum exchange and gray
+// links, then the collision of the straight path.
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO, bool WIDE>
+__device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, double &fx, double &fy, double &fz)
+{
+    using P = Pop<S>;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
+    const int ph = a.phase, nph = a.phase ^ 1;
+    const bool out = x == nx - 2;  // also writes the outlet copy at x = nx-1
+    const bool out_sol = out && bit(a.sbits, a.nwx, ny, x + 1, y, z);
+    auto put = [&](int q, int64_t m, int cx, int cy, int cz, T v) {
+        if (ESO) {
+            if (slot_inside(q, cx, cy, cz, nx, ny, nz)) P::template st<T>(a.G, slot_of(q, m, nph, a.lay, nx, ny), v);
+        } else {
+            P::template st<T>(a.G, pix(q, m, a.lay), v);
+        }
+    };
+
+    // near-wall cell: halfway bounce-back, moving wall, momentum
+    // exchange (js/lbm.js:334-368)
+    T f[Q];
+    if constexpr (!WIDE) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int64_t j = i - cell_off(q, nx, ny);
+            const int sj = a.solid[j];
+            if (sj) {
+                const int64_t k = ESO ? slot_of(OPP[q], i, ph, a.lay, nx, ny) : pix(OPP[q], i, a.lay);
+                const T fd = P::template ld<T>(a.F, k);
+                T corr = 0;
+                if (a.mov) {
+                    const T cu = (T)EX[q] * (T)a.uwx[j] + (T)EY[q] * (T)a.uwy[j] + (T)EZ[q] * (T)a.uwz[j];
+                    if (cu != T(0)) {
+                        const T wq = q == 0 ? T(1.0 / 3) : (q <= 6 ? T(1.0 / 18) : T(1.0 / 36));
+                        corr = T(6) * wq * (T)a.rho[i] * cu;
+                    }
+                }
+                f[q] = fd + corr;
+                if (sj == SOLID_OBJ) {
+                    const double d = (double)(T(2) * fd + corr);
+                    fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
+                }
+            } else if (!ESO && a.wgt && a.wgt[j] > 0.0f) {
+                // gray upstream cell: a fraction w_j of the link bounces back
+                const T wj = (T)a.wgt[j];
+                const T fd = P::template ld<T>(a.F, pix(OPP[q], i, a.lay));
+                f[q] = (T(1) - wj) * P::template ld<T>(a.F, pix(q, j, a.lay)) + wj * fd;
+                const double d = (double)(wj * (T(2) * fd));
+                fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
+            } else {
+                const int64_t k = ESO ? slot_of(q, j, ph, a.lay, nx, ny) : pix(q, j, a.lay);
+                f[q] = P::template ld<T>(a.F, k);
+            }
+        }
+    }
+    else {
+        // WIDE: both candidates of every link (the pulled value and the
+        // bounce-back value) are loaded up front with the upstream solid
+        // states, so the cell costs one memory round trip (more registers)
+        int sj[Q];
+        T fp[Q], fb[Q];
+        float wj[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int64_t j = i - cell_off(q, nx, ny);
+            sj[q] = a.solid[j];
+            fp[q] = P::template ld<T>(a.F, ESO ? slot_of(q, j, ph, a.lay, nx, ny) : pix(q, j, a.lay));
+            fb[q] = P::template ld<T>(a.F, ESO ? slot_of(OPP[q], i, ph, a.lay, nx, ny) : pix(OPP[q], i, a.lay));
+            wj[q] = (!ESO && a.wgt) ? a.wgt[j] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            T corr = 0;
+            if (a.mov && sj[q]) {
+                const int64_t j = i - cell_off(q, nx, ny);
+                const T cu = (T)EX[q] * (T)a.uwx[j] + (T)EY[q] * (T)a.uwy[j] + (T)EZ[q] * (T)a.uwz[j];
+                if (cu != T(0)) {
+                    const T wq = q == 0 ? T(1.0 / 3) : (q <= 6 ? T(1.0 / 18) : T(1.0 / 36));
+                    corr = T(6) * wq * (T)a.rho[i] * cu;
+                }
+            }
+            const bool gray = !sj[q] && wj[q] > 0.0f;
+            const T w = (T)wj[q];
+            const T mix = (T(1) - w) * fp[q] + w * fb[q];
+            f[q] = sj[q] ? fb[q] + corr : (gray ? mix : fp[q]);
+            const double d = sj[q] == SOLID_OBJ ? (double)(T(2) * fb[q] + corr)
+                                                : (gray ? (double)(w * (T(2) * fb[q])) : 0.0);
+            fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
+        }
+    }
+    T g[Q], r, vx, vy, vz;
+    if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
+        collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
+                               vx, vy, vz);
+    else
+        collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        put(q, i, x, y, z, g[q]);
+        if (out && !(ESO && out_sol)) put(q, i + 1, x + 1, y, z, g[q]);
+    }
+    if (MACRO) {
+        a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
+        if (out) { a.rho[i + 1] = (MAC)r; a.ux[i + 1] = (MAC)vx; a.uy[i + 1] = (MAC)vy; a.uz[i + 1] = (MAC)vz; }
+    } else if (a.rho_near) {
+        a.rho[i] = (MAC)r;  // next step's moving-wall correction reads it
+    }
+}
+
+// Where the near-wall cells are stepped: binary16 storage keeps the
+// streaming kernel within 64 VGPRs (8 waves per SIMD), where the near-wall
+// path would spill, so it steps them in a launch of its own (k_near) after
+// the streaming kernel; wider storage runs them as the first blocks of the
+// streaming launch (no second launch, no gap between launches).
+template <class S> constexpr bool fused_near() { return !std::is_same<S, __half>::value; }
+
+// Block sum of the per-thread force: wave shuffles, then the waves' sums in
+// wave order, written by thread 0 to out[0..2].
+__device__ __forceinline__ void block_force(double fx, double fy, double fz, double *out)
+{
+    __shared__ double red[16][3];
+    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { red[wv][0] = fx; red[wv][1] = fy; red[wv][2] = fz; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sx = 0, sy = 0, sz = 0;
+        const int nw = blockDim.x >> 6;
+        for (int w = 0; w < nw; w++) { sx += red[w][0]; sy += red[w][1]; sz += red[w][2]; }
+        out[0] = sx; out[1] = sy; out[2] = sz;
+    }
+}
+
 template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO, int WAVES>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES))) k_step(StepArgs<S, MAC> a)
 {
@@ -346,9 +483,25 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
     if constexpr (WAVES <= 4) asm volatile("" ::: "v127");
     if (a.prev_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 64)
         reduce_partials(a.prev_partial, a.prev_count, a.prev_out);
+    if constexpr (fused_near<S>()) {
+        if ((int)blockIdx.z < a.near_planes) {
+            // The first near_planes z planes of the grid are near-wall
+            // blocks: one listed cell per thread. They are dispatched first,
+            // so their long dependent load chains overlap the streaming
+            // blocks; each writes its FP64 force partial.
+            const int64_t nb = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+            const int64_t k = nb * blockDim.x + threadIdx.x;
+            double fx = 0, fy = 0, fz = 0;
+            if (k < a.near_cnt)
+                near_cell<S, T, MAC, MACRO, LES, TRT, ESO, false>(a, (int64_t)a.near[a.near0 + k], fx, fy, fz);
+            block_force(fx, fy, fz, a.partial + nb * 3);
+            return;
+        }
+    }
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = a.revy ? a.ny - 1 - (int)blockIdx.y : (int)blockIdx.y;
-    const int z = a.z0 + (a.revz ? (int)(gridDim.z - 1 - blockIdx.z) : (int)blockIdx.z);
+    const int zb = (int)blockIdx.z - a.near_planes, nzr = (int)gridDim.z - a.near_planes;
+    const int z = a.z0 + (a.revz ? nzr - 1 - zb : zb);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int nph = a.phase ^ 1;
 
@@ -432,111 +585,23 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
             }
         }
         // near-wall cells (interior fluid with a solid or gray upstream
-        // neighbour) are stepped by k_near
+        // neighbour) are stepped by the near-wall blocks
     }
 }
 
-// One near-wall cell (an interior fluid cell with a solid or gray upstream
-// neighbour): halfway bounce-back, moving wall, momentum exchange and gray
-// links, then the collision of the straight path.
-template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO>
-__device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, double &fx, double &fy, double &fz)
-{
-    using P = Pop<S>;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
-    const int ph = a.phase, nph = a.phase ^ 1;
-    const bool out = x == nx - 2;  // also writes the outlet copy at x = nx-1
-    const bool out_sol = out && bit(a.sbits, a.nwx, ny, x + 1, y, z);
-    auto put = [&](int q, int64_t m, int cx, int cy, int cz, T v) {
-        if (ESO) {
-            if (slot_inside(q, cx, cy, cz, nx, ny, nz)) P::template st<T>(a.G, slot_of(q, m, nph, a.lay, nx, ny), v);
-        } else {
-            P::template st<T>(a.G, pix(q, m, a.lay), v);
-        }
-    };
-
-    // near-wall cell: halfway bounce-back, moving wall, momentum
-    // exchange (js/lbm.js:334-368). Both candidates of every link (the
-    // pulled value and the bounce-back value) are loaded up front, together
-    // with the upstream solid states, so the cell costs one memory round
-    // trip; the selects below then pick per link.
-    T f[Q];
-    {
-        int sj[Q];
-        T fp[Q], fb[Q];
-        float wj[Q];
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int64_t j = i - cell_off(q, nx, ny);
-            sj[q] = a.solid[j];
-            fp[q] = P::template ld<T>(a.F, ESO ? slot_of(q, j, ph, a.lay, nx, ny) : pix(q, j, a.lay));
-            fb[q] = P::template ld<T>(a.F, ESO ? slot_of(OPP[q], i, ph, a.lay, nx, ny) : pix(OPP[q], i, a.lay));
-            wj[q] = (!ESO && a.wgt) ? a.wgt[j] : 0.0f;
-        }
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            T corr = 0;
-            if (a.mov && sj[q]) {
-                const int64_t j = i - cell_off(q, nx, ny);
-                const T cu = (T)EX[q] * (T)a.uwx[j] + (T)EY[q] * (T)a.uwy[j] + (T)EZ[q] * (T)a.uwz[j];
-                if (cu != T(0)) {
-                    const T wq = q == 0 ? T(1.0 / 3) : (q <= 6 ? T(1.0 / 18) : T(1.0 / 36));
-                    corr = T(6) * wq * (T)a.rho[i] * cu;
-                }
-            }
-            // solid upstream: bounce-back (plus the wall term); gray upstream:
-            // a fraction w_j of the link bounces back; else the plain pull
-            const bool gray = !sj[q] && wj[q] > 0.0f;
-            const T w = (T)wj[q];
-            const T mix = (T(1) - w) * fp[q] + w * fb[q];
-            f[q] = sj[q] ? fb[q] + corr : (gray ? mix : fp[q]);
-            const double d = sj[q] == SOLID_OBJ ? (double)(T(2) * fb[q] + corr)
-                                                : (gray ? (double)(w * (T(2) * fb[q])) : 0.0);
-            fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
-        }
-    }
-    T g[Q], r, vx, vy, vz;
-    if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
-        collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
-                               vx, vy, vz);
-    else
-        collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        put(q, i, x, y, z, g[q]);
-        if (out && !(ESO && out_sol)) put(q, i + 1, x + 1, y, z, g[q]);
-    }
-    if (MACRO) {
-        a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
-        if (out) { a.rho[i + 1] = (MAC)r; a.ux[i + 1] = (MAC)vx; a.uy[i + 1] = (MAC)vy; a.uz[i + 1] = (MAC)vz; }
-    } else if (a.rho_near) {
-        a.rho[i] = (MAC)r;  // next step's moving-wall correction reads it
-    }
-}
-
-// The near-wall cells of one step from the cell list (cell order), one
-// thread per listed cell, grid-stride. Their long dependent load chains run
-// here, apart from the streaming kernel, and each block writes its FP64
-// force partial: wave shuffles, then the waves' sums in wave order.
+// The near-wall cells of one step as a launch of their own (binary16
+// storage): 256-thread blocks over the list (grid-stride when it is longer
+// than the partial buffer allows), the one-round-trip gather, one force
+// partial per block.
+constexpr int NEAR_THREADS = 256;
 template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO>
 __global__ void __launch_bounds__(NEAR_THREADS) k_near(StepArgs<S, MAC> a)
 {
     double fx = 0, fy = 0, fz = 0;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < a.near_cnt;
          k += (int64_t)gridDim.x * blockDim.x)
-        near_cell<S, T, MAC, MACRO, LES, TRT, ESO>(a, (int64_t)a.near[a.near0 + k], fx, fy, fz);
-    __shared__ double red[NEAR_THREADS / 64][3];
-    fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { red[wv][0] = fx; red[wv][1] = fy; red[wv][2] = fz; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double sx = 0, sy = 0, sz = 0;
-        for (int w = 0; w < NEAR_THREADS / 64; w++) { sx += red[w][0]; sy += red[w][1]; sz += red[w][2]; }
-        double *o = a.partial + (int64_t)blockIdx.x * 3;
-        o[0] = sx; o[1] = sy; o[2] = sz;
-    }
+        near_cell<S, T, MAC, MACRO, LES, TRT, ESO, true>(a, (int64_t)a.near[a.near0 + k], fx, fy, fz);
+    block_force(fx, fy, fz, a.partial + (int64_t)blockIdx.x * 3);
 }
 
 // A pending force reduction on its own (one wavefront).
@@ -796,33 +861,26 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     }
     const int bx = row_block(h->nx);
     const int gx = (h->nx + bx - 1) / bx;
-    // the near-wall cells of rows z in [bz0, bz1); each k_near block writes one force partial
+    // the near-wall cells of rows z in [bz0, bz1), one per thread of the
+    // near-wall blocks; each of those blocks writes one force partial
     a.near = h->d_near;
     a.near0 = h->near_z[(size_t)bz0];
     a.near_cnt = h->near_z[(size_t)bz1] - a.near0;
-    const int64_t count = std::min<int64_t>((a.near_cnt + NEAR_THREADS - 1) / NEAR_THREADS, NEAR_BLOCKS);
-    // The two kernels touch disjoint cells (and, in place, disjoint slots),
-    // so the near-wall kernel runs beside the streaming kernel on the side
-    // stream: forked after everything before this step, joined before
-    // anything after it.
-    const bool side = count > 0 && h->side_stream;
-    if (side) {
-        WT_HIP(hipEventRecord(h->ev_fork, h->stream));
-        WT_HIP(hipStreamWaitEvent(h->side_stream, h->ev_fork, 0));
-        hipLaunchKernelGGL((k_near<S, T, MAC, MACRO, LES, TRT, ESO>), dim3((unsigned)count), dim3(NEAR_THREADS), 0,
-                           h->side_stream, a);
-        WT_HIP(hipGetLastError());
-        WT_HIP(hipEventRecord(h->ev_join, h->side_stream));
-    }
-    hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO, step_waves<S>()>), dim3(gx, h->ny, bz1 - bz0), dim3(bx), 0,
-                       h->stream, a);
+    const int64_t row_blocks = (int64_t)gx * h->ny;
+    const int64_t near_blocks = fused_near<S>() ? (a.near_cnt + bx - 1) / bx
+                                                : std::min<int64_t>((a.near_cnt + NEAR_THREADS - 1) / NEAR_THREADS,
+                                                                    h->partial_cap);
+    a.near_planes = fused_near<S>() ? (int)((near_blocks + row_blocks - 1) / row_blocks) : 0;
+    const int64_t count = fused_near<S>() ? a.near_planes * row_blocks : near_blocks;
+    hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO, step_waves<S>()>),
+                       dim3(gx, h->ny, bz1 - bz0 + a.near_planes), dim3(bx), 0, h->stream, a);
     WT_HIP(hipGetLastError());
-    if (side) {
-        WT_HIP(hipStreamWaitEvent(h->stream, h->ev_join, 0));
-    } else if (count > 0) {
-        hipLaunchKernelGGL((k_near<S, T, MAC, MACRO, LES, TRT, ESO>), dim3((unsigned)count), dim3(NEAR_THREADS), 0,
-                           h->stream, a);
-        WT_HIP(hipGetLastError());
+    if constexpr (!fused_near<S>()) {
+        if (near_blocks > 0) {
+            hipLaunchKernelGGL((k_near<S, T, MAC, MACRO, LES, TRT, ESO>), dim3((unsigned)near_blocks),
+                               dim3(NEAR_THREADS), 0, h->stream, a);
+            WT_HIP(hipGetLastError());
+        }
     }
     // this launch's force is reduced by the next launch, or by lbm_flush_force
     // (build switch WT_DEFER_FORCE=0: by its own launch right away)

@@ -66,11 +66,6 @@ constexpr float F16_INV_SCALE = 1.0f / 32768.0f;
 #ifndef WT_ROW_BLOCK_MAX
 #define WT_ROW_BLOCK_MAX 1024
 #endif
-// Near-wall kernel: threads per block, and at most this many blocks (and so
-// force partials) per launch; the blocks stride over longer lists.
-constexpr int NEAR_THREADS = 256;
-constexpr int NEAR_BLOCKS = 4096;
-
 inline int row_block(int nx) { int b = (nx + 63) / 64 * 64; return b > WT_ROW_BLOCK_MAX ? WT_ROW_BLOCK_MAX : b; }
 
 struct Body {
@@ -111,11 +106,6 @@ struct wt_lattice {
     double *d_forces = nullptr;   // per-step force totals of the current batch
     int forces_cap = 0;
     hipStream_t stream = nullptr;
-    // the near-wall kernel of a step runs on side_stream beside the
-    // streaming kernel: forked from and joined back into `stream` per step
-    // (WT_NEAR_STREAM=0 at wt_create runs it after the streaming kernel)
-    hipStream_t side_stream = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // per-launch timing of the step kernel (wt_set_timing)
     int timing = 0;
