@@ -343,9 +343,7 @@ def run_scale(N=3000, K=500):
     steps, then the checkpointed adjoint of the mean C_d over the last K
     steps with respect to a gray skin (weight 0.5 on the first fluid layer
     around the body)."""
-    import torch
-
-    from windtunnel_amd.lbm import LBM, Voxel
+    from windtunnel_amd.lbm import LBM, Voxel, device_memory
     from windtunnel_amd.synth import noisy_sphere
     dims = (640, 320, 320)
     res = dict(grid="640x320x320", storage="f16", steps=N, tau=0.505, u0=0.08, Lambda=3 / 16, Cs=0.14)
@@ -386,7 +384,7 @@ def run_scale(N=3000, K=500):
     t0 = time.perf_counter()
     F = L.adjoint_forward(N)
     fwd = time.perf_counter() - t0
-    free, total = torch.cuda.mem_get_info()
+    free, total = device_memory()
     g, rev_ms = L.adjoint_backward(Fbar)
     L.adjoint_release()
     gray = np.flatnonzero(g)

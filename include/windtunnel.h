@@ -205,6 +205,11 @@ int wt_get_walls(const wt_lattice *h, float *uwx, float *uwy, float *uwz);
  * stream (no host sync inside the timed region). */
 int wt_time_steps(wt_lattice *h, int nsteps, double *ms);
 
+/* Free and total bytes of the current device's memory (hipMemGetInfo), for
+ * sizing long adjoint runs (no reference counterpart): the checkpoints of
+ * wt_adjoint_forward take at most half of what is free. */
+int wt_device_memory(uint64_t *free_bytes, uint64_t *total_bytes);
+
 /* Per-launch timing of the step kernel: while on, every step kernel is
  * bracketed by HIP events on the lattice's stream and its duration added to
  * a running total (the force-sum kernel is excluded). Turning it on clears

@@ -19,6 +19,16 @@ const Voxel = require(path.join(JS, 'voxel.js'));
   try { new LBM(16, 8, 8); } catch (e) { err = e; }
   assert(err && err.code === 'WT-2', 'no device -> WT_E_DEVICE, got ' + (err && err.code));
 
+  // the addon binds the C ABI one to one
+  const addon = require(path.join(JS, '..', 'wt_napi.node'));
+  const exported = ('abiVersion create setParams getParams reset voxelize voxelizeGrid voxelizeParts step ' +
+    'stepSlabs stepSlab boundaries setParity getState getFields getPops getSolid getWalls timeSteps ' +
+    'deviceMemory setGray adjointForward adjointBackward adjointRelease voxelBounds voxelPlan voxelBuild').split(' ');
+  assert(Object.keys(addon).sort().join() === exported.sort().join(), 'addon exports: ' + Object.keys(addon).join(' '));
+  err = null;
+  try { addon.deviceMemory(); } catch (e) { err = e; }
+  assert(err && err.code === 'WT-2', 'deviceMemory without a device -> WT_E_DEVICE');
+
   const ref = load('vox_errors');
   const msg = f => { try { f(); return null; } catch (e) { return e.message; } };
   assert(msg(() => Voxel.build(new Float32Array(0), 96)) === ref.empty, 'empty');

@@ -634,6 +634,18 @@ static napi_value js_time_steps(napi_env env, napi_callback_info info)
     return out;
 }
 
+static napi_value js_device_memory(napi_env env, napi_callback_info info)
+{
+    (void)info;
+    uint64_t f = 0, t = 0;
+    WT(env, wt_device_memory(&f, &t));
+    napi_value o;
+    CALL(env, napi_create_object(env, &o));
+    set_num(env, o, "free", (double)f);
+    set_num(env, o, "total", (double)t);
+    return o;
+}
+
 static napi_value meta_object(napi_env env, const wt_voxmeta *m)
 {
     napi_value o, arr;
@@ -727,7 +739,7 @@ static napi_value init(napi_env env, napi_value exports)
         {"getParams", js_get_params}, {"reset", js_reset}, {"voxelize", js_voxelize},
         {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs},
         {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
-        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps},
+        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory},
         {"voxelizeParts", js_voxelize_parts}, {"getWalls", js_get_walls}, {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},
         {"adjointRelease", js_adjoint_release},
         {"voxelBounds", js_voxel_bounds}, {"voxelPlan", js_voxel_plan}, {"voxelBuild", js_voxel_build}};

@@ -94,7 +94,11 @@ template <class S, class M> struct AdjArgs {
     // term (row >> lsh) * lstride and the lane term of the x shift, as in
     // the forward step (rows are whole storage blocks).
     const uint32_t *hot;
-    const uint32_t *rows;  // dispatch order of the rows (row id y + ny z per workgroup row)
+    // cold cells (computed, not hot) in cell order, taken by the blocks of
+    // the first cold_planes z planes of the grid
+    const uint32_t *cold;
+    int64_t ncold;
+    int cold_planes;
     int64_t muo[Q], spo[Q], sto[Q];
     int lsh, lstride, lmask;
 };
@@ -138,107 +142,122 @@ template <class T, class E> __device__ __forceinline__ T ld_el(const E *base, in
     return sval<T>((const E *)((const char *)(base + (uoff + rb)) + lane), 0);
 }
 
+// Workgroups of at most ADJ_THREADS threads: the cold path holds several
+// 19-value arrays at once, and a 256-thread workgroup lets it keep them in
+// registers (1024 threads would cap every wave at 128 VGPRs and spill).
 // Build switch: WT_ADJ_WAVES asks for that many resident waves per SIMD
-// (a register budget) for the adjoint kernel.
+// (a register budget) instead.
+constexpr int ADJ_THREADS = 256;
+inline int adj_block(int nx) { return std::min(wt::row_block(nx), ADJ_THREADS); }
 #ifdef WT_ADJ_WAVES
-#define WT_ADJ_ATTR __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WT_ADJ_WAVES)))
+#define WT_ADJ_ATTR __launch_bounds__(ADJ_THREADS) __attribute__((amdgpu_waves_per_eu(WT_ADJ_WAVES)))
 #else
-#define WT_ADJ_ATTR __launch_bounds__(1024)
+#define WT_ADJ_ATTR __launch_bounds__(ADJ_THREADS)
 #endif
 
-// One reverse step: one workgroup per lattice row (as the forward step),
-// hot cells on the straight path, the other computed cells (near-wall and
-// gray cells) through cold_cell.
+// One cold cell (near-wall or gray, or next to a gray cell): the general
+// reverse step. Every load is issued up front; links whose far cell is not
+// computed read mu = 0 there (mu is only ever written at computed cells) or
+// are masked by a select.
+template <class S, class M, class T, bool LES, bool TRT>
+__device__ __forceinline__ void adj_cold(const AdjArgs<S, M> &a, int x, int y, int z)
+{
+    const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
+    const bool nr = bitset(a.nbits, a.nwx, a.ny, x, y, z);
+    const float wi = a.wgt ? a.wgt[i] : 0.0f;
+    T mun[Q], muh[Q];  // mu_k at i + c_q, and at i itself
+    int sj[Q];         // solid state of i + c_q
+    float wj[Q];       // gray weight of i + c_q
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int64_t n1 = i + off_of(q, a.nx, a.ny);
+        mun[q] = (T)a.mu[pix(q, n1, a.lay)];
+        muh[q] = (T)a.mu[pix(q, i, a.lay)];
+        sj[q] = a.solid[n1];
+        wj[q] = a.wgt ? a.wgt[n1] : 0.0f;
+    }
+    // lambda_k(i): gray-aware streaming, plus bounce-back links of a near cell
+    T lam[Q];
+    const T wm = (T)wi;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        T v = (T(1) - wm) * mun[q];
+        const double w = sj[q] ? 1.0 : (double)wj[q];
+        if (nr && w > 0) {
+            const int qb = OPP[q];
+            const bool flink = sj[q] == wt::SOLID_OBJ || (sj[q] == 0 && wj[q] > 0.0f);
+            const double cF = flink ? EX[qb] * a.Fb[0] + EY[qb] * a.Fb[1] + EZ[qb] * a.Fb[2] : 0.0;
+            v += (T)w * (muh[qb] - T(2) * (T)cF);
+        }
+        lam[q] = v;
+    }
+    if (wi > 0.0f) {
+        // dJ/dw_i over the links leaving i into computed cells
+        double sk[Q], fdn[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            sk[q] = sval<double>(a.Sk, pix(q, i, a.lay));
+            fdn[q] = sval<double>(a.Sk, pix(OPP[q], i + off_of(q, a.nx, a.ny), a.lay));
+        }
+        double g = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
+            const bool comp = X >= 1 && X <= a.nx - 2 && Y >= 1 && Y <= a.ny - 2 && Z >= 1 && Z <= a.nz - 2 && !sj[q];
+            if (comp)
+                g += (double)mun[q] * (fdn[q] - sk[q]) - 2 * fdn[q] * (EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2]);
+        }
+        a.dJdw[i] += g;
+    }
+    if (!a.Sp) return;
+    if (x == a.nx - 2) {
+        // the outlet copies this cell: fold in lambda of x = nx-1
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            if (EX[q] != 1) lam[q] += (T)a.mu[pix(q, i + 1 + off_of(q, a.nx, a.ny), a.lay)];
+    }
+    // pre-collision populations of step k-1 (gray-aware pull from j = i - c_q = i + c_qbar)
+    T spj[Q], sph[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        spj[q] = sval<T>(a.Sp, pix(q, i - off_of(q, a.nx, a.ny), a.lay));
+        sph[q] = sval<T>(a.Sp, pix(q, i, a.lay));
+    }
+    T f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int qb = OPP[q];
+        const double w = !nr ? 0.0 : (sj[qb] ? 1.0 : (double)wj[qb]);
+        if (w == 1) f[q] = sph[qb];
+        else if (w > 0) f[q] = (T(1) - (T)w) * spj[q] + (T)w * sph[qb];
+        else f[q] = spj[q];
+    }
+    T mu[Q];
+    adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
+#pragma unroll
+    for (int q = 0; q < Q; q++) a.mun[pix(q, i, a.lay)] = (M)mu[q];
+}
+
+// One reverse step. The first cold_planes z planes of the grid are cold
+// blocks, one listed cold cell per thread, dispatched first so their long
+// dependent load chains overlap the rest; the other blocks are one per
+// lattice row (as the forward step) and take the hot cells on the
+// straight path.
 template <class S, class M, class T, bool LES, bool TRT>
 __global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S, M> a)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    // rows holding cold cells come first in the dispatch order, so their
-    // long dependent load chains overlap the whole sweep
-    const uint32_t row = a.rows[blockIdx.y + (int64_t)gridDim.y * blockIdx.z];
-    const int y = (int)(row % (uint32_t)a.ny), z = (int)(row / (uint32_t)a.ny);
-    if (x >= a.nx) return;
-    if (!bitset(a.hot, a.nwx, a.ny, x, y, z)) {
-        if (!computed(a, x, y, z)) return;  // outlet lambdas are folded in by the x = nx-2 cells
-        // Cold cell (near-wall, gray). Every load is issued up front; links
-        // whose far cell is not computed read mu = 0 there (mu is only ever
-        // written at computed cells) or are masked by a select.
-        const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
-        const bool nr = bitset(a.nbits, a.nwx, a.ny, x, y, z);
-        const float wi = a.wgt ? a.wgt[i] : 0.0f;
-        T mun[Q], muh[Q];  // mu_k at i + c_q, and at i itself
-        int sj[Q];         // solid state of i + c_q
-        float wj[Q];       // gray weight of i + c_q
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int64_t n1 = i + off_of(q, a.nx, a.ny);
-            mun[q] = (T)a.mu[pix(q, n1, a.lay)];
-            muh[q] = (T)a.mu[pix(q, i, a.lay)];
-            sj[q] = a.solid[n1];
-            wj[q] = a.wgt ? a.wgt[n1] : 0.0f;
-        }
-        // lambda_k(i): gray-aware streaming, plus bounce-back links of a near cell
-        T lam[Q];
-        const T wm = (T)wi;
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            T v = (T(1) - wm) * mun[q];
-            const double w = sj[q] ? 1.0 : (double)wj[q];
-            if (nr && w > 0) {
-                const int qb = OPP[q];
-                const bool flink = sj[q] == wt::SOLID_OBJ || (sj[q] == 0 && wj[q] > 0.0f);
-                const double cF = flink ? EX[qb] * a.Fb[0] + EY[qb] * a.Fb[1] + EZ[qb] * a.Fb[2] : 0.0;
-                v += (T)w * (muh[qb] - T(2) * (T)cF);
-            }
-            lam[q] = v;
-        }
-        if (wi > 0.0f) {
-            // dJ/dw_i over the links leaving i into computed cells
-            double sk[Q], fdn[Q];
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                sk[q] = sval<double>(a.Sk, pix(q, i, a.lay));
-                fdn[q] = sval<double>(a.Sk, pix(OPP[q], i + off_of(q, a.nx, a.ny), a.lay));
-            }
-            double g = 0;
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
-                const bool comp = X >= 1 && X <= a.nx - 2 && Y >= 1 && Y <= a.ny - 2 && Z >= 1 && Z <= a.nz - 2 && !sj[q];
-                if (comp)
-                    g += (double)mun[q] * (fdn[q] - sk[q]) - 2 * fdn[q] * (EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2]);
-            }
-            a.dJdw[i] += g;
-        }
-        if (!a.Sp) return;
-        if (x == a.nx - 2) {
-            // the outlet copies this cell: fold in lambda of x = nx-1
-#pragma unroll
-            for (int q = 0; q < Q; q++)
-                if (EX[q] != 1) lam[q] += (T)a.mu[pix(q, i + 1 + off_of(q, a.nx, a.ny), a.lay)];
-        }
-        // pre-collision populations of step k-1 (gray-aware pull from j = i - c_q = i + c_qbar)
-        T spj[Q], sph[Q];
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            spj[q] = sval<T>(a.Sp, pix(q, i - off_of(q, a.nx, a.ny), a.lay));
-            sph[q] = sval<T>(a.Sp, pix(q, i, a.lay));
-        }
-        T f[Q];
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int qb = OPP[q];
-            const double w = !nr ? 0.0 : (sj[qb] ? 1.0 : (double)wj[qb]);
-            if (w == 1) f[q] = sph[qb];
-            else if (w > 0) f[q] = (T(1) - (T)w) * spj[q] + (T)w * sph[qb];
-            else f[q] = spj[q];
-        }
-        T mu[Q];
-        adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
-#pragma unroll
-        for (int q = 0; q < Q; q++) a.mun[pix(q, i, a.lay)] = (M)mu[q];
+    if ((int)blockIdx.z < a.cold_planes) {
+        const int64_t nb = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+        const int64_t k = nb * blockDim.x + threadIdx.x;
+        if (k >= a.ncold) return;
+        const int64_t i = a.cold[k];
+        const int x = (int)(i % a.nx), y = (int)((i / a.nx) % a.ny), z = (int)(i / ((int64_t)a.nx * a.ny));
+        adj_cold<S, M, T, LES, TRT>(a, x, y, z);
         return;
     }
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = (int)blockIdx.y, z = (int)blockIdx.z - a.cold_planes;
+    if (x >= a.nx || !bitset(a.hot, a.nwx, a.ny, x, y, z)) return;
     {
         // no gray weight in reach and every neighbour computed: lambda_k is
         // the plain gather of mu_k, f_{k-1} the plain pull of s_{k-1}
@@ -293,8 +312,8 @@ __global__ void __launch_bounds__(256) k_adj_cold_list(const uint32_t *sbits, co
 template <class S, class M, class T>
 int launch_adj(wt_lattice *h, const AdjArgs<S, M> &a)
 {
-    const int bx = wt::row_block(h->nx), gx = (h->nx + bx - 1) / bx;
-    const dim3 grid(gx, h->ny, h->nz);
+    const int bx = adj_block(h->nx), gx = (h->nx + bx - 1) / bx;
+    const dim3 grid(gx, h->ny, h->nz + a.cold_planes);
     const bool les = h->p.les, trt = h->p.trt;
     if (les && trt) hipLaunchKernelGGL((k_adjoint<S, M, T, true, true>), grid, dim3(bx), 0, h->stream, a);
     else if (les) hipLaunchKernelGGL((k_adjoint<S, M, T, true, false>), grid, dim3(bx), 0, h->stream, a);
@@ -415,20 +434,13 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     unsigned int ncold = 0;
     WT_HIP(hipMemcpyAsync(&ncold, ccount.p, sizeof ncold, hipMemcpyDeviceToHost, h->stream));
     WT_HIP(hipStreamSynchronize(h->stream));
-    // dispatch order of the rows: those holding cold cells first
+    // cold cells in cell order (the list kernel appends them in any order)
     std::vector<uint32_t> cl(ncold);
-    if (ncold) WT_HIP(hipMemcpy(cl.data(), cold.p, sizeof(uint32_t) * ncold, hipMemcpyDeviceToHost));
-    const int64_t nrows = (int64_t)h->ny * h->nz;
-    std::vector<uint8_t> is_cold((size_t)nrows, 0);
-    for (uint32_t c : cl) is_cold[c / (uint32_t)h->nx] = 1;
-    std::vector<uint32_t> order;
-    order.reserve((size_t)nrows);
-    for (int pass = 1; pass >= 0; pass--)
-        for (int64_t r = 0; r < nrows; r++)
-            if (is_cold[(size_t)r] == pass) order.push_back((uint32_t)r);
-    Buf rows;
-    WT_HIP(hipMalloc(&rows.p, sizeof(uint32_t) * (size_t)nrows));
-    WT_HIP(hipMemcpy(rows.p, order.data(), sizeof(uint32_t) * (size_t)nrows, hipMemcpyHostToDevice));
+    if (ncold) {
+        WT_HIP(hipMemcpy(cl.data(), cold.p, sizeof(uint32_t) * ncold, hipMemcpyDeviceToHost));
+        std::sort(cl.begin(), cl.end());
+        WT_HIP(hipMemcpy(cold.p, cl.data(), sizeof(uint32_t) * ncold, hipMemcpyHostToDevice));
+    }
     int W = 0;
     if (int rc = replay_window(K, st, &W)) return rc;
     std::vector<Buf> rep((size_t)W + 1);
@@ -445,7 +457,13 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     a.ns = h->ns; a.lay = wt::Lay{h->ns, h->bsh}; a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.nwx = h->nwx;
     a.tau = h->p.tau; a.K = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
     a.hot = (const uint32_t *)hot.p;
-    a.rows = (const uint32_t *)rows.p;
+    a.cold = (const uint32_t *)cold.p;
+    a.ncold = ncold;
+    {
+        const int bx = adj_block(h->nx);
+        const int64_t row_blocks = (int64_t)((h->nx + bx - 1) / bx) * h->ny;
+        a.cold_planes = (int)(((ncold + bx - 1) / bx + row_blocks - 1) / row_blocks);
+    }
     a.lsh = h->bsh;
     a.lstride = h->bsh ? Q << h->bsh : 1;
     a.lmask = h->bsh ? (1 << h->bsh) - 1 : 0;

@@ -105,6 +105,17 @@ extern "C" {
 const char *wt_last_error(void) { return g_err; }
 int wt_abi_version(void) { return WT_ABI_VERSION; }
 
+int wt_device_memory(uint64_t *free_bytes, uint64_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return wt_fail(WT_E_ARG, "null output pointer");
+    if (int rc = check_device()) return rc;
+    size_t f = 0, t = 0;
+    WT_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return 0;
+}
+
 int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lattice **out)
 {
     if (!out) return wt_fail(WT_E_ARG, "null output pointer");

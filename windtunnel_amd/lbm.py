@@ -87,6 +87,7 @@ SIGNATURES = {
     "wt_get_pops": (_I, [_P, _I, _P]),
     "wt_get_solid": (_I, [_P, _P, _P]),
     "wt_time_steps": (_I, [_P, _I, _P]),
+    "wt_device_memory": (_I, [_P, _P]),
     "wt_set_timing": (_I, [_P, _I]),
     "wt_get_timing": (_I, [_P, _P, _P]),
     "wt_set_gray": (_I, [_P, _P]),
@@ -140,6 +141,13 @@ EY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1], dt
 EZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1], dtype=np.int32)
 W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
 FLUID, SOLID_OBJ, SOLID_WALL = 0, 1, 2
+
+
+def device_memory() -> tuple:
+    """(free, total) bytes of the current device's memory (wt_device_memory)."""
+    f, t = C.c_uint64(), C.c_uint64()
+    _check(lib().wt_device_memory(C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)
 
 
 def _toint32(v: float) -> int:
