@@ -335,6 +335,36 @@ def run_cpu(wl, threads):
     return n / best / 1e6
 
 
+# ---------------------------------------------------------------------- c1
+def run_c1(steps=2000):
+    """The CPU-runnable workload on the GPU: 96x48x48 BGK, sphere r = 8 at
+    (32, 24, 24) tested at cell centres, tau = 0.56, u0 = 0.05, 2000 steps,
+    with the reference's boundaries and with the configuration's own
+    (free-slip side walls, equilibrium outflow); mean C_d over the last
+    1000 steps."""
+    from windtunnel_amd.lbm import LBM
+    dims = (96, 48, 48)
+    lo, hi = [23, 15, 15], [41, 33, 33]
+    z, y, x = np.meshgrid(np.arange(lo[2], hi[2] + 1) + 0.5, np.arange(lo[1], hi[1] + 1) + 0.5,
+                          np.arange(lo[0], hi[0] + 1) + 0.5, indexing="ij")
+    mask = ((x - 32) ** 2 + (y - 24) ** 2 + (z - 24) ** 2 <= 64).astype(np.uint8).reshape(-1)
+    out = {"grid": "96x48x48", "steps": steps, "tau": 0.56, "u0": 0.05, "collision": "BGK"}
+    for mode in ("reference", "channel"):
+        L = LBM(*dims)
+        L.set_params(tau=0.56, les=False, trt=False, u0=0.05, csm=0.14, **{"lambda": 0.25})
+        L.set_boundary_mode(mode)
+        L.reset()
+        L.voxelize_mask(lo + hi, mask)
+        t0 = time.perf_counter()
+        F = L.step(steps)
+        wall = time.perf_counter() - t0
+        q = 0.5 * 0.05 ** 2 * L.frontalArea
+        out[mode] = dict(cd=float(np.mean(F[steps // 2:, 0]) / q), mlups=round(L.n * steps / wall / 1e6, 1),
+                         finite=bool(np.all(np.isfinite(F))))
+        L.close()
+    return out
+
+
 # ------------------------------------------------------------------- scale
 def run_scale(N=3000, K=500):
     """640x320x320 TRT+LES in binary16 storage around the seeded leaky
@@ -422,6 +452,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-scale", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--fd-cells", type=int, default=64, help="central-difference cells of the adjoint leg (0: none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy three times (HBM counter calibration)")
@@ -477,6 +508,10 @@ def main():
     if not a.no_adjoint:
         log("adjoint 192x96x96, 3000 steps")
         out["adjoint"] = run_adjoint(a.fd_cells)
+    if not a.no_c1:
+        log("c1 96x48x48 BGK sphere, reference and channel boundaries, 2000 steps")
+        out["c1"] = run_c1()
+        log(f"c1: {json.dumps(out['c1'])}")
     if not a.no_scale:
         log("scale 640x320x320 f16, level-7 leaky icosphere, 3000 steps + adjoint")
         out["scale"] = run_scale()

@@ -201,6 +201,22 @@ int wt_get_solid(wt_lattice *h, uint8_t *solid, uint8_t *near);
  * floats each; any pointer may be NULL. */
 int wt_get_walls(const wt_lattice *h, float *uwx, float *uwy, float *uwz);
 
+/* Boundary variants (set per lattice; the default is WT_BC_REFERENCE).
+ * WT_BC_REFERENCE: the reference's _boundaries (js/lbm.js:473-512):
+ *   equilibrium inlet, ceiling, floor and spanwise end planes, then the
+ *   zero-gradient outlet copy.
+ * WT_BC_CHANNEL (no reference counterpart: the "equilibrium outflow,
+ *   free-slip side walls" of the 96x48x48 configuration): applied after
+ *   the reference's boundaries. The outlet x = nx-1 becomes the equilibrium
+ *   at rho = 1 and the velocity of x = nx-2. The y and z boundary planes
+ *   become free-slip walls half a cell outside the first fluid rows:
+ *   each population leaving a fluid cell towards a wall returns mirrored
+ *   (specular reflection). The inlet stays the equilibrium at u0. AB layout; wt_step,
+ *   wt_step_slabs and wt_time_steps only (the slab calls and the adjoint
+ *   refuse it with WT_E_STATE / WT_E_ARG). */
+enum { WT_BC_REFERENCE = 0, WT_BC_CHANNEL = 1 };
+int wt_set_boundary_mode(wt_lattice *h, int mode);
+
 /* Device time of nsteps steps, measured with HIP events on the lattice's
  * stream (no host sync inside the timed region). */
 int wt_time_steps(wt_lattice *h, int nsteps, double *ms);

@@ -52,6 +52,7 @@ def lib():
             "wto_voxel_build": (I, [P, C.c_long, I, P, P]),
             "wto_grid_mask": (V, [P, P, P, D, P, P, D, P, P]),
             "wto_set_gray": (I, [P, P]),
+            "wto_set_channel": (V, [P, I]),
             "wto_adjoint": (I, [P, I, P, P, P]),
             "wto_collide_adj": (V, [P, P, D, D, D, I, I, P]),
         }
@@ -101,6 +102,13 @@ class OracleLBM:
 
     def params(self) -> dict:
         return dict(self._params)
+
+    def set_boundary_mode(self, mode: str):
+        """'reference' (js/lbm.js:473-512) or 'channel' (free-slip side walls,
+        equilibrium outflow; applied after the reference boundaries)."""
+        if mode not in ("reference", "channel"):
+            raise ValueError(mode)
+        lib().wto_set_channel(self._h, int(mode == "channel"))
 
     # -- state ----------------------------------------------------------
     def array(self, name: str) -> np.ndarray:

@@ -46,6 +46,7 @@ typedef struct {
     long off[19];
     double u0, tau, csm, lambda;
     int les, trt, movingWalls, ground, parity;
+    int channel;          /* 1: channel boundaries (free-slip sides, equilibrium outflow) */
     long steps, frontalArea, planformArea, solidCells, rideHeight;
     double force[3];
 } wto_lbm;
@@ -65,6 +66,15 @@ static int js_toint32(double d)
     if (m < 0) m += 4294967296.0;
     uint32_t u = (uint32_t)m;
     return (int32_t)u;
+}
+
+/* Direction with the same e_x and e_y / e_z flipped where cy / cz is set. */
+static int reflect_q(int q, int cy, int cz)
+{
+    const int ey = cy ? -EY[q] : EY[q], ez = cz ? -EZ[q] : EZ[q];
+    for (int r = 0; r < 19; r++)
+        if (EX[r] == EX[q] && EY[r] == ey && EZ[r] == ez) return r;
+    return q;
 }
 
 static void init_tables(void)
@@ -149,6 +159,8 @@ void wto_set_params(wto_lbm *L, double tau, int les, double csm, int trt, double
     L->movingWalls = movingWalls;
 }
 
+void wto_set_channel(wto_lbm *L, int on) { L->channel = !!on; }
+
 void wto_get_params(const wto_lbm *L, double out[4], int iout[3])
 {
     out[0] = L->tau; out[1] = L->csm; out[2] = L->lambda; out[3] = L->u0;
@@ -172,6 +184,7 @@ void wto_step(wto_lbm *L, int nsteps, double *forces)
         double sf[3];
         DISPATCH(L, step_slab, L, 1, L->nz - 1, sf);
         DISPATCH(L, boundaries, L, 1, 0, 0, 0, 0);
+        if (L->channel) DISPATCH(L, channel, L);
         L->parity ^= 1;
         L->force[0] = sf[0]; L->force[1] = sf[1]; L->force[2] = sf[2];
         L->steps++;
@@ -216,6 +229,7 @@ static void *pool_thread(void *p)
 int wto_step_pool(wto_lbm *L, int nsteps, int nworkers, int threaded, double *forces)
 {
     if (nworkers < 1 || nworkers > L->nz - 2) return -1;
+    if (L->channel) return -2; /* the pool form keeps the reference boundaries */
     pool_arg *A = calloc((size_t)nworkers, sizeof(pool_arg));
     double *per = calloc((size_t)nsteps * nworkers * 3, sizeof(double));
     const int lo = 1, hi = L->nz - 1, span = hi - lo;

@@ -543,6 +543,16 @@ static napi_value js_set_parity(napi_env env, napi_callback_info info)
     return NULL;
 }
 
+static napi_value js_set_boundary_mode(napi_env env, napi_callback_info info)
+{
+    napi_value a[2];
+    args(env, info, 2, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    WT(env, wt_set_boundary_mode(h, (int)num(env, a[1], 0)));
+    return NULL;
+}
+
 static napi_value js_get_state(napi_env env, napi_callback_info info)
 {
     napi_value a[1], o, f;
@@ -739,7 +749,7 @@ static napi_value init(napi_env env, napi_value exports)
         {"getParams", js_get_params}, {"reset", js_reset}, {"voxelize", js_voxelize},
         {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs},
         {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
-        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory},
+        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory}, {"setBoundaryMode", js_set_boundary_mode},
         {"voxelizeParts", js_voxelize_parts}, {"getWalls", js_get_walls}, {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},
         {"adjointRelease", js_adjoint_release},
         {"voxelBounds", js_voxel_bounds}, {"voxelPlan", js_voxel_plan}, {"voxelBuild", js_voxel_build}};

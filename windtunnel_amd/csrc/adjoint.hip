@@ -531,6 +531,7 @@ int check_adjoint_lattice(const wt_lattice *h)
     if (!h) return wt_fail(WT_E_ARG, "null lattice handle");
     if (h->layout != WT_LAYOUT_AB) return wt_fail(WT_E_ARG, "the adjoint needs the AB layout");
     if (h->p.movingWalls) return wt_fail(WT_E_ARG, "the adjoint does not support moving walls");
+    if (h->bc != WT_BC_REFERENCE) return wt_fail(WT_E_ARG, "the adjoint takes the reference boundaries only");
     return 0;
 }
 

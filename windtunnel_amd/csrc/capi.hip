@@ -72,7 +72,7 @@ void release(wt_lattice *h)
     adj_release(h);
     for (void *p : {(void *)h->d_buf[0], (void *)h->d_solid, (void *)h->d_sbits,
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
-                    (void *)h->d_forces, (void *)h->d_near})
+                    (void *)h->d_forces, (void *)h->d_near, (void *)h->d_ring})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -388,6 +388,8 @@ static int run_steps(wt_lattice *h, int nsteps, int nslabs, double *forces)
                 if (nslabs > 1) slab_range(h->nz, w, nslabs, &bz0, &bz1);
                 if (int rc = lbm_launch_step(h, bz0, bz1, macro, h->d_forces + (int64_t)s * per + 3 * w)) return rc;
             }
+            if (h->bc == WT_BC_CHANNEL)
+                if (int rc = lbm_channel(h, macro)) return rc;
             if (h->timing) WT_HIP(hipEventRecord(h->tev[2 * s + 1], h->stream));
             h->parity ^= 1;
             h->steps++;
@@ -416,9 +418,31 @@ static int run_steps(wt_lattice *h, int nsteps, int nslabs, double *forces)
 
 int wt_step(wt_lattice *h, int nsteps, double *forces) { return run_steps(h, nsteps, 1, forces); }
 
+int wt_set_boundary_mode(wt_lattice *h, int mode)
+{
+    WT_CHECK_H(h);
+    if (mode != WT_BC_REFERENCE && mode != WT_BC_CHANNEL) return wt_fail(WT_E_ARG, "unknown boundary mode %d", mode);
+    if (mode == WT_BC_CHANNEL && h->layout != WT_LAYOUT_AB)
+        return wt_fail(WT_E_ARG, "channel boundaries need the AB layout");
+    if (mode == WT_BC_CHANNEL && !h->d_ring) {
+        // interior (y, z) rows next to a side wall, each once
+        std::vector<int> ring;
+        for (int z = 1; z < h->nz - 1; z++)
+            for (int y = 1; y < h->ny - 1; y++)
+                if (y == 1 || y == h->ny - 2 || z == 1 || z == h->nz - 2) { ring.push_back(y); ring.push_back(z); }
+        if (int rc = alloc((void **)&h->d_ring, sizeof(int) * ring.size())) return rc;
+        WT_HIP(hipMemcpy(h->d_ring, ring.data(), sizeof(int) * ring.size(), hipMemcpyHostToDevice));
+        h->nring = (int64_t)ring.size() / 2;
+    }
+    h->bc = mode;
+    h->epoch++;
+    return 0;
+}
+
 int wt_step_slab(wt_lattice *h, int z0, int z1, double force[3])
 {
     WT_CHECK_H(h);
+    if (h->bc != WT_BC_REFERENCE) return wt_fail(WT_E_STATE, "the slab calls take the reference boundaries only");
     if (z0 < 1 || z1 > h->nz - 1 || z0 > z1)
         return wt_fail(WT_E_ARG, "slab [%d,%d) is not within the interior rows [1,%d)", z0, z1, h->nz - 1);
     double f[3] = {0, 0, 0};
@@ -435,6 +459,7 @@ int wt_step_slab(wt_lattice *h, int z0, int z1, double force[3])
 int wt_boundaries(wt_lattice *h, int z0, int z1, int doLo, int doHi)
 {
     WT_CHECK_H(h);
+    if (h->bc != WT_BC_REFERENCE) return wt_fail(WT_E_STATE, "the slab calls take the reference boundaries only");
     if (z0 < 0 || z1 > h->nz || z0 > z1) return wt_fail(WT_E_ARG, "rows [%d,%d) out of range", z0, z1);
     // the planes of interior rows were written by wt_step_slab with those
     // rows; the end planes belong to whoever is told to take them
@@ -556,6 +581,8 @@ int wt_time_steps(wt_lattice *h, int nsteps, double *ms)
     WT_HIP(hipEventRecord(h->ev0, h->stream));
     for (int s = 0; s < nsteps; s++) {
         if (int rc = lbm_launch_step(h, 0, h->nz, s == nsteps - 1, h->d_forces + 3 * (s % h->forces_cap))) return rc;
+        if (h->bc == WT_BC_CHANNEL)
+            if (int rc = lbm_channel(h, s == nsteps - 1)) return rc;
         h->parity ^= 1;
         h->steps++;
     }

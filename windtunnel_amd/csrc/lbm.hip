@@ -610,6 +610,103 @@ __global__ void __launch_bounds__(64) k_force_sum(const double *partial, int64_t
     reduce_partials(partial, count, out);
 }
 
+// ------------------------------------------------ channel boundaries
+// Configuration variant with free-slip side walls and an equilibrium
+// outflow (no reference function), applied after the step as a fix-up of
+// what the step wrote (AB layout). Thread roles:
+//  - one per interior (y, z) of the outlet plane x = nx-1: f^eq(1, u) with u
+//    the velocity of the post-collision populations of x = nx-2 (0 at a solid
+//    cell), its fields on the last step of a batch, then its reflections;
+//  - one per interior fluid cell x in [1, nx-2] of a row next to a side wall
+//    (`ring`): its reflections.
+// Reflection: each population q that leaves the cell towards a side wall is
+// copied into the ghost cell it faces across that wall, in the slot of q with
+// e_y and/or e_z flipped, where the neighbour pulls it next step (halfway
+// specular reflection). Every ghost slot has exactly one writer.
+constexpr int reflect_dir(int q, bool fy, bool fz)
+{
+    for (int r = 0; r < Q; r++)
+        if (EX[r] == EX[q] && EY[r] == (fy ? -EY[q] : EY[q]) && EZ[r] == (fz ? -EZ[q] : EZ[q])) return r;
+    return q;
+}
+
+template <class S, class MAC> struct ChanArgs {
+    S *G;
+    const uint8_t *solid;
+    MAC *rho, *ux, *uy, *uz;
+    const int2 *ring;  // (y, z) of the interior rows next to a side wall
+    int64_t nring;
+    Lay lay;
+    int nx, ny, nz, macro;
+};
+
+template <class S> __device__ __forceinline__ S to_store(double v);
+template <> __device__ __forceinline__ float to_store<float>(double v) { return (float)v; }
+template <> __device__ __forceinline__ double to_store<double>(double v) { return v; }
+template <> __device__ __forceinline__ __half to_store<__half>(double v) { return __float2half_rn((float)v * F16_SCALE); }
+
+template <class S, class MAC>
+__device__ __forceinline__ void reflect_out(const ChanArgs<S, MAC> &a, int x, int y, int z, const S (&v)[Q])
+{
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        const int ty = y + EY[q], tz = z + EZ[q];
+        const bool cy = ty == 0 || ty == a.ny - 1, cz = tz == 0 || tz == a.nz - 1;
+        if (!(cy || cz)) continue;
+        const int64_t ghost = x + (int64_t)a.nx * ((cy ? ty : y) + (int64_t)a.ny * (cz ? tz : z));
+        const int64_t k = cy ? (cz ? pix(reflect_dir(q, true, true), ghost, a.lay) : pix(reflect_dir(q, true, false), ghost, a.lay))
+                             : pix(reflect_dir(q, false, true), ghost, a.lay);
+        a.G[k] = v[q];
+    }
+}
+
+template <class S, class T, class MAC>
+__global__ void __launch_bounds__(256) k_channel(ChanArgs<S, MAC> a)
+{
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nout = (int64_t)(a.ny - 2) * (a.nz - 2);
+    if (k < nout) {
+        const int y = 1 + (int)(k % (a.ny - 2)), z = 1 + (int)(k / (a.ny - 2));
+        const int64_t i = (a.nx - 1) + (int64_t)a.nx * (y + (int64_t)a.ny * z), j = i - 1;
+        T vx = 0, vy = 0, vz = 0;
+        if (!a.solid[j]) {
+            T f[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) f[q] = Pop<S>::template ld<T>(a.G, pix(q, j, a.lay));
+            const T dr = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] +
+                         f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+            const T inv = T(1) / (T(1) + dr);
+            vx = (f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14]) * inv;
+            vy = (f[3] - f[4] + f[7] - f[8] - f[9] + f[10] + f[15] - f[16] + f[17] - f[18]) * inv;
+            vz = (f[5] - f[6] + f[11] - f[12] - f[13] + f[14] + f[15] - f[16] - f[17] + f[18]) * inv;
+        }
+        // _writeEq at rho = 1 in the reference's expression order (js/lbm.js:155-162)
+        const T r = T(1);
+        const T usq = T(1.5) * (vx * vx + vy * vy + vz * vz);
+        S v[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const T cu = T(3) * ((T)EX[q] * vx + (T)EY[q] * vy + (T)EZ[q] * vz);
+            const T w = q == 0 ? T(1.0 / 3) : (q <= 6 ? T(1.0 / 18) : T(1.0 / 36));
+            v[q] = to_store<S>((double)(w * ((r - T(1)) + r * (cu + T(0.5) * cu * cu - usq))));
+            a.G[pix(q, i, a.lay)] = v[q];
+        }
+        if (a.macro) { a.rho[i] = (MAC)1; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz; }
+        if (y == 1 || y == a.ny - 2 || z == 1 || z == a.nz - 2) reflect_out(a, a.nx - 1, y, z, v);
+        return;
+    }
+    k -= nout;
+    if (k >= (int64_t)(a.nx - 2) * a.nring) return;
+    const int x = 1 + (int)(k % (a.nx - 2));
+    const int2 yz = a.ring[k / (a.nx - 2)];
+    const int64_t i = x + (int64_t)a.nx * (yz.x + (int64_t)a.ny * yz.y);
+    if (a.solid[i]) return;
+    S v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) v[q] = a.G[pix(q, i, a.lay)];
+    reflect_out(a, x, yz.x, yz.y, v);
+}
+
 // ------------------------------------------------------- reset / voxelize
 
 template <class S, class MAC> struct InitArgs {
@@ -1000,6 +1097,31 @@ int lbm_launch_step(wt_lattice *h, int bz0, int bz1, bool macro, double *d_force
     if (h->storage == WT_STORE_F16)
         return step_layout<__half, float, float>(h, bz0, bz1, macro, d_force);
     return step_layout<double, double, double>(h, bz0, bz1, macro, d_force);
+}
+
+template <class S, class T, class MAC> int channel_t(wt_lattice *h, bool macro)
+{
+    ChanArgs<S, MAC> a{};
+    a.G = (S *)h->d_buf[h->parity ^ 1];
+    a.solid = h->d_solid;
+    MAC *m = (MAC *)h->d_mac;
+    a.rho = m; a.ux = m + h->n; a.uy = m + 2 * h->n; a.uz = m + 3 * h->n;
+    a.ring = (const int2 *)h->d_ring;
+    a.nring = h->nring;
+    a.lay = Lay{h->ns, h->bsh};
+    a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.macro = macro;
+    const int64_t total = (int64_t)(h->ny - 2) * (h->nz - 2) + (int64_t)(h->nx - 2) * h->nring;
+    hipLaunchKernelGGL((k_channel<S, T, MAC>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream, a);
+    WT_HIP(hipGetLastError());
+    return 0;
+}
+
+int lbm_channel(wt_lattice *h, bool macro)
+{
+    if (h->storage == WT_STORE_F32 && h->arith == WT_ARITH_F32) return channel_t<float, float, float>(h, macro);
+    if (h->storage == WT_STORE_F32) return channel_t<float, double, float>(h, macro);
+    if (h->storage == WT_STORE_F16) return channel_t<__half, float, float>(h, macro);
+    return channel_t<double, double, double>(h, macro);
 }
 
 int lbm_flush_force(wt_lattice *h)

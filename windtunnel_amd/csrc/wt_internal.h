@@ -123,6 +123,10 @@ struct wt_lattice {
     // neighbour) in cell order; near_z[z] is the index of the first one in
     // plane z (nz + 1 entries), so a z range is one slice of the list
     uint32_t *d_near = nullptr;
+    // boundary variant (WT_BC_*); channel: the (y, z) rows next to a side wall
+    int bc = 0;
+    int *d_ring = nullptr;
+    int64_t nring = 0;
     int64_t near_cap = 0;
     std::vector<int64_t> near_z;
 
@@ -143,6 +147,7 @@ int wt_fail(int code, const char *fmt, ...);
 int lbm_reset(wt_lattice *h);
 int lbm_apply_voxelize(wt_lattice *h, const uint8_t *ops);
 int lbm_rebuild_masks(wt_lattice *h);
+int lbm_channel(wt_lattice *h, bool macro);
 // Launches one step over z rows [bz0,bz1). Its force (when force_out is
 // given) is reduced into force_out by the NEXT step launch or by
 // lbm_flush_force, which must run before force_out is read.

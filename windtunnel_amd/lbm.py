@@ -88,6 +88,7 @@ SIGNATURES = {
     "wt_get_solid": (_I, [_P, _P, _P]),
     "wt_time_steps": (_I, [_P, _I, _P]),
     "wt_device_memory": (_I, [_P, _P]),
+    "wt_set_boundary_mode": (_I, [_P, _I]),
     "wt_set_timing": (_I, [_P, _I]),
     "wt_get_timing": (_I, [_P, _P, _P]),
     "wt_set_gray": (_I, [_P, _P]),
@@ -416,6 +417,15 @@ class LBM:
         lo = True if whole else bool(doLo)
         hi = True if z1 is None else bool(doHi)
         _check(lib().wt_boundaries(self._h, a, b, int(lo), int(hi)))
+        self._fields = None
+
+    def set_boundary_mode(self, mode: str) -> None:
+        """'reference' (js/lbm.js:473-512) or 'channel': free-slip side walls
+        and an equilibrium outflow (wt_set_boundary_mode)."""
+        modes = {"reference": 0, "channel": 1}
+        if mode not in modes:
+            raise ValueError(f"unknown boundary mode {mode!r}")
+        _check(lib().wt_set_boundary_mode(self._h, modes[mode]))
         self._fields = None
 
     def time_steps(self, n: int) -> float:
