@@ -201,6 +201,21 @@ int wt_get_solid(wt_lattice *h, uint8_t *solid, uint8_t *near);
  * floats each; any pointer may be NULL. */
 int wt_get_walls(const wt_lattice *h, float *uwx, float *uwy, float *uwz);
 
+/* The app's slice view (js/main.js:925-965) computed on the device from the
+ * fields of the last step, for plane z = k (axis 0, the 'xy' view) or
+ * y = k (axis 1, the 'xz' view): nx * h floats, x fastest, h = ny or nz.
+ * field 0: speed |u| / u0; 1: C_p = ((r - 1) / 3) / (u0^2 / 2) of the density
+ * averaged with its four in-plane neighbours (weights 2,1,1,1,1 over 6);
+ * 2: the in-plane vorticity component, central differences (the cell
+ * itself stands in for a missing neighbour at the plane's edge). Solid
+ * cells read 0; solid (may be NULL) receives the plane's solid map. */
+int wt_slice(wt_lattice *h, int axis, int k, int field, float *out, uint8_t *solid);
+
+/* sample() (js/lbm.js:515-539) at count points: trilinear velocity from
+ * the fields of the last step at xyz[3p..3p+2], u0 outside the cell-centre
+ * range, into uvw[3p..3p+2]. */
+int wt_sample(wt_lattice *h, const double *xyz, int64_t count, double *uvw);
+
 /* Boundary variants (set per lattice; the default is WT_BC_REFERENCE).
  * WT_BC_REFERENCE: the reference's _boundaries (js/lbm.js:473-512):
  *   equilibrium inlet, ceiling, floor and spanwise end planes, then the

@@ -236,3 +236,38 @@ def grid_mask(grid, gdim, gmin, ginv, M, pos, scale, box) -> np.ndarray:
     lib().wto_grid_mask(_ptr(grid), _ptr(gdim), _ptr(gmin), float(ginv), _ptr(M), _ptr(pos),
                         float(scale), _ptr(box), _ptr(mask))
     return mask
+
+
+def slice_view(fields: dict, solid: np.ndarray, dims, axis: str, k: int, field: str, u0: float):
+    """The app's slice values before colour mapping (js/main.js:925-965) over
+    whole fields (arrays of n cells, x fastest): axis 'xy' (plane z = k) or
+    'xz' (plane y = k); field 'speed' |u|/u0, 'pressure' C_p of the density
+    averaged with its in-plane neighbours, 'vorticity' the in-plane curl
+    component by central differences; a missing neighbour at the plane's
+    edge is the cell itself. Binary64 from the stored values, returned as
+    float32 [h, nx]; solid cells read 0 (the app paints them instead). The
+    speed uses sqrt(x^2 + y^2 + z^2) where the app calls Math.hypot (equal
+    to within an ulp of binary64)."""
+    nx, ny, nz = dims
+    sh = (nz, ny, nx)
+    pl = (lambda a: np.asarray(a).reshape(sh)[k, :, :]) if axis == "xy" else (lambda a: np.asarray(a).reshape(sh)[:, k, :])
+    ux, uy, uz, rho = (pl(fields[n]).astype(np.float64) for n in ("ux", "uy", "uz", "rho"))
+    sol = pl(solid) != 0
+    h = ux.shape[0]
+    a_p = np.minimum(np.arange(nx) + 1, nx - 1)
+    a_m = np.maximum(np.arange(nx) - 1, 0)
+    b_p = np.minimum(np.arange(h) + 1, h - 1)
+    b_m = np.maximum(np.arange(h) - 1, 0)
+    if field == "speed":
+        v = np.sqrt(ux * ux + uy * uy + uz * uz) / u0
+    elif field == "pressure":
+        r = (rho * 2 + rho[:, a_p] + rho[:, a_m] + rho[b_p, :] + rho[b_m, :]) / 6
+        v = ((r - 1) / 3) / (0.5 * u0 * u0)
+    elif axis == "xy":
+        v = (uy[:, a_p] - uy[:, a_m]) * 0.5 - (ux[b_p, :] - ux[b_m, :]) * 0.5
+    else:
+        v = (ux[b_p, :] - ux[b_m, :]) * 0.5 - (uz[:, a_p] - uz[:, a_m]) * 0.5
+    out = v.astype(np.float32)
+    out[sol] = 0
+    return out
+

@@ -146,6 +146,56 @@ check('voxelizeBody rasterises the catalogue bodies as the reference does (lbm_b
   }
 });
 
+check('slice and sampleMany equal the app\'s own formulas over the downloaded fields', () => {
+  // updateSlice (js/main.js:925-965) before its colour maps, and sample() per point
+  const fx = load('lbm_bgk_sphere');
+  const L = new LBM(...fx.dims);
+  applyParams(L, fx.params);
+  L.reset();
+  replay(L, fx.voxelize);
+  L.stepMany(120);
+  const { nx, ny, nz, ux, uy, uz, rho, solid } = L;
+  const u0 = L.u0;
+  for (const axis of ['xy', 'xz']) {
+    const xy = axis === 'xy';
+    const h = xy ? ny : nz, k = xy ? nz >> 1 : ny >> 1;
+    const at = xy ? (x, y) => x + nx * (y + ny * k) : (x, z) => x + nx * (k + ny * z);
+    for (const field of ['speed', 'pressure', 'vorticity']) {
+      const got = L.slice(axis, k, field);
+      const want = new Float32Array(nx * h);
+      for (let b = 0; b < h; b++) {
+        for (let a = 0; a < nx; a++) {
+          const i = at(a, b), o = a + b * nx;
+          assert(got.solid[o] === solid[i], 'slice solid map');
+          if (solid[i]) continue;
+          const ip = a < nx - 1 ? at(a + 1, b) : i, im = a > 0 ? at(a - 1, b) : i;
+          const jp = b < h - 1 ? at(a, b + 1) : i, jm = b > 0 ? at(a, b - 1) : i;
+          if (field === 'speed') want[o] = Math.hypot(ux[i], uy[i], uz[i]) / u0;
+          else if (field === 'pressure') {
+            const r = (rho[i] * 2 + rho[ip] + rho[im] + rho[jp] + rho[jm]) / 6;
+            want[o] = ((r - 1) / 3) / (0.5 * u0 * u0);
+          } else if (xy) want[o] = (uy[ip] - uy[im]) * 0.5 - (ux[jp] - ux[jm]) * 0.5;
+          else want[o] = (ux[jp] - ux[jm]) * 0.5 - (uz[ip] - uz[im]) * 0.5;
+        }
+      }
+      // Math.hypot and sqrt of the sum of squares may differ by an ulp of binary64
+      const tol = field === 'speed' ? 1.2e-7 : 0;
+      for (let o = 0; o < want.length; o++)
+        if (Math.abs(got.values[o] - want[o]) > tol * Math.abs(want[o]))
+          throw new Error(`${axis} ${field} at ${o}: ${got.values[o]} vs ${want[o]}`);
+    }
+  }
+  const pts = new Float64Array(3 * 500);
+  let seed = 7;
+  const rnd = () => ((seed = (seed * 1103515245 + 12345) >>> 0) / 4294967296);
+  for (let p = 0; p < 500; p++) { pts[3 * p] = rnd() * (nx + 2) - 1; pts[3 * p + 1] = rnd() * ny; pts[3 * p + 2] = rnd() * nz; }
+  const many = L.sampleMany(pts), one = [0, 0, 0];
+  for (let p = 0; p < 500; p++) {
+    L.sample(pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], one);
+    for (let c = 0; c < 3; c++) assert(many[3 * p + c] === one[c], 'sampleMany point ' + p);
+  }
+});
+
 check('Voxel.build is bit-exact (vox_box, vox_hitcap, vox_sphere)', () => {
   for (const name of ['vox_box', 'vox_hitcap', 'vox_sphere']) {
     const fx = load(name);

@@ -597,6 +597,58 @@ static napi_value js_get_fields(napi_env env, napi_callback_info info)
     return NULL;
 }
 
+/* slice(h, axis, k, field, out Float32Array, solid Uint8Array | null) */
+static napi_value js_slice(napi_env env, napi_callback_info info)
+{
+    napi_value a[6];
+    args(env, info, 6, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    wt_state_t st;
+    if (wt_get_state(h, &st) != 0) {
+        napi_throw_error(env, NULL, wt_last_error());
+        return NULL;
+    }
+    const int axis = (int)num(env, a[1], 0);
+    const size_t cells = (size_t)st.nx * (size_t)(axis == 0 ? st.ny : st.nz);
+    int ok;
+    size_t ob = 0, sb = 0;
+    void *out = tdata(env, a[4], &ob, &ok);
+    if (!ok) return NULL;
+    void *sol = tdata(env, a[5], &sb, &ok);
+    if (!ok) return NULL;
+    if (!out) {
+        napi_throw_type_error(env, NULL, "slice needs an output Float32Array");
+        return NULL;
+    }
+    if (!enough(env, ob, cells * 4, "slice output") || (sol && !enough(env, sb, cells, "slice solid"))) return NULL;
+    WT(env, wt_slice(h, axis, (int)num(env, a[2], 0), (int)num(env, a[3], 0), out, sol));
+    return NULL;
+}
+
+/* sampleMany(h, xyz Float64Array, out Float64Array) */
+static napi_value js_sample_many(napi_env env, napi_callback_info info)
+{
+    napi_value a[3];
+    args(env, info, 3, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    size_t xb = 0, ob = 0;
+    void *xyz = tdata(env, a[1], &xb, &ok);
+    if (!ok) return NULL;
+    void *out = tdata(env, a[2], &ob, &ok);
+    if (!ok) return NULL;
+    if (!xyz || !out) {
+        napi_throw_type_error(env, NULL, "sampleMany needs Float64Array points and output");
+        return NULL;
+    }
+    const size_t count = xb / 24;
+    if (!enough(env, ob, count * 24, "sampleMany output")) return NULL;
+    WT(env, wt_sample(h, xyz, (int64_t)count, out));
+    return NULL;
+}
+
 static napi_value js_get_pops(napi_env env, napi_callback_info info)
 {
     napi_value a[3];
@@ -749,7 +801,7 @@ static napi_value init(napi_env env, napi_value exports)
         {"getParams", js_get_params}, {"reset", js_reset}, {"voxelize", js_voxelize},
         {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs},
         {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
-        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory}, {"setBoundaryMode", js_set_boundary_mode},
+        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory}, {"setBoundaryMode", js_set_boundary_mode}, {"slice", js_slice}, {"sampleMany", js_sample_many},
         {"voxelizeParts", js_voxelize_parts}, {"getWalls", js_get_walls}, {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},
         {"adjointRelease", js_adjoint_release},
         {"voxelBounds", js_voxel_bounds}, {"voxelPlan", js_voxel_plan}, {"voxelBuild", js_voxel_build}};

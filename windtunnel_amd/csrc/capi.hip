@@ -72,7 +72,8 @@ void release(wt_lattice *h)
     adj_release(h);
     for (void *p : {(void *)h->d_buf[0], (void *)h->d_solid, (void *)h->d_sbits,
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
-                    (void *)h->d_forces, (void *)h->d_near, (void *)h->d_ring})
+                    (void *)h->d_forces, (void *)h->d_near, (void *)h->d_ring,
+                    (void *)h->d_prod})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);

@@ -127,6 +127,9 @@ struct wt_lattice {
     int bc = 0;
     int *d_ring = nullptr;
     int64_t nring = 0;
+    // staging of the visual products (wt_slice, wt_sample), grown on demand
+    void *d_prod = nullptr;
+    size_t prod_cap = 0;
     int64_t near_cap = 0;
     std::vector<int64_t> near_z;
 

@@ -251,6 +251,30 @@ class LBM {
     return ms;
   }
 
+  /**
+   * The app's slice view on the device (js/main.js:925-965): axis 'xy'
+   * (plane z = k) or 'xz' (plane y = k), field 'speed' (|u|/u0), 'pressure'
+   * (C_p of the neighbour-averaged density) or 'vorticity'. Returns
+   * { values: Float32Array(nx*h), solid: Uint8Array(nx*h) }, x fastest;
+   * colour mapping stays with the caller. u0 is the one of the last step.
+   */
+  slice(axis, k, field) {
+    const ax = axis === 'xz' ? 1 : 0;
+    const f = { speed: 0, pressure: 1, vorticity: 2 }[field];
+    if (f === undefined) throw new Error('unknown slice field ' + field);
+    const cells = this.nx * (ax ? this.nz : this.ny);
+    const values = new Float32Array(cells), solid = new Uint8Array(cells);
+    native().slice(this._h, ax, k, f, values, solid);
+    return { values, solid };
+  }
+
+  /** sample() at many points on the device: xyz Float64Array(3m) -> Float64Array(3m). */
+  sampleMany(xyz, out) {
+    const o = out || new Float64Array(xyz.length);
+    native().sampleMany(this._h, xyz, o);
+    return o;
+  }
+
   /** Trilinear velocity sample at lattice position (js/lbm.js:515-539). */
   sample(x, y, z, out) {
     const nx = this.nx, ny = this.ny, nz = this.nz;

@@ -89,6 +89,8 @@ SIGNATURES = {
     "wt_time_steps": (_I, [_P, _I, _P]),
     "wt_device_memory": (_I, [_P, _P]),
     "wt_set_boundary_mode": (_I, [_P, _I]),
+    "wt_slice": (_I, [_P, _I, _I, _I, _P, _P]),
+    "wt_sample": (_I, [_P, _P, C.c_int64, _P]),
     "wt_set_timing": (_I, [_P, _I]),
     "wt_get_timing": (_I, [_P, _P, _P]),
     "wt_set_gray": (_I, [_P, _P]),
@@ -418,6 +420,30 @@ class LBM:
         hi = True if z1 is None else bool(doHi)
         _check(lib().wt_boundaries(self._h, a, b, int(lo), int(hi)))
         self._fields = None
+
+    SLICE_FIELDS = {"speed": 0, "pressure": 1, "vorticity": 2}
+
+    def slice(self, axis: str, k: int, field: str):
+        """The app's slice view on the device (wt_slice, js/main.js:925-965):
+        axis 'xy' (plane z = k) or 'xz' (plane y = k); field 'speed' (|u|/u0),
+        'pressure' (C_p) or 'vorticity'. Returns (values [h, nx] float32,
+        solid [h, nx] uint8)."""
+        ax = {"xy": 0, "xz": 1}[axis]
+        h = self.ny if ax == 0 else self.nz
+        out = np.empty((h, self.nx), dtype=np.float32)
+        sol = np.empty((h, self.nx), dtype=np.uint8)
+        _check(lib().wt_slice(self._h, ax, int(k), self.SLICE_FIELDS[field], _ptr(out), _ptr(sol)))
+        return out, sol
+
+    def sample_many(self, points) -> np.ndarray:
+        """sample() (js/lbm.js:515-539) at many points on the device
+        (wt_sample): points [..., 3] lattice coordinates -> [..., 3] velocity."""
+        p = np.ascontiguousarray(points, dtype=np.float64)
+        if p.shape[-1:] != (3,):
+            raise ValueError("points must have a last axis of 3")
+        out = np.empty_like(p)
+        _check(lib().wt_sample(self._h, _ptr(p), int(p.size // 3), _ptr(out)))
+        return out
 
     def set_boundary_mode(self, mode: str) -> None:
         """'reference' (js/lbm.js:473-512) or 'channel': free-slip side walls
