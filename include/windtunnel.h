@@ -216,6 +216,17 @@ int wt_slice(wt_lattice *h, int axis, int k, int field, float *out, uint8_t *sol
  * range, into uvw[3p..3p+2]. */
 int wt_sample(wt_lattice *h, const double *xyz, int64_t count, double *uvw);
 
+/* MeshIO.parse (js/meshio.js:322-340 with its readers, :19-214): a model
+ * file's bytes into a float32 triangle soup, 9 values per triangle in the
+ * file's coordinates, for Voxel.build. The extension of `name` (or, when it
+ * is none of stl/obj/glb/gltf, the content) picks binary or ASCII STL, OBJ
+ * (polygons fan-triangulated) or binary glTF 2.0 (node transforms applied).
+ * Host code; no device needed. With out == NULL, *count receives the number
+ * of floats; otherwise *count is the capacity of out on entry and the number
+ * written on return. Unreadable files fail with WT_E_ARG and the reference's
+ * message in wt_last_error(). */
+int wt_mesh_parse(const char *name, const void *buf, int64_t len, float *out, int64_t *count);
+
 /* Boundary variants (set per lattice; the default is WT_BC_REFERENCE).
  * WT_BC_REFERENCE: the reference's _boundaries (js/lbm.js:473-512):
  *   equilibrium inlet, ceiling, floor and spanwise end planes, then the

@@ -17,19 +17,19 @@ const zlib = require('zlib');
 const crypto = require('crypto');
 const synth = require('./synth.js');
 
-const REF = process.argv[2] && !process.argv[2].startsWith('lbm_') && !process.argv[2].startsWith('vox_')
-  ? process.argv[2] : '/root/reference';
-const ONLY = process.argv.slice(2).filter(a => a.startsWith('lbm_') || a.startsWith('vox_'));
+const CASE = a => a.startsWith('lbm_') || a.startsWith('vox_') || a.startsWith('mesh_');
+const REF = process.argv[2] && !CASE(process.argv[2]) ? process.argv[2] : '/root/reference';
+const ONLY = process.argv.slice(2).filter(CASE);
 const OUT = path.join(__dirname, 'fixtures');
 fs.mkdirSync(OUT, { recursive: true });
 
-const ctx = { console };
+const ctx = { console, TextDecoder };
 ctx.window = ctx; ctx.self = ctx;
 vm.createContext(ctx);
-for (const f of ['js/lbm.js', 'js/voxel.js', 'js/shapes.js']) {
+for (const f of ['js/lbm.js', 'js/voxel.js', 'js/shapes.js', 'js/meshio.js']) {
   vm.runInContext(fs.readFileSync(path.join(REF, f), 'utf8'), ctx, { filename: f });
 }
-const { LBM, Voxel, Shapes } = ctx;
+const { LBM, Voxel, Shapes, MeshIO } = ctx;
 
 // ---------------------------------------------------------------- helpers
 const bytesOf = a => Buffer.from(a.buffer, a.byteOffset, a.byteLength);
@@ -455,3 +455,31 @@ if (want('vox_meshcube_lookup')) {
     mesh: pack(new Float32Array(Shapes.BODIES.meshcube.tris)), stats: v.stats };
   write('vox_meshcube_lookup', out);
 }
+
+if (want('mesh_io')) {
+  // MeshIO.parse (js/meshio.js:322-340) on the files of gen_meshio.py, under
+  // their own names and, for the ones with a mesh extension, under a name
+  // whose extension the dispatch does not know (so the content is sniffed)
+  const dir = path.join(OUT, 'meshio');
+  const cases = [];
+  for (const file of fs.readdirSync(dir).sort()) {
+    const names = [file];
+    if (/\.(stl|obj|glb)$/.test(file)) names.push(file.replace(/\.[a-z]+$/, '') + '_as.dat');
+    for (const name of names) {
+      const b = fs.readFileSync(path.join(dir, file));
+      const ab = b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength);
+      try {
+        const P = MeshIO.parse(name, ab);
+        let nan = 0;
+        for (let i = 0; i < P.length; i++) if (P[i] !== P[i]) nan++;
+        cases.push({ file, name, ok: true, count: P.length, sha256: sha(P), nan,
+          head: Array.from(P.subarray(0, 9)).map(v => (v !== v ? 'NaN' : v)) });
+      } catch (e) {
+        cases.push({ file, name, ok: false, error: e.message });
+      }
+    }
+  }
+  write('mesh_io', { cases });
+  console.log(`  mesh_io: ${cases.length} cases`);
+}
+

@@ -23,7 +23,7 @@ const Voxel = require(path.join(JS, 'voxel.js'));
   const addon = require(path.join(JS, '..', 'wt_napi.node'));
   const exported = ('abiVersion create setParams getParams reset voxelize voxelizeGrid voxelizeParts step ' +
     'stepSlabs stepSlab boundaries setParity getState getFields getPops getSolid getWalls timeSteps ' +
-    'deviceMemory setBoundaryMode slice sampleMany setGray adjointForward adjointBackward adjointRelease voxelBounds voxelPlan voxelBuild').split(' ');
+    'deviceMemory setBoundaryMode slice sampleMany meshParse setGray adjointForward adjointBackward adjointRelease voxelBounds voxelPlan voxelBuild').split(' ');
   assert(Object.keys(addon).sort().join() === exported.sort().join(), 'addon exports: ' + Object.keys(addon).join(' '));
   err = null;
   try { addon.deviceMemory(); } catch (e) { err = e; }
@@ -35,6 +35,21 @@ const Voxel = require(path.join(JS, 'voxel.js'));
   assert(msg(() => Voxel.build(new Float32Array(9).fill(1.5), 96)) === ref.degenerate, 'degenerate');
   assert(msg(() => Voxel.build(new Float32Array([0, 0, 0, 1, 0, 0, 0, 1, 1]), 400)) === ref.too_large, 'too large');
   assert(Voxel.bounds(new Float32Array([1, 2, 3, -1, 5, 0])).join() === '-1,2,0,1,5,3', 'bounds');
+
+  // MeshIO.parse needs no device: the reference's results and messages (mesh_io)
+  const MeshIO = require(path.join(JS, 'meshio.js'));
+  const fs = require('fs'), crypto = require('crypto');
+  const dir = path.join(__dirname, '..', 'golden', 'fixtures', 'meshio');
+  for (const c of load('mesh_io').cases) {
+    const b = fs.readFileSync(path.join(dir, c.file));
+    const ab = b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength);
+    let P = null, m = null;
+    try { P = MeshIO.parse(c.name, ab); } catch (e) { m = e.message; }
+    if (!c.ok) { assert(m === c.error, c.name + ' message: ' + m); continue; }
+    assert(P && P.length === c.count, c.name + ' count');
+    const h = crypto.createHash('sha256').update(Buffer.from(P.buffer, P.byteOffset, P.byteLength)).digest('hex');
+    assert(h === c.sha256, c.name + ' soup');
+  }
 
   // a worker that fails rejects the pool's promise, leaving the caller to fall back
   const sim = { nx: 8, ny: 8, nz: 8, tau: 0.6, les: true, u0: 0.05, movingWalls: false, parity: 0, steps: 0,

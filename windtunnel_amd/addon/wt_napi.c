@@ -766,6 +766,38 @@ static napi_value js_voxel_plan(napi_env env, napi_callback_info info)
     return meta_object(env, &m);
 }
 
+/* meshParse(name, ArrayBuffer | typed array) -> Float32Array (MeshIO.parse) */
+static napi_value js_mesh_parse(napi_env env, napi_callback_info info)
+{
+    napi_value a[2];
+    args(env, info, 2, a);
+    char name[1024];
+    size_t nl = 0;
+    if (napi_get_value_string_utf8(env, a[0], name, sizeof name, &nl) != napi_ok) {
+        napi_throw_type_error(env, NULL, "meshParse needs a file name");
+        return NULL;
+    }
+    void *data = NULL;
+    size_t bytes = 0;
+    bool isab = false;
+    napi_is_arraybuffer(env, a[1], &isab);
+    if (isab) {
+        CALL(env, napi_get_arraybuffer_info(env, a[1], &data, &bytes));
+    } else {
+        int ok;
+        data = tdata(env, a[1], &bytes, &ok);
+        if (!ok) return NULL;
+    }
+    int64_t count = 0;
+    WT(env, wt_mesh_parse(name, data, (int64_t)bytes, NULL, &count));
+    void *out = NULL;
+    napi_value ab, arr;
+    CALL(env, napi_create_arraybuffer(env, (size_t)count * 4, &out, &ab));
+    WT(env, wt_mesh_parse(name, data, (int64_t)bytes, (float *)out, &count));
+    CALL(env, napi_create_typedarray(env, napi_float32_array, (size_t)count, ab, 0, &arr));
+    return arr;
+}
+
 static napi_value js_voxel_build(napi_env env, napi_callback_info info)
 {
     napi_value a[3];
@@ -801,7 +833,7 @@ static napi_value init(napi_env env, napi_value exports)
         {"getParams", js_get_params}, {"reset", js_reset}, {"voxelize", js_voxelize},
         {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs},
         {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
-        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory}, {"setBoundaryMode", js_set_boundary_mode}, {"slice", js_slice}, {"sampleMany", js_sample_many},
+        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory}, {"setBoundaryMode", js_set_boundary_mode}, {"slice", js_slice}, {"sampleMany", js_sample_many}, {"meshParse", js_mesh_parse},
         {"voxelizeParts", js_voxelize_parts}, {"getWalls", js_get_walls}, {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},
         {"adjointRelease", js_adjoint_release},
         {"voxelBounds", js_voxel_bounds}, {"voxelPlan", js_voxel_plan}, {"voxelBuild", js_voxel_build}};

@@ -90,6 +90,7 @@ SIGNATURES = {
     "wt_device_memory": (_I, [_P, _P]),
     "wt_set_boundary_mode": (_I, [_P, _I]),
     "wt_slice": (_I, [_P, _I, _I, _I, _P, _P]),
+    "wt_mesh_parse": (_I, [C.c_char_p, _P, C.c_int64, _P, _P]),
     "wt_sample": (_I, [_P, _P, C.c_int64, _P]),
     "wt_set_timing": (_I, [_P, _I]),
     "wt_get_timing": (_I, [_P, _P, _P]),
@@ -144,6 +145,19 @@ EY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1], dt
 EZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1], dtype=np.int32)
 W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
 FLUID, SOLID_OBJ, SOLID_WALL = 0, 1, 2
+
+
+def mesh_parse(name: str, data: bytes) -> np.ndarray:
+    """MeshIO.parse (wt_mesh_parse): a model file's bytes (STL, OBJ, GLB) into
+    a float32 triangle soup of 9 values per triangle; ValueError with the
+    reference's message for unreadable files."""
+    buf = np.frombuffer(data, dtype=np.uint8)
+    cnt = C.c_int64(0)
+    nm = name.encode()
+    _check(lib().wt_mesh_parse(nm, _ptr(buf), int(buf.size), None, C.byref(cnt)))
+    out = np.empty(cnt.value, dtype=np.float32)
+    _check(lib().wt_mesh_parse(nm, _ptr(buf), int(buf.size), _ptr(out), C.byref(cnt)))
+    return out
 
 
 def device_memory() -> tuple:
