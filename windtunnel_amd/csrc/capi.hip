@@ -73,7 +73,7 @@ void release(wt_lattice *h)
     for (void *p : {(void *)h->d_buf[0], (void *)h->d_solid, (void *)h->d_sbits,
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
                     (void *)h->d_forces, (void *)h->d_near, (void *)h->d_ring,
-                    (void *)h->d_prod})
+                    (void *)h->d_prod, (void *)h->d_ops})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -257,18 +257,40 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     if (!empty && (x0 < 1 || y0 < 1 || z0 < 1 || x1 > nx - 2 || y1 > ny - 2 || z1 > nz - 2))
         return wt_fail(WT_E_ARG, "box [%d,%d,%d]-[%d,%d,%d] is not within the interior", x0, y0, z0, x1, y1, z1);
     if (!empty && !mask) return wt_fail(WT_E_ARG, "null mask for a non-empty box");
-    const int64_t n = h->n;
-    std::vector<uint8_t> ops((size_t)n, 0);
+    if (!empty && !wv) {
+        const int64_t cnt = (int64_t)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
+        for (int64_t k = 0; k < cnt; k++)
+            if (mask[k] == 2) return wt_fail(WT_E_ARG, "mask holds wall-velocity cells but wv is null");
+    }
+    const int64_t n = h->n, plane = (int64_t)nx * ny;
+    if (h->h_ops.empty()) h->h_ops.assign((size_t)n, 0);
+    uint8_t *ops = h->h_ops.data();
     uint8_t *solid = h->h_solid.data();
     float *uwx = h->h_uw.data(), *uwy = uwx + n, *uwz = uwx + 2 * n;
-    // js/lbm.js:194-202: the old body becomes fluid at u0
+    // planes whose solid state or ops change, and planes whose wall velocities change
+    int zlo = nz, zhi = -1, ulo = nz, uhi = -1;
+    auto touch = [](int &lo, int &hi, int z) { lo = std::min(lo, z); hi = std::max(hi, z); };
+    auto set_uw = [&](int64_t i, int z, float x, float y, float w) {
+        if (uwx[i] != x || uwy[i] != y || uwz[i] != w) {
+            uwx[i] = x; uwy[i] = y; uwz[i] = w;
+            touch(ulo, uhi, z);
+        }
+    };
+    // js/lbm.js:194-202: the old body becomes fluid at u0 (all of it lies in
+    // the previous body's box)
     h->ground = !!ground;
     int mov = 0;
-    for (int64_t i = 0; i < n; i++) {
-        if (solid[i] == wt::SOLID_OBJ) {
-            solid[i] = wt::FLUID; ops[i] |= 1;
-            uwx[i] = uwy[i] = uwz[i] = 0;
-        }
+    {
+        const int *ob = h->obj_box;
+        for (int z = ob[2]; z <= ob[5]; z++)
+            for (int y = ob[1]; y <= ob[4]; y++)
+                for (int x = ob[0]; x <= ob[3]; x++) {
+                    const int64_t i = x + nx * ((int64_t)y + (int64_t)ny * z);
+                    if (solid[i] != wt::SOLID_OBJ) continue;
+                    solid[i] = wt::FLUID; ops[i] |= 1;
+                    set_uw(i, z, 0, 0, 0);
+                    touch(zlo, zhi, z);
+                }
     }
     // js/lbm.js:204-214: the road row
     const int gy = ground ? wt::SOLID_WALL : wt::FLUID;
@@ -276,16 +298,20 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     if (rv != 0) mov = 1;
     for (int z = 0; z < nz; z++) {
         for (int x = 0; x < nx; x++) {
-            const int64_t i = x + (int64_t)nx * ny * z;
-            if (solid[i] != gy) { solid[i] = (uint8_t)gy; if (gy == wt::FLUID) ops[i] |= 1; }
-            uwx[i] = gy == wt::SOLID_WALL ? (float)rv : 0.0f;
-            uwy[i] = uwz[i] = 0;
+            const int64_t i = x + plane * z;
+            if (solid[i] != gy) {
+                solid[i] = (uint8_t)gy;
+                if (gy == wt::FLUID) ops[i] |= 1;
+                touch(zlo, zhi, z);
+            }
+            set_uw(i, z, gy == wt::SOLID_WALL ? (float)rv : 0.0f, 0, 0);
         }
     }
     // js/lbm.js:216-246: the new body, its projections and lowest cells
     std::vector<uint8_t> col((size_t)ny * nz, 0), plan((size_t)nx * nz, 0);
     std::vector<int16_t> low((size_t)nx * nz, 32767);
     int64_t cells = 0, k = 0, kw = 0;
+    int nb[6] = {nx, ny, nz, -1, -1, -1};  // box of the new body's cells
     if (!empty) {
         for (int z = z0; z <= z1; z++)
             for (int y = y0; y <= y1; y++) {
@@ -294,21 +320,25 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
                     if (!mask[k]) continue;
                     solid[i] = wt::SOLID_OBJ;
                     ops[i] |= 2;
-                    uwx[i] = uwy[i] = uwz[i] = 0;
+                    touch(zlo, zhi, z);
+                    float wx = 0, wy = 0, wz = 0;
                     if (mask[k] == 2) {
-                        if (!wv) return wt_fail(WT_E_ARG, "mask holds wall-velocity cells but wv is null");
-                        uwx[i] = wv[3 * kw]; uwy[i] = wv[3 * kw + 1]; uwz[i] = wv[3 * kw + 2];
+                        wx = wv[3 * kw]; wy = wv[3 * kw + 1]; wz = wv[3 * kw + 2];
                         kw++;
                         mov = 1;
                     }
+                    set_uw(i, z, wx, wy, wz);
                     col[y + (size_t)ny * z] = 1;
                     const size_t pk = x + (size_t)nx * z;
                     plan[pk] = 1;
                     if (y < low[pk]) low[pk] = (int16_t)y;
                     cells++;
+                    nb[0] = std::min(nb[0], x); nb[1] = std::min(nb[1], y); nb[2] = std::min(nb[2], z);
+                    nb[3] = std::max(nb[3], x); nb[4] = std::max(nb[4], y); nb[5] = std::max(nb[5], z);
                 }
             }
     }
+    std::copy(nb, nb + 6, h->obj_box);
     int64_t a = 0, p = 0;
     for (uint8_t c : col) a += c;
     for (uint8_t c : plan) p += c;
@@ -322,15 +352,31 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     h->body.rideHeight = gaps.empty() ? 0 : gaps[gaps.size() >> 1];
     h->p.movingWalls = mov;
 
-    uint8_t *d_ops = nullptr;
-    if (int rc = alloc((void **)&d_ops, (size_t)n)) return rc;
+    // device: the changed planes only
     int rc = 0;
-    if (hipMemcpyAsync(d_ops, ops.data(), (size_t)n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
-        rc = wt_fail(WT_E_DEVICE, "voxelize upload failed");
-    if (!rc) rc = lbm_apply_voxelize(h, d_ops);
-    if (!rc) rc = upload_body(h);
+    if (!h->d_ops) {
+        rc = alloc((void **)&h->d_ops, (size_t)n);
+        if (!rc && hipMemset(h->d_ops, 0, (size_t)n) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "hipMemset failed");
+    }
+    if (!rc && zhi >= zlo) {
+        const int64_t c0 = plane * zlo, c1 = plane * (zhi + 1);
+        if (hipMemcpyAsync(h->d_ops + c0, ops + c0, (size_t)(c1 - c0), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemcpyAsync(h->d_solid + c0, solid + c0, (size_t)(c1 - c0), hipMemcpyHostToDevice, h->stream) !=
+                hipSuccess)
+            rc = wt_fail(WT_E_DEVICE, "voxelize upload failed");
+        if (!rc) rc = lbm_apply_voxelize(h, h->d_ops, c0, c1);
+    }
+    if (!rc && uhi >= ulo) {
+        const int64_t c0 = plane * ulo, c1 = plane * (uhi + 1);
+        for (int c = 0; c < 3 && !rc; c++)
+            if (hipMemcpyAsync(h->d_uw + c * n + c0, h->h_uw.data() + c * n + c0, sizeof(float) * (size_t)(c1 - c0),
+                               hipMemcpyHostToDevice, h->stream) != hipSuccess)
+                rc = wt_fail(WT_E_DEVICE, "voxelize upload failed");
+    }
+    // the near-wall mask of a plane depends on the planes beside it
+    if (!rc && zhi >= zlo) rc = lbm_rebuild_masks_range(h, std::max(0, zlo - 1), std::min(nz - 1, zhi + 1));
     if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "voxelize failed");
-    (void)hipFree(d_ops);
+    if (zhi >= zlo) std::fill(ops + plane * zlo, ops + plane * (zhi + 1), 0);
     if (rc) return rc;
     if (out) {
         out->frontalArea = h->body.frontalArea; out->planformArea = h->body.planformArea;

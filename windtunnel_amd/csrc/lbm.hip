@@ -712,8 +712,10 @@ __global__ void __launch_bounds__(256) k_channel(ChanArgs<S, MAC> a)
 template <class S, class MAC> struct InitArgs {
     S *f, *g;  // AB: f and g; esoteric: f = the single buffer, g unused
     MAC *rho, *ux, *uy, *uz;
-    const uint8_t *solid, *ops;
+    const uint8_t *solid;
+    uint8_t *ops;               // voxelize: per-cell ops, cleared once applied
     int64_t n, ns;
+    int64_t c0, c1;             // voxelize: cells [c0, c1) that may carry ops
     Lay lay;
     int nx, ny, nz, rep, eso;
     double u0;
@@ -756,10 +758,11 @@ __global__ void __launch_bounds__(256) k_reset(InitArgs<S, MAC> a)
 template <class S, class MAC>
 __global__ void __launch_bounds__(256) k_apply_ops(InitArgs<S, MAC> a)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    const int64_t i = a.c0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= a.c1) return;
     const int op = a.ops[i];
     if (!op) return;
+    a.ops[i] = 0;  // the buffer stays zero between voxelize calls
     if (op & 1) {
         write_state(a, i, a.eq_u0, true);
         a.rho[i] = 1; a.ux[i] = (MAC)a.u0; a.uy[i] = 0; a.uz[i] = 0;
@@ -792,11 +795,11 @@ __global__ void __launch_bounds__(256) k_read_pops(const S *B, const uint8_t *so
 // Bit-packed solid mask and near-wall mask (js/lbm.js:268-282): near = an
 // interior fluid cell with a solid cell among its 18 upstream neighbours.
 __global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, const float *wgt, uint32_t *sbits,
-                                               uint32_t *nbits, int nx, int ny, int nz, int nwx)
+                                               uint32_t *nbits, int nx, int ny, int nz, int nwx, int64_t w0,
+                                               int64_t w1)
 {
-    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t nwords = (int64_t)nwx * ny * nz;
-    if (w >= nwords) return;
+    const int64_t w = w0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (w >= w1) return;
     const int wx = (int)(w % nwx);
     const int64_t row = w / nwx;
     const int y = (int)(row % ny), z = (int)(row / ny);
@@ -862,7 +865,7 @@ __global__ void __launch_bounds__(256) k_grid_mask(GridArgs a)
     a.mask[k] = m;
 }
 
-template <class S, class MAC> InitArgs<S, MAC> init_args(wt_lattice *h, const uint8_t *ops)
+template <class S, class MAC> InitArgs<S, MAC> init_args(wt_lattice *h, uint8_t *ops)
 {
     InitArgs<S, MAC> a{};
     const bool eso = h->layout == WT_LAYOUT_ESOTERIC;
@@ -886,10 +889,13 @@ template <class S, class MAC> int reset_t(wt_lattice *h)
     return 0;
 }
 
-template <class S, class MAC> int ops_t(wt_lattice *h, const uint8_t *d_ops)
+template <class S, class MAC> int ops_t(wt_lattice *h, uint8_t *d_ops, int64_t c0, int64_t c1)
 {
     auto a = init_args<S, MAC>(h, d_ops);
-    const int64_t nb = (h->n + 255) / 256;
+    a.c0 = c0;
+    a.c1 = c1;
+    const int64_t nb = (c1 - c0 + 255) / 256;
+    if (nb <= 0) return 0;
     hipLaunchKernelGGL((k_apply_ops<S, MAC>), dim3((unsigned)nb), dim3(256), 0, h->stream, a);
     WT_HIP(hipGetLastError());
     return 0;
@@ -1037,21 +1043,26 @@ int lbm_reset(wt_lattice *h)
     }
 }
 
-int lbm_apply_voxelize(wt_lattice *h, const uint8_t *d_ops)
+int lbm_apply_voxelize(wt_lattice *h, uint8_t *d_ops, int64_t c0, int64_t c1)
 {
     switch (h->storage) {
-    case WT_STORE_F32: return ops_t<float, float>(h, d_ops);
-    case WT_STORE_F16: return ops_t<__half, float>(h, d_ops);
-    default: return ops_t<double, double>(h, d_ops);
+    case WT_STORE_F32: return ops_t<float, float>(h, d_ops, c0, c1);
+    case WT_STORE_F16: return ops_t<__half, float>(h, d_ops, c0, c1);
+    default: return ops_t<double, double>(h, d_ops, c0, c1);
     }
 }
 
-int lbm_rebuild_masks(wt_lattice *h)
+int lbm_rebuild_masks(wt_lattice *h) { return lbm_rebuild_masks_range(h, 0, h->nz - 1); }
+
+int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1)
 {
     const int64_t nwords = (int64_t)h->nwx * h->ny * h->nz;
-    hipLaunchKernelGGL(k_masks, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, h->stream, h->d_solid,
-                       h->d_wgt, h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx);
-    WT_HIP(hipGetLastError());
+    const int64_t w0 = (int64_t)h->nwx * h->ny * z0, w1 = (int64_t)h->nwx * h->ny * (z1 + 1);
+    if (w1 > w0) {
+        hipLaunchKernelGGL(k_masks, dim3((unsigned)((w1 - w0 + 255) / 256)), dim3(256), 0, h->stream, h->d_solid,
+                           h->d_wgt, h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx, w0, w1);
+        WT_HIP(hipGetLastError());
+    }
     // list of the near-wall cells in cell order, and where each z plane's start
     std::vector<uint32_t> bits((size_t)nwords);
     WT_HIP(hipMemcpyAsync(bits.data(), h->d_nbits, sizeof(uint32_t) * (size_t)nwords, hipMemcpyDeviceToHost,

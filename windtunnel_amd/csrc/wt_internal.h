@@ -135,6 +135,11 @@ struct wt_lattice {
 
     std::vector<uint8_t> h_solid;  // host mirror of the solid array
     std::vector<float> h_uw;       // host mirror of the wall velocities (3n)
+    // voxelize: the last body's box (every SOLID_OBJ cell lies in it; x0 > x1:
+    // none), and the per-cell ops on the host and device, kept zero between calls
+    int obj_box[6] = {1, 1, 1, 0, 0, 0};
+    std::vector<uint8_t> h_ops;
+    uint8_t *d_ops = nullptr;
 };
 
 // error reporting (capi.hip)
@@ -148,8 +153,11 @@ int wt_fail(int code, const char *fmt, ...);
 
 // lattice kernels (lbm.hip)
 int lbm_reset(wt_lattice *h);
-int lbm_apply_voxelize(wt_lattice *h, const uint8_t *ops);
+// apply the voxelize ops of cells [c0, c1) (and clear them in d_ops)
+int lbm_apply_voxelize(wt_lattice *h, uint8_t *d_ops, int64_t c0, int64_t c1);
 int lbm_rebuild_masks(wt_lattice *h);
+// masks of planes z0..z1 only (and the near-wall list of the whole lattice)
+int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1);
 int lbm_channel(wt_lattice *h, bool macro);
 // Launches one step over z rows [bz0,bz1). Its force (when force_out is
 // given) is reduced into force_out by the NEXT step launch or by
