@@ -335,6 +335,73 @@ def run_cpu(wl, threads):
     return n / best / 1e6
 
 
+def host_cpus() -> dict:
+    """Host description: model, logical CPUs, physical cores, and the CPUs this
+    process may run on (nproc)."""
+    cores = set()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            phys = core = None
+            for line in fh:
+                if line.startswith("physical id"):
+                    phys = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    core = line.split(":")[1].strip()
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    return dict(model=cpu_model(), logical=os.cpu_count(), physical_cores=len(cores) or None,
+                nproc=len(os.sched_getaffinity(0)))
+
+
+def run_cpu_detail(threads):
+    """The oracle at 1 thread and at `threads` threads on the CPU-runnable
+    96x48x48 BGK sphere (12 warm-up steps, best of 3 x 20 steps: the
+    reference benchmark's protocol, js/main.js:614-620) and on the
+    256x128x128 TRT+LES mesh body (2 warm-up steps, best of 3 x 4 steps)."""
+    from oracle.oracle import OracleLBM
+
+    def timed(O, warm, reps, steps, t):
+        O.step_pool(warm, t, threaded=True)
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            O.step_pool(steps, t, threaded=True)
+            best = min(best, (time.perf_counter() - t0) / steps)
+        return round(O.n / best / 1e6, 2)
+
+    out = {"host": host_cpus()}
+    lo, hi = [23, 15, 15], [41, 33, 33]
+    z, y, x = np.meshgrid(np.arange(lo[2], hi[2] + 1) + 0.5, np.arange(lo[1], hi[1] + 1) + 0.5,
+                          np.arange(lo[0], hi[0] + 1) + 0.5, indexing="ij")
+    mask = ((x - 32) ** 2 + (y - 24) ** 2 + (z - 24) ** 2 <= 64).astype(np.uint8).reshape(-1)
+    c1 = {}
+    for t in sorted({1, threads}):
+        O = OracleLBM(96, 48, 48)
+        O.set_params(tau=0.56, les=False, trt=False, u0=0.05, csm=0.14, **{"lambda": 0.25})
+        O.reset()
+        O.voxelize_mask(lo + hi, mask)
+        c1[f"{t}_threads"] = timed(O, 12, 3, 20, t)
+        del O
+    out["c1_96x48x48_bgk_mlups"] = c1
+    wl = WORKLOADS["c2"]
+    box, m2 = _cpu_mask(wl["dims"], wl["scale"])
+    c2 = {}
+    for t in sorted({1, threads}):
+        O = OracleLBM(*wl["dims"])
+        O.set_params(tau=wl["tau"], les=True, csm=wl["csm"], trt=True, u0=wl["u0"], **{"lambda": wl["lam"]})
+        O.reset()
+        O.voxelize_mask(box, m2)
+        c2[f"{t}_threads"] = timed(O, 2, 3, 4, t)
+        del O
+    out["c2_256x128x128_trt_les_mlups"] = c2
+    out["reference_lbm_js_node_1_thread_mlups"] = {
+        "c1": 7.7, "c2": 2.0,
+        "note": "js/lbm.js itself under Node v12, measured in the development container (8 x Xeon @ 2.70 GHz), "
+                "not on this host"}
+    return out
+
+
 # ---------------------------------------------------------------------- c1
 def run_c1(steps=2000):
     """The CPU-runnable workload on the GPU: 96x48x48 BGK, sphere r = 8 at
@@ -520,6 +587,8 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline, {threads} threads")
         v = run_cpu(wl, threads)
+        log("cpu detail: 1 and all threads, 96x48x48 and 256x128x128")
+        out["cpu_detail"] = run_cpu_detail(threads)
         out["cpu_baseline"] = {"value": round(v, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
                                "sample": f"oracle (C restatement of js/lbm.js, binary64 arithmetic, float32 "
                                          f"storage) on the full {nx}x{ny}x{nz} lattice with the same body, "

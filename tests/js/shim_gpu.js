@@ -196,6 +196,25 @@ check('slice and sampleMany equal the app\'s own formulas over the downloaded fi
   }
 });
 
+check('boundaryMode selects the channel boundaries', () => {
+  const fx = load('lbm_c1');
+  const run = mode => {
+    const L = new LBM(...fx.dims);
+    applyParams(L, fx.params);
+    L.boundaryMode = mode;
+    L.reset();
+    replay(L, fx.voxelize);
+    L.stepMany(50);
+    return L;
+  };
+  const A = run('reference'), B = run('channel');
+  assert(B.boundaryMode === 'channel' && A.boundaryMode === 'reference', 'mode property');
+  assert(sha(A.ux) !== sha(B.ux), 'the channel boundaries change the flow');
+  let threw = false;
+  try { B.boundaryMode = 'periodic'; } catch (e) { threw = true; }
+  assert(threw, 'unknown mode refused');
+});
+
 check('Voxel.build is bit-exact (vox_box, vox_hitcap, vox_sphere)', () => {
   for (const name of ['vox_box', 'vox_hitcap', 'vox_sphere']) {
     const fx = load(name);

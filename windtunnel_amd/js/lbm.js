@@ -82,6 +82,14 @@ class LBM {
 
   _dirty() { this._fields = null; }
 
+  /** Boundaries: 'reference' (js/lbm.js:473-512) or 'channel' (free-slip
+   * side walls, equilibrium outflow; wt_set_boundary_mode). */
+  get boundaryMode() { return this._bc || 'reference'; }
+  set boundaryMode(m) {
+    native().setBoundaryMode(this._h, m === 'channel' ? 1 : (m === 'reference' ? 0 : -1));
+    this._bc = m;
+  }
+
   get parity() { return native().getState(this._h).parity; }
   set parity(p) { native().setParity(this._h, p & 1); }
   get nu() { return (this.tau - 0.5) / 3; }
