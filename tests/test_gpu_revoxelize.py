@@ -1,0 +1,77 @@
+"""A sequence of re-voxelizes on one lattice (body moved, ground and road
+velocity toggled, wall-velocity cells added and removed, an empty box)
+against the oracle, whose voxelize scans the whole lattice as
+js/lbm.js:191-266 does: the device keeps only the previous body's box and
+uploads only the planes that changed, and must end in the same state bit
+for bit (populations, fields, solid and near maps, wall velocities, body
+statistics) after every call and after steps in between."""
+import numpy as np
+import pytest
+
+from _util import sha
+from oracle.oracle import OracleLBM
+from windtunnel_amd.lbm import LBM
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (40, 20, 18)
+
+
+def sphere_job(c, r, wheel=False):
+    lo = [int(np.floor(c[k] - r)) - 1 for k in range(3)]
+    hi = [int(np.ceil(c[k] + r)) + 1 for k in range(3)]
+    lo = [max(1, v) for v in lo]
+    hi = [min(DIMS[k] - 2, hi[k]) for k in range(3)]
+    z, y, x = np.meshgrid(np.arange(lo[2], hi[2] + 1) + 0.5, np.arange(lo[1], hi[1] + 1) + 0.5,
+                          np.arange(lo[0], hi[0] + 1) + 0.5, indexing="ij")
+    inside = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 <= r * r
+    mask = inside.astype(np.uint8)
+    wv = np.zeros(0, np.float32)
+    if wheel:  # the lower half spins: wall velocity cells
+        low = inside & (y < c[1])
+        mask[low] = 2
+        k = int(low.sum())
+        wv = np.tile(np.array([0.02, 0.0, -0.01], np.float32), k)
+    return lo + hi, mask.reshape(-1), wv
+
+
+JOBS = [
+    (sphere_job((12, 8, 9), 3.5), False, 0.0),
+    (sphere_job((14, 8, 9), 3.5), True, 0.03),           # moved, ground with a moving road
+    (sphere_job((14, 9, 9), 4.0, wheel=True), True, 0.03),
+    (sphere_job((20, 10, 8), 3.0, wheel=True), False, 0.0),
+    ([5, 5, 5, 4, 4, 4], False, 0.0),                     # empty box: the body leaves
+    (sphere_job((25, 6, 10), 3.0), True, 0.0),
+]
+
+
+def state(L, oracle):
+    if oracle:
+        keys = ("f", "g", "rho", "ux", "uy", "uz", "solid", "near", "uwx", "uwy", "uwz")
+        return {k: sha(L.array(k)) for k in keys}
+    F = L.fields()
+    s, nr = L.solid_near()
+    out = {"f": sha(L.f), "g": sha(L.g), "solid": sha(s), "near": sha(nr)}
+    out.update({k: sha(F[k]) for k in ("rho", "ux", "uy", "uz")})
+    out.update(dict(zip(("uwx", "uwy", "uwz"), (sha(w) for w in L.walls()))))
+    return out
+
+
+def test_revoxelize_sequence_equals_oracle():
+    O = OracleLBM(*DIMS)
+    G = LBM(*DIMS, arith="f64")
+    for L in (O, G):
+        L.set_params(tau=0.6, les=True, trt=True, csm=0.14, u0=0.05, **{"lambda": 3 / 16})
+        L.reset()
+    for j, (job, ground, road) in enumerate(JOBS):
+        box, mask, wv = job if isinstance(job, tuple) else (job, np.zeros(0, np.uint8), np.zeros(0, np.float32))
+        print(f"[revox] job {j}: box {box} ground {ground} road {road}", flush=True)
+        so = O.voxelize_mask(box, mask, wv, ground, road)
+        G.voxelize_mask(box, mask, wv, ground, road)
+        assert (G.frontalArea, G.planformArea, G.solidCells, G.rideHeight) == (
+            so["frontalArea"], so["planformArea"], so["solidCells"], so["rideHeight"]), j
+        assert bool(G.movingWalls) == bool(so["movingWalls"]), j
+        assert state(G, False) == state(O, True), j
+        fo, fg = O.step(7), G.step(7)
+        np.testing.assert_allclose(fg, fo, rtol=1e-11, atol=1e-14)
+        assert state(G, False) == state(O, True), j
