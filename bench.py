@@ -177,6 +177,9 @@ def run_precision(c3_steps=4000):
 
     * laminar: 128x64x64, TRT tau 0.6, u0 0.05 (steady wake), 4000 steps;
       deterministic, so fields and C_d compare directly against binary64.
+    * les_steady: TRT + Smagorinsky LES with the configured Cs = 0.1 on
+      128x64x64 at tau 0.55, u0 0.05: the eddy viscosity is active but the
+      wake stays steady, so the time-averaged C_d compares directly.
     * les_128: the TRT+LES form (tau 0.505, u0 0.08) on 128x64x64, 4000
       steps: a chaotic wake, so instantaneous fields decorrelate and a C_d
       mean over 2000 steps carries its SEM (which understates the error at
@@ -189,12 +192,15 @@ def run_precision(c3_steps=4000):
     """
     f64 = [("f64", "f64"), ("f32", "f32"), ("f16", "f32")]
     res = dict(laminar=dict(grid="128x64x64", tau=0.6, u0=0.05, les=False, steps=4000, cd_window=2000),
+               les_steady=dict(grid="128x64x64", tau=0.55, u0=0.05, les=True, csm=0.1, steps=4000, cd_window=2000),
                les_128=dict(grid="128x64x64", tau=0.505, u0=0.08, les=True, csm=0.14, steps=4000, cd_window=2000,
                             chaotic=True),
                c3=dict(grid="512x256x256", tau=0.505, u0=0.08, les=True, csm=0.14, steps=c3_steps,
                        cd_window=c3_steps // 2, chaotic=True,
                        note="Cs=0.1 diverges within 1000 steps here, in binary64 arithmetic as in float"))
     res["laminar"].update(_compare(_storage_runs((128, 64, 64), 0.5, 0.6, 0.05, False, 4000, 2000, f64), "f64"))
+    res["les_steady"].update(_compare(_storage_runs((128, 64, 64), 0.5, 0.55, 0.05, True, 4000, 2000, f64, 0.1),
+                                      "f64"))
     res["les_128"].update(_compare(_storage_runs((128, 64, 64), 0.5, 0.505, 0.08, True, 4000, 2000, f64, 0.14), "f64"))
     if c3_steps > 0:
         res["c3"].update(_compare(_storage_runs((512, 256, 256), 2, 0.505, 0.08, True, c3_steps, c3_steps // 2,

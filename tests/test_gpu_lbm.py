@@ -242,7 +242,9 @@ def test_worker_protocol_through_step_slab(layout):
 
 
 def test_fp16_storage_tracks_fp64():
-    """binary16 deviations with float32 collision against the FP64 oracle."""
+    """binary16 deviations with float32 collision against the FP64 oracle
+    (a short transient: stochastic rounding's noise, about 1.4x that of
+    round-to-nearest per store, sets the bound here)."""
     fx = load("lbm_trt_les")
     L = make(fx, storage="f16")
     replay_voxelize(L, fx["voxelize"])
@@ -252,8 +254,32 @@ def test_fp16_storage_tracks_fp64():
     F = L.fields()
     u = np.sqrt(F["ux"].astype(np.float64) ** 2 + F["uy"].astype(np.float64) ** 2)
     uo = np.sqrt(O.array("ux") ** 2 + O.array("uy") ** 2)
-    assert rel_l2(u, uo) < 2e-3
+    assert rel_l2(u, uo) < 5e-3
     assert rel_l2(F["rho"], O.array("rho")) < 1e-4
+
+
+def test_fp16_steady_drag_matches_fp64():
+    """A steady laminar wake over 3000 steps: binary16 storage with
+    stochastic rounding keeps the time-averaged C_d and the velocity field
+    at the binary64 values; round-to-nearest stalls updates below half a
+    binary16 step and ends ~0.3 % off in C_d (1 % and more at lower
+    viscosity)."""
+    dims = (96, 40, 40)
+    runs = {}
+    for storage, arith in (("f64", "f64"), ("f16", "f32")):
+        L = LBM(*dims, storage=storage, arith=arith)
+        L.set_params(tau=0.58, les=False, trt=True, csm=0.14, u0=0.05, **{"lambda": 3 / 16})
+        L.reset()
+        z, y, x = np.meshgrid(np.arange(12, 29) + 0.5, np.arange(12, 29) + 0.5, np.arange(22, 39) + 0.5, indexing="ij")
+        m = ((x - 30) ** 2 + (y - 20) ** 2 + (z - 20) ** 2 <= 36).astype(np.uint8).reshape(-1)
+        L.voxelize_mask([22, 12, 12, 38, 28, 28], m)
+        F = L.step(3000)
+        runs[storage] = (np.mean(F[-1000:, 0]), L.fields()["ux"].astype(np.float64))
+        L.close()
+    cd_err = abs(runs["f16"][0] / runs["f64"][0] - 1)
+    u_err = rel_l2(runs["f16"][1], runs["f64"][1])
+    print(f"[f16 steady] C_d rel err {cd_err:.2e}, ux rel-L2 {u_err:.2e}", flush=True)
+    assert cd_err < 5e-4 and u_err < 2e-3
 
 
 def test_determinism_fp32():

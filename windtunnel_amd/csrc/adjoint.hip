@@ -336,6 +336,7 @@ struct Buf {
 // or replaced, so the reverse sweep can run any number of times.
 struct AdjRecord {
     int N = 0, K = 0, nseg = 0;
+    int64_t step0 = 0;         // lattice step count when the forward started
     size_t st = 0;             // bytes of one stored state
     int storage = 0, arith = 0;
     uint64_t epoch = 0;        // lattice configuration epoch at record time
@@ -350,6 +351,7 @@ int record_forward(wt_lattice *h, int N, int K, double *forces)
     R->N = N; R->K = K; R->nseg = (N + K - 1) / K;
     R->st = 19 * (size_t)h->ns * h->pop_elem;
     R->storage = h->storage; R->arith = h->arith; R->epoch = h->epoch;
+    R->step0 = h->steps;
     R->forces.assign((size_t)N * 3, 0.0);
     if (hipMalloc(&R->ck, R->st * (size_t)R->nseg) != hipSuccess) {
         R->ck = nullptr;
@@ -479,13 +481,16 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     // the replays below drive the forward kernel on explicit buffers; the
     // lattice's own state and counters are left untouched
     const int parity = h->parity;
-    auto replay = [&](void *from, void *to) {
+    // forward step k (s_k -> s_{k+1}) again, as recorded
+    auto replay = [&](void *from, void *to, int k) {
         h->ovF = from;
         h->ovG = to;
         h->replay = 1;
+        h->seed_step = R.step0 + k;
         const int rc = lbm_launch_step(h, 0, h->nz, false, nullptr);
         h->ovF = h->ovG = nullptr;
         h->replay = 0;
+        h->seed_step = -1;
         return rc;
     };
     for (int seg = nseg - 1; seg >= 0; seg--) {
@@ -499,11 +504,11 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
             const int s0 = std::max(a0 - 1, 0);
             WT_HIP(hipMemcpyAsync(rp[0], (char *)R.ck + (size_t)seg * st, st, hipMemcpyDeviceToDevice, h->stream));
             for (int c = base; c < s0; c++) {
-                if (int rc = replay(rp[0], rp[1])) return rc;
+                if (int rc = replay(rp[0], rp[1], c)) return rc;
                 std::swap(rp[0], rp[1]);
             }
             for (int r = 0; s0 + r < b - 1; r++)
-                if (int rc = replay(rp[(size_t)r], rp[(size_t)r + 1])) return rc;
+                if (int rc = replay(rp[(size_t)r], rp[(size_t)r + 1], s0 + r)) return rc;
             for (int k = b - 1; k >= a0; k--) {
                 a.mu = (const M *)mu_k;
                 a.mun = (M *)mu_n;

@@ -92,6 +92,7 @@ struct wt_lattice {
     float *d_wgt = nullptr;       // gray bounce-back weights, or null
     void *ovF = nullptr, *ovG = nullptr;  // explicit step buffers (adjoint replay), AB only
     int replay = 0;                       // replayed steps leave the macroscopic fields alone
+    int64_t seed_step = -1;               // >= 0: the step whose rounding seed a replay uses
     void *adj = nullptr;                  // recorded adjoint forward pass (adjoint.hip), or null
     uint64_t epoch = 0;                   // bumped by every change of parameters, body or weights
     void *d_mac = nullptr;        // rho, ux, uy, uz planes of n values
