@@ -408,6 +408,32 @@ def run_cpu_detail(threads):
     return out
 
 
+# ------------------------------------------------------------------- micro
+def run_micro(steps=100):
+    """Kernel microbenchmark: 512x256x256 TRT+LES with no body, populations
+    set to random deviations uniform in +-1e-3 (seed 7): the step kernel
+    alone against the HBM roofline."""
+    from windtunnel_amd.lbm import LBM
+    dims = WORKLOADS["c3"]["dims"]
+    n = dims[0] * dims[1] * dims[2]
+    r = np.random.default_rng(7).random(19 * n, dtype=np.float32)
+    r *= 2e-3
+    r -= 1e-3
+    out = {"grid": "512x256x256", "body": None, "populations": "uniform +-1e-3 deviations, seed 7", "steps": steps}
+    for storage, bpc in (("f32", 152), ("f16", 76)):
+        L = LBM(*dims, storage=storage)
+        set_flow(L, WORKLOADS["c3"])
+        L.reset()
+        L.set_pops(r)
+        L.time_steps(10)
+        ms = L.time_steps(steps) / steps
+        out[storage] = dict(kernel_ms=round(ms, 5), mlups=round(n / (ms * 1e-3) / 1e6, 1),
+                            roofline_frac=round(n * bpc / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4))
+        L.close()
+    del r
+    return out
+
+
 # ---------------------------------------------------------------------- c1
 def run_c1(steps=2000):
     """The CPU-runnable workload on the GPU: 96x48x48 BGK, sphere r = 8 at
@@ -526,6 +552,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-micro", action="store_true")
     ap.add_argument("--fd-cells", type=int, default=64, help="central-difference cells of the adjoint leg (0: none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy three times (HBM counter calibration)")
@@ -574,6 +601,10 @@ def main():
                                     "unit": "GB/s", "frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                     "traffic": traffic.get("f16"), "kernel_ms": round(r16["kernel_ms"], 5),
                                     "bytes_per_cell": 76, "bytes_per_launch": nx * ny * nz * 76}}
+    if not a.no_micro:
+        log("micro 512x256x256, no body, random populations")
+        out["micro"] = run_micro()
+        log(f"micro: {json.dumps(out['micro'])}")
     if not a.no_precision:
         log("precision: 128x64x64 laminar and LES (f64/f32/f16), 512x256x256 f16 vs f32, 4000 steps")
         out["precision"] = run_precision()

@@ -194,6 +194,16 @@ int wt_get_fields(wt_lattice *h, void *rho, void *ux, void *uy, void *uz);
  * only which=0 is available; solid cells read as 0 there. */
 int wt_get_pops(wt_lattice *h, int which, void *out);
 
+/* The inverse, for restoring a saved state (no reference counterpart): the
+ * current populations from f[q*n + cell] deviations in the same types
+ * (binary16 storage rounds the floats to nearest). In place, values of
+ * solid cells and of slots pointing out of the lattice are not stored. The
+ * macroscopic fields keep their values until the next step computes them.
+ * Binary16 storage rounds collision results with bits drawn from the step
+ * count, so a restored binary16 run repeats the original bit for bit only
+ * at the same step count. */
+int wt_set_pops(wt_lattice *h, const void *in);
+
 /* solid (0 fluid, 1 body, 2 road) and near-wall flags (js/lbm.js:82-83). */
 int wt_get_solid(wt_lattice *h, uint8_t *solid, uint8_t *near);
 

@@ -22,7 +22,7 @@ const Voxel = require(path.join(JS, 'voxel.js'));
   // the addon binds the C ABI one to one
   const addon = require(path.join(JS, '..', 'wt_napi.node'));
   const exported = ('abiVersion create setParams getParams reset voxelize voxelizeGrid voxelizeParts step ' +
-    'stepSlabs stepSlab boundaries setParity getState getFields getPops getSolid getWalls timeSteps ' +
+    'stepSlabs stepSlab boundaries setParity getState getFields getPops setPops getSolid getWalls timeSteps ' +
     'deviceMemory setBoundaryMode slice sampleMany meshParse setGray adjointForward adjointBackward adjointRelease voxelBounds voxelPlan voxelBuild').split(' ');
   assert(Object.keys(addon).sort().join() === exported.sort().join(), 'addon exports: ' + Object.keys(addon).join(' '));
   err = null;

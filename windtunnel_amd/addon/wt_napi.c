@@ -665,6 +665,22 @@ static napi_value js_get_pops(napi_env env, napi_callback_info info)
     return NULL;
 }
 
+static napi_value js_set_pops(napi_env env, napi_callback_info info)
+{
+    napi_value a[2];
+    args(env, info, 2, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    size_t nb = 0, cells, fb;
+    if (!lattice_info(env, h, &cells, &fb)) return NULL;
+    void *in = tdata(env, a[1], &nb, &ok);
+    if (!ok) return NULL;
+    if (!enough(env, in ? nb : 0, 19 * cells * fb, "setPops input")) return NULL;
+    WT(env, wt_set_pops(h, in));
+    return NULL;
+}
+
 static napi_value js_get_solid(napi_env env, napi_callback_info info)
 {
     napi_value a[3];
@@ -833,7 +849,7 @@ static napi_value init(napi_env env, napi_value exports)
         {"getParams", js_get_params}, {"reset", js_reset}, {"voxelize", js_voxelize},
         {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs},
         {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
-        {"getPops", js_get_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory}, {"setBoundaryMode", js_set_boundary_mode}, {"slice", js_slice}, {"sampleMany", js_sample_many}, {"meshParse", js_mesh_parse},
+        {"getPops", js_get_pops}, {"setPops", js_set_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory}, {"setBoundaryMode", js_set_boundary_mode}, {"slice", js_slice}, {"sampleMany", js_sample_many}, {"meshParse", js_mesh_parse},
         {"voxelizeParts", js_voxelize_parts}, {"getWalls", js_get_walls}, {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},
         {"adjointRelease", js_adjoint_release},
         {"voxelBounds", js_voxel_bounds}, {"voxelPlan", js_voxel_plan}, {"voxelBuild", js_voxel_build}};

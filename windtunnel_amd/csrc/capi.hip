@@ -577,6 +577,23 @@ int wt_get_fields(wt_lattice *h, void *rho, void *ux, void *uy, void *uz)
     return 0;
 }
 
+int wt_set_pops(wt_lattice *h, const void *in)
+{
+    WT_CHECK_H(h);
+    if (!in) return wt_fail(WT_E_ARG, "null populations");
+    const size_t cnt = 19 * (size_t)h->n;
+    const size_t eb = h->storage == WT_STORE_F64 ? 8 : 4;
+    void *d_in = nullptr;
+    if (int rc = alloc(&d_in, cnt * eb)) return rc;
+    int rc = 0;
+    if (hipMemcpyAsync(d_in, in, cnt * eb, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = wt_fail(WT_E_DEVICE, "population upload failed");
+    if (!rc) rc = lbm_set_pops(h, d_in);
+    if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "population write failed");
+    (void)hipFree(d_in);
+    return rc;
+}
+
 int wt_get_pops(wt_lattice *h, int which, void *out)
 {
     WT_CHECK_H(h);

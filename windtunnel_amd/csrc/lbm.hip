@@ -826,6 +826,24 @@ __global__ void __launch_bounds__(256) k_apply_ops(InitArgs<S, MAC> a)
 // Device buffer -> the reference's f[q*n + cell] order. AB: the buffer's
 // value of every cell. Esoteric: the current representation; values a cell
 // never stores (solid cells; out-of-lattice slots of boundary cells) read 0.
+// The inverse: the reference's f[q*n + cell] order into the device buffer
+// (AB: the buffer `B`; in place: the current representation, where a solid
+// cell and a boundary cell's outward slots hold nothing and are skipped).
+template <class S, class IN>
+__global__ void __launch_bounds__(256) k_write_pops(S *B, const uint8_t *solid, const IN *in, int64_t n, int nx,
+                                                    int ny, int nz, int rep, int eso, Lay lay)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
+    for (int q = 0; q < Q; q++) {
+        const double v = (double)in[q * n + i];
+        if (!eso) Pop<S>::template st<double>(B, pix(q, i, lay), v);
+        else if (!solid[i] && slot_inside(q, x, y, z, nx, ny, nz))
+            Pop<S>::template st<double>(B, slot_of(q, i, rep, lay, nx, ny), v);
+    }
+}
+
 template <class S, class OUT>
 __global__ void __launch_bounds__(256) k_read_pops(const S *B, const uint8_t *solid, OUT *out, int64_t n, int nx,
                                                    int ny, int nz, int rep, int eso, Lay lay)
@@ -1197,6 +1215,25 @@ int lbm_flush_force(wt_lattice *h)
     h->pend_partial = nullptr;
     h->pend_out = nullptr;
     h->pend_count = 0;
+    WT_HIP(hipGetLastError());
+    return 0;
+}
+
+int lbm_set_pops(wt_lattice *h, const void *d_in)
+{
+    const bool eso = h->layout == WT_LAYOUT_ESOTERIC;
+    void *B = eso ? h->d_buf[0] : h->d_buf[h->parity];
+    const unsigned nb = (unsigned)((h->n + 255) / 256);
+    const Lay lay{h->ns, h->bsh};
+    if (h->storage == WT_STORE_F64)
+        hipLaunchKernelGGL((k_write_pops<double, double>), dim3(nb), dim3(256), 0, h->stream, (double *)B, h->d_solid,
+                           (const double *)d_in, h->n, h->nx, h->ny, h->nz, h->parity, (int)eso, lay);
+    else if (h->storage == WT_STORE_F16)
+        hipLaunchKernelGGL((k_write_pops<__half, float>), dim3(nb), dim3(256), 0, h->stream, (__half *)B, h->d_solid,
+                           (const float *)d_in, h->n, h->nx, h->ny, h->nz, h->parity, (int)eso, lay);
+    else
+        hipLaunchKernelGGL((k_write_pops<float, float>), dim3(nb), dim3(256), 0, h->stream, (float *)B, h->d_solid,
+                           (const float *)d_in, h->n, h->nx, h->ny, h->nz, h->parity, (int)eso, lay);
     WT_HIP(hipGetLastError());
     return 0;
 }

@@ -157,6 +157,7 @@ int lbm_reset(wt_lattice *h);
 // apply the voxelize ops of cells [c0, c1) (and clear them in d_ops)
 int lbm_apply_voxelize(wt_lattice *h, uint8_t *d_ops, int64_t c0, int64_t c1);
 int lbm_rebuild_masks(wt_lattice *h);
+int lbm_set_pops(wt_lattice *h, const void *d_in);
 // masks of planes z0..z1 only (and the near-wall list of the whole lattice)
 int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1);
 int lbm_channel(wt_lattice *h, bool macro);

@@ -90,6 +90,7 @@ SIGNATURES = {
     "wt_device_memory": (_I, [_P, _P]),
     "wt_set_boundary_mode": (_I, [_P, _I]),
     "wt_slice": (_I, [_P, _I, _I, _I, _P, _P]),
+    "wt_set_pops": (_I, [_P, _P]),
     "wt_mesh_parse": (_I, [C.c_char_p, _P, C.c_int64, _P, _P]),
     "wt_sample": (_I, [_P, _P, C.c_int64, _P]),
     "wt_set_timing": (_I, [_P, _I]),
@@ -281,6 +282,16 @@ class LBM:
         out = np.empty(19 * self.n, dtype=dt)
         _check(lib().wt_get_pops(self._h, int(which), _ptr(out)))
         return out
+
+    def set_pops(self, f) -> None:
+        """Current populations from deviations in the f[q*n + cell] order
+        (wt_set_pops): float, or double for f64 storage."""
+        dt = np.float64 if self.storage == "f64" else np.float32
+        a = np.ascontiguousarray(f, dtype=dt).reshape(-1)
+        if a.size != 19 * self.n:
+            raise ValueError(f"populations must hold 19 x {self.n} values")
+        _check(lib().wt_set_pops(self._h, _ptr(a)))
+        self._fields = None
 
     f = property(lambda s: s.pops(0))
     g = property(lambda s: s.pops(1))
