@@ -98,8 +98,9 @@ template <class S, class T> __device__ __forceinline__ void st_at(S *base, uint3
 
 // Binary16 stores of collision results round stochastically: the scaled
 // value's float32 bit pattern gets 13 random low bits added to its
-// magnitude (the bits binary16 drops), which are then cleared, so a value
-// rounds up with probability equal to its distance from the value below.
+// magnitude (the bits binary16 drops), and the conversion then rounds
+// toward zero, so a value rounds up with probability equal to its distance
+// from the value below.
 // Round-to-nearest loses every update smaller than half a binary16 step
 // and biases slowly relaxing steady flows (C_d 1.3 % off binary64 in a
 // steady LES wake at 128x64x64, against 1e-5 with this rounding). The bits
@@ -125,14 +126,36 @@ template <class S> __device__ __forceinline__ uint32_t cell_bits(int64_t i, uint
 }
 // the 13 rounding bits of direction q
 __device__ __forceinline__ uint32_t dir_bits(uint32_t r, int q) { return (r + (uint32_t)q * 0x9E3779B9u) >> 19; }
+// Two values already in binary16 units (x F16_SCALE) to binary16 with one
+// conversion instruction: stochastically rounded with bits ra, rb (round
+// toward zero after the random bits are added), or to nearest.
+typedef __fp16 wt_half2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ wt_half2 f16_pair(float a, float b, uint32_t ra, uint32_t rb)
+{
+    if constexpr (!WT_F16_NEAREST) {
+        a = __uint_as_float(__float_as_uint(a) + ra);
+        b = __uint_as_float(__float_as_uint(b) + rb);
+        return __builtin_amdgcn_cvt_pkrtz(a, b);
+    } else {
+        return wt_half2{(__fp16)a, (__fp16)b};
+    }
+}
+// binary16 store of one half of a pair (the high half is stored straight
+// from the pair's register)
+__device__ __forceinline__ void st_h16(__half *p, __fp16 v)
+{
+#ifdef WT_NT_STORE
+    __builtin_nontemporal_store(v, (__fp16 *)p);
+#else
+    *(__fp16 *)p = v;
+#endif
+}
+
 // store of a collision result at element k (ptr + k), rounding bits rq
 template <class S, class T> __device__ __forceinline__ void st_col(S *p, int64_t k, T v, uint32_t rq)
 {
     if constexpr (rounds_stochastically<S>()) {
-        // with the 13 dropped bits cleared the value is exact in binary16
-        // (above its subnormals), so the plain conversion truncates
-        const uint32_t u = (__float_as_uint((float)v * F16_SCALE) + rq) & 0xFFFFE000u;
-        st_pop(p, k, __float2half_rn(__uint_as_float(u)));
+        st_h16(p + k, f16_pair((float)v * F16_SCALE, 0.0f, rq, 0).x);
     } else {
         Pop<S>::template st<T>(p, k, v);
     }
@@ -297,26 +320,32 @@ __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq1
 // 3 W r cu, products are fused, and 1/x and sqrt use the hardware's
 // single-precision approximations (about 1 ulp). Same model, same results to
 // float rounding; the binary64 path above keeps the reference's order.
-template <bool LES, bool TRT>
+// SCL: f and g are in binary16 units (deviations x F16_SCALE, a power of
+// two), so binary16 loads and stores need no scaling; every product and sum
+// is the unscaled one times the scale, exactly. r and u are unscaled.
+template <bool LES, bool TRT, bool SCL = false>
 __device__ __forceinline__ void collide_fast(const float (&f)[Q], float tau0, float omega0, float sq18cs2, float lam,
                                              float (&g)[Q], float &r, float &vx, float &vy, float &vz)
 {
+    constexpr float SC = SCL ? F16_SCALE : 1.0f;
     float s[9], d[9];
 #pragma unroll
     for (int p = 0; p < 9; p++) { s[p] = f[2 * p + 1] + f[2 * p + 2]; d[p] = f[2 * p + 1] - f[2 * p + 2]; }
     const float dr = f[0] + (((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])) + s[8]);
-    r = 1.0f + dr;
+    if constexpr (SCL) r = __builtin_fmaf(dr, F16_INV_SCALE, 1.0f);
+    else r = 1.0f + dr;
     const float inv = __builtin_amdgcn_rcpf(r);
+    const float invs = SCL ? inv * F16_INV_SCALE : inv;  // 1/r for scaled sums
     // pair directions: x, y, z, x+y, x-y, x+z, x-z, y+z, y-z
-    vx = (d[0] + d[3] + d[4] + d[5] + d[6]) * inv;
-    vy = (d[1] + d[3] - d[4] + d[7] + d[8]) * inv;
-    vz = (d[2] + d[5] - d[6] + d[7] - d[8]) * inv;
+    vx = (d[0] + d[3] + d[4] + d[5] + d[6]) * invs;
+    vy = (d[1] + d[3] - d[4] + d[7] + d[8]) * invs;
+    vz = (d[2] + d[5] - d[6] + d[7] - d[8]) * invs;
     const float usq = __builtin_fmaf(vx, vx, __builtin_fmaf(vy, vy, vz * vz));
-    const float A = __builtin_fmaf(-1.5f * r, usq, dr);
-    const float r45 = 4.5f * r;
+    const float A = __builtin_fmaf((-1.5f * SC) * r, usq, dr);
+    const float r45 = (4.5f * SC) * r;
     const float cu[9] = {vx, vy, vz, vx + vy, vx - vy, vx + vz, vx - vz, vy + vz, vy - vz};
     const float WF = 1.0f / 18, WE = 1.0f / 36;
-    const float oF = 3.0f * WF * r, oE = 3.0f * WE * r;
+    const float oF = ((3.0f * WF) * SC) * r, oE = ((3.0f * WE) * SC) * r;
     float sp[9], sm[9];  // (n_a + n_b)/2 and (n_a - n_b)/2 of each pair
 #pragma unroll
     for (int p = 0; p < 9; p++) {
@@ -336,7 +365,7 @@ __device__ __forceinline__ void collide_fast(const float (&f)[Q], float tau0, fl
         const float q2 = __builtin_fmaf(pxx, pxx, __builtin_fmaf(pyy, pyy, pzz * pzz)) +
                          2.0f * __builtin_fmaf(pxy, pxy, __builtin_fmaf(pxz, pxz, pyz * pyz));
         const float mag = __builtin_amdgcn_sqrtf(8.0f * q2);
-        tauEff = 0.5f * (tau0 + __builtin_amdgcn_sqrtf(__builtin_fmaf(tau0, tau0, sq18cs2 * mag * inv)));
+        tauEff = 0.5f * (tau0 + __builtin_amdgcn_sqrtf(__builtin_fmaf(tau0, tau0, sq18cs2 * mag * invs)));
         om = __builtin_amdgcn_rcpf(tauEff);
     }
     const float omm = TRT ? __builtin_amdgcn_rcpf(__builtin_fmaf(lam, __builtin_amdgcn_rcpf(tauEff - 0.5f), 0.5f)) : om;
@@ -527,8 +556,11 @@ template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO,
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES))) k_step(StepArgs<S, MAC> a)
 {
     using P = Pop<S>;
+    // binary16 storage with the float collision: the straight path works in
+    // binary16 units (no scaling at loads and stores)
+    constexpr bool HALF_UNITS = std::is_same<S, __half>::value && std::is_same<T, float>::value && WT_FAST_COLLIDE;
     // Four waves per SIMD at most: claiming v127 gives every wave a
-    // 128-register file, so a second 1024-thread block does not fit a CU.
+    // 128-register file, so at most four waves fit a SIMD.
     if constexpr (WAVES <= 4) asm volatile("" ::: "v127");
     if (a.prev_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 64)
         reduce_partials(a.prev_partial, a.prev_count, a.prev_out);
@@ -581,25 +613,52 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
             };
             const uint32_t o[3] = {lane(x - 1), lane(x), lane(x + 1)};
             T f[Q];
-#pragma unroll
-            for (int q = 0; q < Q; q++) f[q] = ld_at<S, T>(a.lp[q] + rb, o[1 + ld_shift<ESO>(q)]);
             T g[Q], r, vx, vy, vz;
-            if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
-                collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
-                                       vx, vy, vz);
-            else
-                collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
-            const uint32_t rc = cell_bits<S>(i, a.seed);
+            if constexpr (HALF_UNITS) {
+                // binary16 storage: collide in binary16 units, no scaling
 #pragma unroll
-            for (int q = 0; q < Q; q++) st_col_at<S, T>(a.sp[q] + rb, o[1 + st_shift<ESO>(q)], g[q], dir_bits(rc, q));
+                for (int q = 0; q < Q; q++)
+                    f[q] = __half2float(ld_pop((const __half *)((const char *)(a.lp[q] + rb) + o[1 + ld_shift<ESO>(q)]), 0));
+                collide_fast<LES, TRT, true>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r, vx, vy, vz);
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; q++) f[q] = ld_at<S, T>(a.lp[q] + rb, o[1 + ld_shift<ESO>(q)]);
+                if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
+                    collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
+                                           vx, vy, vz);
+                else
+                    collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+            }
+            const uint32_t rc = cell_bits<S>(i, a.seed);
+            // the post-collision values of direction q go to lane offset os(q, xo)
+            // of the row, for xo = x (home) or x + 1 (outlet copy)
+            auto store_all = [&](const uint32_t (&os)[3]) {
+                if constexpr (HALF_UNITS) {
+                    auto at = [&](int q) { return (__half *)((char *)(a.sp[q] + rb) + os[1 + st_shift<ESO>(q)]); };
+                    st_h16(at(0), f16_pair(g[0], 0.0f, dir_bits(rc, 0), 0).x);
+#pragma unroll
+                    for (int q = 1; q < Q; q += 2) {
+                        const wt_half2 h = f16_pair(g[q], g[q + 1], dir_bits(rc, q), dir_bits(rc, q + 1));
+                        st_h16(at(q), h.x);
+                        st_h16(at(q + 1), h.y);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < Q; q++) st_col_at<S, T>(a.sp[q] + rb, os[1 + st_shift<ESO>(q)], g[q], dir_bits(rc, q));
+                }
+            };
+            store_all(o);
             if (out && !(ESO && out_sol)) {
                 // zero-gradient outlet: the same values again for x = nx-1
-                const uint32_t o2 = lane(x + 2);
+                const uint32_t o2[3] = {o[1], o[2], lane(x + 2)};
+                if constexpr (HALF_UNITS) {
+                    // convert again here (one wave in eight), so the home
+                    // stores above take the high halves straight from the
+                    // packed registers
 #pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    const int s = 1 + st_shift<ESO>(q);
-                    st_col_at<S, T>(a.sp[q] + rb, s == 0 ? o[1] : (s == 1 ? o[2] : o2), g[q], dir_bits(rc, q));
+                    for (int q = 0; q < Q; q++) asm volatile("" : "+v"(g[q]));
                 }
+                store_all(o2);
             }
             if (MACRO) {
                 a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
