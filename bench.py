@@ -3,6 +3,7 @@ roofline, fp32 against binary16-offset storage, plus the adjoint/forward cost.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2]
                     [--no-cpu] [--no-f16] [--no-precision] [--no-adjoint]
+                    [--no-micro] [--no-c1] [--no-c2] [--no-scale]
                     [--fd-cells 64]
 
 Throughput workload (default): D3Q19 TRT (Lambda = 3/16) with Smagorinsky LES
@@ -33,10 +34,17 @@ Adjoint leg: 192x96x96 TRT (tau 0.6, u0 0.04) around a gray-walled ellipsoid
 checkpointing, objective J = mean C_d over the last 500 steps; cost ratio =
 (recompute + adjoint) device time per step / forward device time per step;
 central differences (eps 1e-3, binary64) on seeded gray cells.
+
+C2 leg: the 256x128x128 TRT+LES wake of the same body (tau 0.505, u0 0.08)
+placed as the app places it, 5000 steps in binary64 arithmetic (equal to the
+reference's recorded run step for step), float32 and binary16 storage, with
+the app's time-averaged C_d/C_l statistics.
 """
 from __future__ import annotations
 
 import argparse
+import base64
+import gzip
 import json
 import os
 import re
@@ -419,7 +427,8 @@ def run_micro(steps=100):
     r = np.random.default_rng(7).random(19 * n, dtype=np.float32)
     r *= 2e-3
     r -= 1e-3
-    out = {"grid": "512x256x256", "body": None, "populations": "uniform +-1e-3 deviations, seed 7", "steps": steps}
+    out = {"grid": "512x256x256", "body": None, "populations": "uniform +-1e-3 deviations, numpy default_rng(7)",
+           "steps": steps}
     for storage, bpc in (("f32", 152), ("f16", 76)):
         L = LBM(*dims, storage=storage)
         set_flow(L, WORKLOADS["c3"])
@@ -461,6 +470,72 @@ def run_c1(steps=2000):
         out[mode] = dict(cd=float(np.mean(F[steps // 2:, 0]) / q), mlups=round(L.n * steps / wall / 1e6, 1),
                          finite=bool(np.all(np.isfinite(F))))
         L.close()
+    return out
+
+
+# ---------------------------------------------------------------------- c2
+C2_FIXTURE = os.path.join(ROOT, "tests", "golden", "fixtures", "lbm_c2.json")
+
+
+def _fixture_array(rec: dict) -> np.ndarray:
+    raw = gzip.decompress(base64.b64decode(rec["gz_b64"]))
+    return np.frombuffer(raw, dtype={"f4": np.float32, "f8": np.float64}[rec["dtype"]]).reshape(rec["shape"])
+
+
+def run_c2(steps=5000):
+    """The 256x128x128 TRT (Lambda 3/16) + LES (Cs 0.1) wake at tau 0.505,
+    u0 0.08, of the ~20k-triangle body placed as the app places it
+    (Shapes.orient to unit length, Voxel.build at res 200, the grid-backed
+    Body.makeTest): 5000 steps, the app's time-averaged C_d/C_l (settle 150,
+    pair averages, 16-block SEM). The reference's own 5000-step run under
+    Node (its force series, recorded with the golden vectors) is the
+    comparison: binary64 arithmetic reproduces it bit for bit; the float
+    paths are compared by their statistics (the wake is chaotic)."""
+    from windtunnel_amd.lbm import LBM, Voxel
+    from windtunnel_amd.shapes import orient
+    from windtunnel_amd.stats import coefficient_stats
+    from windtunnel_amd.synth import ellipsoid_box
+    with open(C2_FIXTURE) as fh:
+        fx = json.load(fh)
+    ref = _fixture_array(fx["forces"])[:steps]
+    p = fx["params"]
+    P = ellipsoid_box()
+    orient(P, 0, 1, True, False)
+    t0 = time.perf_counter()
+    vox = Voxel.build(P, fx["mesh"]["addModel"]["res"])
+    vox_s = time.perf_counter() - t0
+    b, bi = fx["voxelize"]["body"], fx["voxelize"]["box_in"]
+
+    def stats(F, area):
+        s = coefficient_stats(F, p["u0"], area)
+        return None if s is None else dict(cd=s["cd"]["mean"], cd_sem=s["cd"]["sem"], cl=s["cl"]["mean"],
+                                           cl_sem=s["cl"]["sem"], samples=s["n"], state=s["state"])
+
+    out = {"grid": "256x128x128", "steps": steps, "tau": p["tau"], "u0": p["u0"], "Lambda": p["lambda"],
+           "Cs": p["csm"], "voxelize_s": round(vox_s, 3), "reference_node_ms_per_step": fx["node_ms_per_step"]}
+    area = None
+    for storage, arith in (("f32", "f64"), ("f32", "f32"), ("f16", "f32")):
+        L = LBM(*fx["dims"], storage=storage, arith=arith)
+        L.set_params(tau=p["tau"], les=p["les"], csm=p["csm"], trt=p["trt"], u0=p["u0"], **{"lambda": p["lambda"]})
+        L.reset()
+        L.voxelize_grid(vox, b["M"], b["pos"], b["scale"],
+                        dict(x0=bi[0], y0=bi[1], z0=bi[2], x1=bi[3], y1=bi[4], z1=bi[5]))
+        area = L.frontalArea
+        t0 = time.perf_counter()
+        F = L.step(steps)
+        wall = time.perf_counter() - t0
+        fin = np.isfinite(F).all(axis=1)
+        first_bad = None if fin.all() else int(np.argmin(fin)) + 1
+        r = dict(mlups=round(L.n * steps / wall / 1e6, 1), finite=first_bad is None)
+        if first_bad is None:
+            r["stats"] = stats(F, area)
+        else:
+            r["first_nonfinite_step"] = first_bad
+        if arith == "f64":
+            r["max_force_dev_vs_reference"] = float(np.abs(F - ref).max())
+        out[f"{storage}_store_{arith}_arith"] = r
+        L.close()
+    out["reference"] = stats(ref, area)
     return out
 
 
@@ -552,6 +627,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-micro", action="store_true")
     ap.add_argument("--fd-cells", type=int, default=64, help="central-difference cells of the adjoint leg (0: none)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -616,6 +692,10 @@ def main():
         log("c1 96x48x48 BGK sphere, reference and channel boundaries, 2000 steps")
         out["c1"] = run_c1()
         log(f"c1: {json.dumps(out['c1'])}")
+    if not a.no_c2:
+        log("c2 256x128x128 TRT+LES wake of the mesh body, 5000 steps: binary64, float32 and binary16 paths")
+        out["c2"] = run_c2()
+        log(f"c2: {json.dumps(out['c2'])}")
     if not a.no_scale:
         log("scale 640x320x320 f16, level-7 leaky icosphere, 3000 steps + adjoint")
         out["scale"] = run_scale()
