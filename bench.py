@@ -3,7 +3,7 @@ roofline, fp32 against binary16-offset storage, plus the adjoint/forward cost.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2]
                     [--no-cpu] [--no-f16] [--no-precision] [--no-adjoint]
-                    [--no-micro] [--no-c1] [--no-c2] [--no-scale]
+                    [--no-exact] [--no-micro] [--no-c1] [--no-c2] [--no-scale]
                     [--fd-cells 64]
 
 Throughput workload (default): D3Q19 TRT (Lambda = 3/16) with Smagorinsky LES
@@ -14,7 +14,8 @@ unioned with a 30x16x10 box, ~20k jittered triangles), voxelized on the GPU
 fused stream/collide/bounce-back/force launch over the whole lattice plus the
 per-step force reduction; the populations are resident in HBM before timing
 starts. The headline value is the float32 storage path; the binary16-offset
-storage path runs on the same lattice and is reported beside it.
+storage path runs on the same lattice and is reported beside it, and so is
+float32 storage with binary64 arithmetic (the reference's own numbers).
 
 Roofline: HBM bound. Algorithmic bytes per cell update = 19 populations read
 + 19 written = 152 B (float32) or 76 B (binary16); MLUPS counts all nx*ny*nz
@@ -105,9 +106,9 @@ def set_flow(L, wl) -> None:
 
 
 # -------------------------------------------------------------- throughput
-def run_gpu(wl, storage, steps, warmup):
+def run_gpu(wl, storage, steps, warmup, arith="f32"):
     from windtunnel_amd.lbm import LBM
-    L = LBM(*wl["dims"], storage=storage)
+    L = LBM(*wl["dims"], storage=storage, arith=arith)
     set_flow(L, wl)
     L.reset()
     vinfo = place_mesh_body(L, wl["scale"])
@@ -623,6 +624,7 @@ def main():
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-scale", action="store_true")
@@ -677,6 +679,14 @@ def main():
                                     "unit": "GB/s", "frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                     "traffic": traffic.get("f16"), "kernel_ms": round(r16["kernel_ms"], 5),
                                     "bytes_per_cell": 76, "bytes_per_launch": nx * ny * nz * 76}}
+    if not a.no_exact:
+        log("f32 storage, binary64 arithmetic")
+        rx = run_gpu(wl, "f32", a.steps, a.warmup, arith="f64")
+        log(f"exact {rx['mlups']:.0f} MLUPS kernel {rx['kernel_ms']:.4f} ms")
+        out["exact"] = {"what": "float32 storage, binary64 collision arithmetic in the reference's expression order "
+                                "(bit for bit the reference's populations)",
+                        "value": round(rx["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(rx["ms_per_step"], 4),
+                        "kernel_ms": round(rx["kernel_ms"], 5), "roofline_frac": round(rx["achieved_gbs"] / HBM_PEAK_GBS, 4)}
     if not a.no_micro:
         log("micro 512x256x256, no body, random populations")
         out["micro"] = run_micro()
