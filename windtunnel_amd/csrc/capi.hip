@@ -73,8 +73,9 @@ void release(wt_lattice *h)
     for (void *p : {(void *)h->d_buf[0], (void *)h->d_solid, (void *)h->d_sbits,
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
                     (void *)h->d_forces, (void *)h->d_near, (void *)h->d_ring,
-                    (void *)h->d_prod, (void *)h->d_ops})
+                    (void *)h->d_prod, (void *)h->d_ops, h->d_stage, (void *)h->d_vgrid})
         if (p) (void)hipFree(p);
+    if (h->h_stage) (void)hipHostFree(h->h_stage);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -87,6 +88,38 @@ int upload_body(wt_lattice *h)
     WT_HIP(hipMemcpyAsync(h->d_uw, h->h_uw.data(), sizeof(float) * 3 * (size_t)h->n, hipMemcpyHostToDevice,
                           h->stream));
     return lbm_rebuild_masks(h);
+}
+
+// Pinned host staging and its device twin of at least `bytes` each.
+static int stage_reserve(wt_lattice *h, size_t bytes)
+{
+    if (bytes <= h->stage_cap) return 0;
+    if (h->h_stage) (void)hipHostFree(h->h_stage);
+    if (h->d_stage) (void)hipFree(h->d_stage);
+    h->h_stage = nullptr;
+    h->d_stage = nullptr;
+    h->stage_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, (size_t)1 << 20);
+    if (hipHostMalloc((void **)&h->h_stage, cap, hipHostMallocDefault) != hipSuccess) {
+        h->h_stage = nullptr;
+        return wt_fail(WT_E_NOMEM, "hipHostMalloc of %zu bytes of voxelize staging failed", cap);
+    }
+    if (int rc = alloc(&h->d_stage, cap)) return rc;
+    h->stage_cap = cap;
+    return 0;
+}
+
+// The cells of box b (x0, y0, z0, x1, y1, z1, inclusive) of a lattice-shaped
+// host array (elem bytes per cell), packed in x, y, z order into out.
+static void gather_box(const wt_lattice *h, const void *src, int elem, const int *b, uint8_t *out)
+{
+    const size_t row = (size_t)(b[3] - b[0] + 1) * (size_t)elem;
+    for (int z = b[2]; z <= b[5]; z++)
+        for (int y = b[1]; y <= b[4]; y++) {
+            const int64_t i = b[0] + h->nx * ((int64_t)y + (int64_t)h->ny * z);
+            std::memcpy(out, (const uint8_t *)src + (size_t)i * (size_t)elem, row);
+            out += row;
+        }
 }
 
 // Worker slab split of js/lbmWorker.js:43-48.
@@ -267,13 +300,17 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     uint8_t *ops = h->h_ops.data();
     uint8_t *solid = h->h_solid.data();
     float *uwx = h->h_uw.data(), *uwy = uwx + n, *uwz = uwx + 2 * n;
-    // planes whose solid state or ops change, and planes whose wall velocities change
-    int zlo = nz, zhi = -1, ulo = nz, uhi = -1;
-    auto touch = [](int &lo, int &hi, int z) { lo = std::min(lo, z); hi = std::max(hi, z); };
-    auto set_uw = [&](int64_t i, int z, float x, float y, float w) {
-        if (uwx[i] != x || uwy[i] != y || uwz[i] != w) {
-            uwx[i] = x; uwy[i] = y; uwz[i] = w;
-            touch(ulo, uhi, z);
+    // boxes (x0, y0, z0, x1, y1, z1) of the cells whose solid state or ops
+    // change, and of the cells whose wall velocities change
+    int cb[6] = {nx, ny, nz, -1, -1, -1}, ub[6] = {nx, ny, nz, -1, -1, -1};
+    auto touch = [](int *b, int x, int y, int z) {
+        b[0] = std::min(b[0], x); b[1] = std::min(b[1], y); b[2] = std::min(b[2], z);
+        b[3] = std::max(b[3], x); b[4] = std::max(b[4], y); b[5] = std::max(b[5], z);
+    };
+    auto set_uw = [&](int64_t i, int x, int y, int z, float vx, float vy, float vz) {
+        if (uwx[i] != vx || uwy[i] != vy || uwz[i] != vz) {
+            uwx[i] = vx; uwy[i] = vy; uwz[i] = vz;
+            touch(ub, x, y, z);
         }
     };
     // js/lbm.js:194-202: the old body becomes fluid at u0 (all of it lies in
@@ -288,8 +325,8 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
                     const int64_t i = x + nx * ((int64_t)y + (int64_t)ny * z);
                     if (solid[i] != wt::SOLID_OBJ) continue;
                     solid[i] = wt::FLUID; ops[i] |= 1;
-                    set_uw(i, z, 0, 0, 0);
-                    touch(zlo, zhi, z);
+                    set_uw(i, x, y, z, 0, 0, 0);
+                    touch(cb, x, y, z);
                 }
     }
     // js/lbm.js:204-214: the road row
@@ -302,9 +339,9 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
             if (solid[i] != gy) {
                 solid[i] = (uint8_t)gy;
                 if (gy == wt::FLUID) ops[i] |= 1;
-                touch(zlo, zhi, z);
+                touch(cb, x, 0, z);
             }
-            set_uw(i, z, gy == wt::SOLID_WALL ? (float)rv : 0.0f, 0, 0);
+            set_uw(i, x, 0, z, gy == wt::SOLID_WALL ? (float)rv : 0.0f, 0, 0);
         }
     }
     // js/lbm.js:216-246: the new body, its projections and lowest cells
@@ -320,14 +357,14 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
                     if (!mask[k]) continue;
                     solid[i] = wt::SOLID_OBJ;
                     ops[i] |= 2;
-                    touch(zlo, zhi, z);
+                    touch(cb, x, y, z);
                     float wx = 0, wy = 0, wz = 0;
                     if (mask[k] == 2) {
                         wx = wv[3 * kw]; wy = wv[3 * kw + 1]; wz = wv[3 * kw + 2];
                         kw++;
                         mov = 1;
                     }
-                    set_uw(i, z, wx, wy, wz);
+                    set_uw(i, x, y, z, wx, wy, wz);
                     col[y + (size_t)ny * z] = 1;
                     const size_t pk = x + (size_t)nx * z;
                     plan[pk] = 1;
@@ -352,31 +389,45 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     h->body.rideHeight = gaps.empty() ? 0 : gaps[gaps.size() >> 1];
     h->p.movingWalls = mov;
 
-    // device: the changed planes only
+    // device: the changed boxes only, gathered into pinned staging, uploaded
+    // at once and scattered (the ops kernel then runs over their planes)
     int rc = 0;
     if (!h->d_ops) {
         rc = alloc((void **)&h->d_ops, (size_t)n);
         if (!rc && hipMemset(h->d_ops, 0, (size_t)n) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "hipMemset failed");
     }
-    if (!rc && zhi >= zlo) {
-        const int64_t c0 = plane * zlo, c1 = plane * (zhi + 1);
-        if (hipMemcpyAsync(h->d_ops + c0, ops + c0, (size_t)(c1 - c0), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-            hipMemcpyAsync(h->d_solid + c0, solid + c0, (size_t)(c1 - c0), hipMemcpyHostToDevice, h->stream) !=
-                hipSuccess)
+    const bool cdirty = cb[5] >= cb[2], udirty = ub[5] >= ub[2];
+    auto box_cells = [](const int *b) {
+        return (size_t)(b[3] - b[0] + 1) * (size_t)(b[4] - b[1] + 1) * (size_t)(b[5] - b[2] + 1);
+    };
+    const size_t cn = cdirty ? box_cells(cb) : 0, un = udirty ? box_cells(ub) : 0;
+    const size_t ubase = (2 * cn + 15) / 16 * 16;  // wall velocities after solid and ops
+    if (!rc) rc = stage_reserve(h, ubase + 3 * sizeof(float) * un);
+    if (!rc && cdirty) {
+        gather_box(h, solid, 1, cb, h->h_stage);
+        gather_box(h, ops, 1, cb, h->h_stage + cn);
+        if (hipMemcpyAsync(h->d_stage, h->h_stage, 2 * cn, hipMemcpyHostToDevice, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "voxelize upload failed");
-        if (!rc) rc = lbm_apply_voxelize(h, h->d_ops, c0, c1);
+        if (!rc) rc = lbm_scatter_box(h, h->d_stage, h->d_solid, 1, cb);
+        if (!rc) rc = lbm_scatter_box(h, (uint8_t *)h->d_stage + cn, h->d_ops, 1, cb);
+        if (!rc) rc = lbm_apply_voxelize(h, h->d_ops, plane * cb[2], plane * (cb[5] + 1));
     }
-    if (!rc && uhi >= ulo) {
-        const int64_t c0 = plane * ulo, c1 = plane * (uhi + 1);
+    if (!rc && udirty) {
+        for (int c = 0; c < 3; c++) gather_box(h, h->h_uw.data() + c * n, 4, ub, h->h_stage + ubase + c * 4 * un);
+        if (hipMemcpyAsync((uint8_t *)h->d_stage + ubase, h->h_stage + ubase, 12 * un, hipMemcpyHostToDevice,
+                           h->stream) != hipSuccess)
+            rc = wt_fail(WT_E_DEVICE, "voxelize upload failed");
         for (int c = 0; c < 3 && !rc; c++)
-            if (hipMemcpyAsync(h->d_uw + c * n + c0, h->h_uw.data() + c * n + c0, sizeof(float) * (size_t)(c1 - c0),
-                               hipMemcpyHostToDevice, h->stream) != hipSuccess)
-                rc = wt_fail(WT_E_DEVICE, "voxelize upload failed");
+            rc = lbm_scatter_box(h, (uint8_t *)h->d_stage + ubase + c * 4 * un, h->d_uw + c * n, 4, ub);
     }
     // the near-wall mask of a plane depends on the planes beside it
-    if (!rc && zhi >= zlo) rc = lbm_rebuild_masks_range(h, std::max(0, zlo - 1), std::min(nz - 1, zhi + 1));
+    if (!rc && cdirty) rc = lbm_rebuild_masks_range(h, std::max(0, cb[2] - 1), std::min(nz - 1, cb[5] + 1));
     if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "voxelize failed");
-    if (zhi >= zlo) std::fill(ops + plane * zlo, ops + plane * (zhi + 1), 0);
+    if (cdirty)  // the host ops are kept zero between calls
+        for (int z = cb[2]; z <= cb[5]; z++)
+            for (int y = cb[1]; y <= cb[4]; y++)
+                std::fill(ops + cb[0] + nx * ((int64_t)y + (int64_t)ny * z),
+                          ops + cb[3] + 1 + nx * ((int64_t)y + (int64_t)ny * z), 0);
     if (rc) return rc;
     if (out) {
         out->frontalArea = h->body.frontalArea; out->planformArea = h->body.planformArea;
@@ -398,10 +449,16 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
                         std::max(0, box[5] - box[2] + 1);
     std::vector<uint8_t> mask((size_t)cnt);
     if (cnt > 0) {
-        const size_t gbytes = (size_t)vm->nx * vm->ny * vm->nz;
-        uint8_t *d_grid = nullptr, *d_mask = nullptr;
-        if (int rc = alloc((void **)&d_grid, gbytes)) return rc;
-        if (int rc = alloc((void **)&d_mask, (size_t)cnt)) { (void)hipFree(d_grid); return rc; }
+        // grid and mask share one device buffer, kept with the lattice
+        const size_t gbytes = (size_t)vm->nx * vm->ny * vm->nz, moff = (gbytes + 255) / 256 * 256;
+        if (moff + (size_t)cnt > h->vgrid_cap) {
+            if (h->d_vgrid) (void)hipFree(h->d_vgrid);
+            h->d_vgrid = nullptr;
+            h->vgrid_cap = 0;
+            if (int rc = alloc((void **)&h->d_vgrid, moff + (size_t)cnt)) return rc;
+            h->vgrid_cap = moff + (size_t)cnt;
+        }
+        uint8_t *d_grid = h->d_vgrid, *d_mask = h->d_vgrid + moff;
         int rc = 0;
         if (hipMemcpyAsync(d_grid, grid, gbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "grid upload failed");
@@ -409,7 +466,6 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
         if (!rc && hipMemcpyAsync(mask.data(), d_mask, (size_t)cnt, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "mask download failed");
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "grid lookup failed");
-        (void)hipFree(d_grid); (void)hipFree(d_mask);
         if (rc) return rc;
     }
     return wt_voxelize(h, box, mask.data(), nullptr, ground, roadVel, out);

@@ -925,25 +925,33 @@ __global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, const float
                                                uint32_t *nbits, int nx, int ny, int nz, int nwx, int64_t w0,
                                                int64_t w1)
 {
-    const int64_t w = w0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (w >= w1) return;
-    const int wx = (int)(w % nwx);
-    const int64_t row = w / nwx;
+    // one thread per cell of words [w0, w1) (rows padded to whole words);
+    // each half wavefront is one word, assembled by a ballot
+    const int64_t t = w0 * 32 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t w = t >> 5;
+    const int rw = nwx * 32;  // cells of a padded row
+    const int64_t row = t / rw;
+    const int x = (int)(t - row * rw);
     const int y = (int)(row % ny), z = (int)(row / ny);
-    uint32_t sb = 0, nb = 0;
-    for (int b = 0; b < 32; b++) {
-        const int x = wx * 32 + b;
-        if (x >= nx) break;
+    bool s = false, near = false;
+    if (w < w1 && x < nx) {
         const int64_t i = x + (int64_t)nx * (y + (int64_t)ny * z);
-        if (solid[i]) { sb |= 1u << b; continue; }
-        if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2 || z < 1 || z > nz - 2) continue;
-        for (int q = 1; q < Q; q++) {
-            const int64_t j = i - cell_off(q, nx, ny);
-            if (solid[j] || (wgt && wgt[j] > 0.0f)) { nb |= 1u << b; break; }
+        s = solid[i] != 0;
+        if (!s && x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && z >= 1 && z <= nz - 2) {
+#pragma unroll
+            for (int q = 1; q < Q; q++) {
+                const int64_t j = i - cell_off(q, nx, ny);
+                near |= solid[j] != 0 || (wgt && wgt[j] > 0.0f);
+            }
         }
     }
-    sbits[w] = sb;
-    nbits[w] = nb;
+    const uint64_t sm = __ballot(s), nm = __ballot(near);
+    const int lane = threadIdx.x & 63;
+    if ((lane & 31) == 0 && w < w1) {
+        const int sh = lane;  // 0 or 32
+        sbits[w] = (uint32_t)(sm >> sh);
+        nbits[w] = (uint32_t)(nm >> sh);
+    }
 }
 
 // ECMAScript ToInt32 of a double (the `|0` operator).
@@ -1188,46 +1196,104 @@ int lbm_rebuild_masks(wt_lattice *h) { return lbm_rebuild_masks_range(h, 0, h->n
 
 int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1)
 {
-    const int64_t nwords = (int64_t)h->nwx * h->ny * h->nz;
-    const int64_t w0 = (int64_t)h->nwx * h->ny * z0, w1 = (int64_t)h->nwx * h->ny * (z1 + 1);
+    // without a valid list, the whole lattice
+    if (h->near_full || h->near_z.size() != (size_t)h->nz + 1) { z0 = 0; z1 = h->nz - 1; }
+    const int64_t pw = (int64_t)h->nwx * h->ny;  // words per plane
+    const int64_t w0 = pw * z0, w1 = pw * (z1 + 1);
     if (w1 > w0) {
-        hipLaunchKernelGGL(k_masks, dim3((unsigned)((w1 - w0 + 255) / 256)), dim3(256), 0, h->stream, h->d_solid,
-                           h->d_wgt, h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx, w0, w1);
+        hipLaunchKernelGGL(k_masks, dim3((unsigned)(((w1 - w0) * 32 + 255) / 256)), dim3(256), 0, h->stream,
+                           h->d_solid, h->d_wgt, h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx, w0, w1);
         WT_HIP(hipGetLastError());
     }
-    // list of the near-wall cells in cell order, and where each z plane's start
-    std::vector<uint32_t> bits((size_t)nwords);
-    WT_HIP(hipMemcpyAsync(bits.data(), h->d_nbits, sizeof(uint32_t) * (size_t)nwords, hipMemcpyDeviceToHost,
-                          h->stream));
-    WT_HIP(hipStreamSynchronize(h->stream));
-    std::vector<uint32_t> list;
-    h->near_z.assign((size_t)h->nz + 1, 0);
-    for (int z = 0; z < h->nz; z++) {
-        h->near_z[(size_t)z] = (int64_t)list.size();
+    // near-wall cells of planes z0..z1 in cell order
+    std::vector<uint32_t> bits((size_t)std::max<int64_t>(0, w1 - w0));
+    if (w1 > w0) {
+        WT_HIP(hipMemcpyAsync(bits.data(), h->d_nbits + w0, sizeof(uint32_t) * (size_t)(w1 - w0), hipMemcpyDeviceToHost,
+                              h->stream));
+        WT_HIP(hipStreamSynchronize(h->stream));
+    }
+    std::vector<uint32_t> mid;
+    std::vector<int64_t> mid_z;  // start of plane z0 + k within mid
+    for (int z = z0; z <= z1; z++) {
+        mid_z.push_back((int64_t)mid.size());
+        const uint32_t *pb = bits.data() + (size_t)(z - z0) * (size_t)pw;
         for (int y = 0; y < h->ny; y++)
             for (int wx = 0; wx < h->nwx; wx++) {
-                uint32_t w = bits[(size_t)wx + (size_t)h->nwx * (y + (size_t)h->ny * z)];
+                uint32_t w = pb[(size_t)wx + (size_t)h->nwx * y];
                 while (w) {
                     const int b = __builtin_ctz(w);
                     w &= w - 1;
-                    list.push_back((uint32_t)(wx * 32 + b + (int64_t)h->nx * (y + (int64_t)h->ny * z)));
+                    mid.push_back((uint32_t)(wx * 32 + b + (int64_t)h->nx * (y + (int64_t)h->ny * z)));
                 }
             }
     }
-    h->near_z[(size_t)h->nz] = (int64_t)list.size();
-    if ((int64_t)list.size() > h->near_cap) {
+    // splice: the planes before z0 and after z1 keep their cells
+    std::vector<int64_t> oz = h->near_z;
+    if (oz.size() != (size_t)h->nz + 1) oz.assign((size_t)h->nz + 1, 0);
+    const int64_t head = oz[(size_t)z0], tail0 = oz[(size_t)z1 + 1], told = oz[(size_t)h->nz];
+    std::vector<uint32_t> list;
+    list.reserve((size_t)(head + (int64_t)mid.size() + (told - tail0)));
+    list.insert(list.end(), h->h_near.begin(), h->h_near.begin() + head);
+    list.insert(list.end(), mid.begin(), mid.end());
+    list.insert(list.end(), h->h_near.begin() + tail0, h->h_near.begin() + told);
+    const int64_t shift = head + (int64_t)mid.size() - tail0;
+    h->near_z.assign((size_t)h->nz + 1, 0);
+    for (int z = 0; z <= h->nz; z++) {
+        if (z < z0) h->near_z[(size_t)z] = oz[(size_t)z];
+        else if (z <= z1) h->near_z[(size_t)z] = head + mid_z[(size_t)(z - z0)];
+        else h->near_z[(size_t)z] = oz[(size_t)z] + shift;
+    }
+    h->h_near.swap(list);
+    h->near_full = true;  // until the device copy below is in place
+    const int64_t cnt = (int64_t)h->h_near.size();
+    if (cnt > h->near_cap) {
         if (h->d_near) WT_HIP(hipFree(h->d_near));
         h->d_near = nullptr;
         h->near_cap = 0;
-        if (hipMalloc((void **)&h->d_near, sizeof(uint32_t) * list.size()) != hipSuccess) {
+        if (hipMalloc((void **)&h->d_near, sizeof(uint32_t) * (size_t)cnt) != hipSuccess) {
             h->d_near = nullptr;
-            h->near_z.assign((size_t)h->nz + 1, 0);
-            return wt_fail(WT_E_NOMEM, "hipMalloc of the near-wall list (%zu cells) failed", list.size());
+            h->near_z.assign((size_t)h->nz + 1, 0);  // no near-wall cells until a full rebuild
+            h->h_near.clear();
+            h->near_full = true;
+            return wt_fail(WT_E_NOMEM, "hipMalloc of the near-wall list (%lld cells) failed", (long long)cnt);
         }
-        h->near_cap = (int64_t)list.size();
+        h->near_cap = cnt;
     }
-    if (!list.empty())
-        WT_HIP(hipMemcpy(h->d_near, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice));
+    if (cnt > 0)
+        WT_HIP(hipMemcpy(h->d_near, h->h_near.data(), sizeof(uint32_t) * (size_t)cnt, hipMemcpyHostToDevice));
+    h->near_full = false;
+    return 0;
+}
+
+template <class E>
+__global__ void __launch_bounds__(256) k_scatter_box(const E *src, E *dst, int bx0, int by0, int bz0, int w, int hgt,
+                                                     int64_t cnt, int nx, int ny)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int x = (int)(t % w);
+    const int64_t r = t / w;
+    const int y = (int)(r % hgt), z = (int)(r / hgt);
+    dst[(bx0 + x) + (int64_t)nx * ((by0 + y) + (int64_t)ny * (bz0 + z))] = src[t];
+}
+
+int lbm_scatter_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6])
+{
+    const int w = box[3] - box[0] + 1, hgt = box[4] - box[1] + 1, d = box[5] - box[2] + 1;
+    if (w <= 0 || hgt <= 0 || d <= 0) return 0;
+    if (box[0] < 0 || box[1] < 0 || box[2] < 0 || box[3] >= h->nx || box[4] >= h->ny || box[5] >= h->nz)
+        return wt_fail(WT_E_ARG, "scatter box outside the lattice");
+    const int64_t cnt = (int64_t)w * hgt * d;
+    const dim3 grid((unsigned)((cnt + 255) / 256));
+    if (elem == 1)
+        hipLaunchKernelGGL(k_scatter_box<uint8_t>, grid, dim3(256), 0, h->stream, (const uint8_t *)src, (uint8_t *)dst,
+                           box[0], box[1], box[2], w, hgt, cnt, h->nx, h->ny);
+    else if (elem == 4)
+        hipLaunchKernelGGL(k_scatter_box<uint32_t>, grid, dim3(256), 0, h->stream, (const uint32_t *)src,
+                           (uint32_t *)dst, box[0], box[1], box[2], w, hgt, cnt, h->nx, h->ny);
+    else
+        return wt_fail(WT_E_ARG, "scatter element of %d bytes", elem);
+    WT_HIP(hipGetLastError());
     return 0;
 }
 

@@ -141,6 +141,19 @@ struct wt_lattice {
     int obj_box[6] = {1, 1, 1, 0, 0, 0};
     std::vector<uint8_t> h_ops;
     uint8_t *d_ops = nullptr;
+    // voxelize uploads: the changed box of the mirrors gathered into pinned
+    // host memory, uploaded at once and scattered on the device; both grown
+    // on demand
+    uint8_t *h_stage = nullptr;
+    void *d_stage = nullptr;
+    size_t stage_cap = 0;
+    // device copy of a voxel grid for the grid-backed body lookup, grown on demand
+    uint8_t *d_vgrid = nullptr;
+    size_t vgrid_cap = 0;
+    // host copy of the near-wall list (d_near), so a plane range is rebuilt
+    // and spliced in; near_full: the next rebuild covers every plane
+    std::vector<uint32_t> h_near;
+    bool near_full = true;
 };
 
 // error reporting (capi.hip)
@@ -158,8 +171,13 @@ int lbm_reset(wt_lattice *h);
 int lbm_apply_voxelize(wt_lattice *h, uint8_t *d_ops, int64_t c0, int64_t c1);
 int lbm_rebuild_masks(wt_lattice *h);
 int lbm_set_pops(wt_lattice *h, const void *d_in);
-// masks of planes z0..z1 only (and the near-wall list of the whole lattice)
+// masks of planes z0..z1 only, and the near-wall list of those planes
+// spliced into the rest of it
 int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1);
+// box[0..5] = x0, y0, z0, x1, y1, z1 (inclusive): the box's cells packed in
+// x, y, z order at src (elem bytes each, elem 1 or 4) written into the
+// lattice-shaped array dst
+int lbm_scatter_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6]);
 int lbm_channel(wt_lattice *h, bool macro);
 // Launches one step over z rows [bz0,bz1). Its force (when force_out is
 // given) is reduced into force_out by the NEXT step launch or by
