@@ -292,8 +292,7 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     if (!empty && !mask) return wt_fail(WT_E_ARG, "null mask for a non-empty box");
     if (!empty && !wv) {
         const int64_t cnt = (int64_t)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
-        for (int64_t k = 0; k < cnt; k++)
-            if (mask[k] == 2) return wt_fail(WT_E_ARG, "mask holds wall-velocity cells but wv is null");
+        if (std::memchr(mask, 2, (size_t)cnt)) return wt_fail(WT_E_ARG, "mask holds wall-velocity cells but wv is null");
     }
     const int64_t n = h->n, plane = (int64_t)nx * ny;
     if (h->h_ops.empty()) h->h_ops.assign((size_t)n, 0);
@@ -307,8 +306,14 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
         b[0] = std::min(b[0], x); b[1] = std::min(b[1], y); b[2] = std::min(b[2], z);
         b[3] = std::max(b[3], x); b[4] = std::max(b[4], y); b[5] = std::max(b[5], z);
     };
+    // h->uw_cells counts the cells with a nonzero wall velocity, so a zero
+    // velocity needs no look at the mirror while there are none
     auto set_uw = [&](int64_t i, int x, int y, int z, float vx, float vy, float vz) {
+        const bool nz_new = vx != 0 || vy != 0 || vz != 0;
+        if (!nz_new && h->uw_cells == 0) return;
         if (uwx[i] != vx || uwy[i] != vy || uwz[i] != vz) {
+            const bool nz_old = uwx[i] != 0 || uwy[i] != 0 || uwz[i] != 0;
+            h->uw_cells += (int64_t)nz_new - (int64_t)nz_old;
             uwx[i] = vx; uwy[i] = vy; uwz[i] = vz;
             touch(ub, x, y, z);
         }
@@ -320,14 +325,18 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     {
         const int *ob = h->obj_box;
         for (int z = ob[2]; z <= ob[5]; z++)
-            for (int y = ob[1]; y <= ob[4]; y++)
+            for (int y = ob[1]; y <= ob[4]; y++) {
+                const int64_t r = nx * ((int64_t)y + (int64_t)ny * z);
+                uint8_t *srow = solid + r, *orow = ops + r;
+                int xa = nx, xb = -1;  // first and last cleared cell of the row
                 for (int x = ob[0]; x <= ob[3]; x++) {
-                    const int64_t i = x + nx * ((int64_t)y + (int64_t)ny * z);
-                    if (solid[i] != wt::SOLID_OBJ) continue;
-                    solid[i] = wt::FLUID; ops[i] |= 1;
-                    set_uw(i, x, y, z, 0, 0, 0);
-                    touch(cb, x, y, z);
+                    if (srow[x] != wt::SOLID_OBJ) continue;
+                    srow[x] = wt::FLUID; orow[x] |= 1;
+                    set_uw(r + x, x, y, z, 0, 0, 0);
+                    xa = std::min(xa, x); xb = x;
                 }
+                if (xb >= 0) { touch(cb, xa, y, z); touch(cb, xb, y, z); }
+            }
     }
     // js/lbm.js:204-214: the road row
     const int gy = ground ? wt::SOLID_WALL : wt::FLUID;
@@ -350,29 +359,38 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     int64_t cells = 0, k = 0, kw = 0;
     int nb[6] = {nx, ny, nz, -1, -1, -1};  // box of the new body's cells
     if (!empty) {
+        const int w = x1 - x0 + 1;
         for (int z = z0; z <= z1; z++)
-            for (int y = y0; y <= y1; y++) {
-                int64_t i = x0 + (int64_t)nx * (y + (int64_t)ny * z);
-                for (int x = x0; x <= x1; x++, i++, k++) {
-                    if (!mask[k]) continue;
+            for (int y = y0; y <= y1; y++, k += w) {
+                const uint8_t *mrow = mask + k;
+                int xa = w, xb = -1;  // first and last body cell of the row (box coordinates)
+                for (int t = 0; t < w; t++)
+                    if (mrow[t]) { xa = t; break; }
+                if (xa == w) continue;
+                for (int t = w - 1; t >= xa; t--)
+                    if (mrow[t]) { xb = t; break; }
+                const int64_t r = (int64_t)nx * (y + (int64_t)ny * z);
+                for (int t = xa; t <= xb; t++) {
+                    if (!mrow[t]) continue;
+                    const int x = x0 + t;
+                    const int64_t i = r + x;
                     solid[i] = wt::SOLID_OBJ;
                     ops[i] |= 2;
-                    touch(cb, x, y, z);
                     float wx = 0, wy = 0, wz = 0;
-                    if (mask[k] == 2) {
+                    if (mrow[t] == 2) {
                         wx = wv[3 * kw]; wy = wv[3 * kw + 1]; wz = wv[3 * kw + 2];
                         kw++;
                         mov = 1;
                     }
                     set_uw(i, x, y, z, wx, wy, wz);
-                    col[y + (size_t)ny * z] = 1;
                     const size_t pk = x + (size_t)nx * z;
                     plan[pk] = 1;
                     if (y < low[pk]) low[pk] = (int16_t)y;
                     cells++;
-                    nb[0] = std::min(nb[0], x); nb[1] = std::min(nb[1], y); nb[2] = std::min(nb[2], z);
-                    nb[3] = std::max(nb[3], x); nb[4] = std::max(nb[4], y); nb[5] = std::max(nb[5], z);
                 }
+                col[y + (size_t)ny * z] = 1;
+                touch(cb, x0 + xa, y, z); touch(cb, x0 + xb, y, z);
+                touch(nb, x0 + xa, y, z); touch(nb, x0 + xb, y, z);
             }
     }
     std::copy(nb, nb + 6, h->obj_box);

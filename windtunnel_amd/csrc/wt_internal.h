@@ -136,6 +136,7 @@ struct wt_lattice {
 
     std::vector<uint8_t> h_solid;  // host mirror of the solid array
     std::vector<float> h_uw;       // host mirror of the wall velocities (3n)
+    int64_t uw_cells = 0;          // cells of h_uw with a nonzero velocity
     // voxelize: the last body's box (every SOLID_OBJ cell lies in it; x0 > x1:
     // none), and the per-cell ops on the host and device, kept zero between calls
     int obj_box[6] = {1, 1, 1, 0, 0, 0};
