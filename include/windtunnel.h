@@ -42,8 +42,10 @@ enum {
 };
 
 /* Population storage. F32 is the reference's Float32Array of deviations
- * f_q - w_q (js/lbm.js:74-75,145-162); F16 stores the same deviations as
- * scaled binary16; F64 stores binary64 (the FP64 oracle run). */
+ * f_q - w_q (js/lbm.js:74-75,145-162); F16 stores the same deviations times
+ * 2^15 as binary16 (|f_q - w_q| up to 65504 * 2^-15, about 2.0: collision
+ * results round stochastically and saturate at that bound); F64 stores
+ * binary64 (the FP64 oracle run). */
 enum { WT_STORE_F32 = 0, WT_STORE_F16 = 1, WT_STORE_F64 = 2 };
 
 /* Collision arithmetic. F64 evaluates the reference's binary64 expression
