@@ -331,8 +331,9 @@ def run_cpu(wl, threads):
     """The oracle (C restatement of js/lbm.js with the worker-pool slab split
     of js/lbmWorker.js) on the host cores: 2 warm-up steps, then best of 3 x 2
     steps on the full lattice, the reference benchmark's best-of protocol
-    (js/main.js:614-620) shortened to bound the run time. The body is the
-    same mesh body: its solid map is taken from the GPU lattice."""
+    (js/main.js:614-620) shortened to bound the run time; then one thread on
+    the same lattice. The body is the same mesh body: its solid map is taken
+    from the GPU lattice. Returns (MLUPS at `threads`, MLUPS at 1 thread)."""
     from oracle.oracle import OracleLBM
     dims = wl["dims"]
     box, mask = _cpu_mask(dims, wl["scale"])
@@ -346,9 +347,16 @@ def run_cpu(wl, threads):
         t0 = time.perf_counter()
         O.step_pool(2, threads, threaded=True)
         best = min(best, (time.perf_counter() - t0) / 2)
+    # one thread on the same lattice: best of 2 x 1 step after 1 warm-up step
+    O.step_pool(1, 1, threaded=False)
+    best1 = float("inf")
+    for _ in range(2):
+        t0 = time.perf_counter()
+        O.step_pool(1, 1, threaded=False)
+        best1 = min(best1, time.perf_counter() - t0)
     n = O.n
     del O
-    return n / best / 1e6
+    return n / best / 1e6, n / best1 / 1e6
 
 
 def host_cpus() -> dict:
@@ -726,9 +734,12 @@ def main():
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline, {threads} threads")
-        v = run_cpu(wl, threads)
+        v, v1 = run_cpu(wl, threads)
         log("cpu detail: 1 and all threads, 96x48x48 and 256x128x128")
         out["cpu_detail"] = run_cpu_detail(threads)
+        out["cpu_detail"][f"c3_{nx}x{ny}x{nz}_trt_les_mlups"] = {
+            "1_threads": round(v1, 2), f"{threads}_threads": round(v, 2),
+            "protocol": "2 warm-up steps, best of 3 x 2 steps; 1 thread: 1 warm-up step, best of 2 x 1 step"}
         out["cpu_baseline"] = {"value": round(v, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
                                "sample": f"oracle (C restatement of js/lbm.js, binary64 arithmetic, float32 "
                                          f"storage) on the full {nx}x{ny}x{nz} lattice with the same body, "
