@@ -73,7 +73,8 @@ void release(wt_lattice *h)
     for (void *p : {(void *)h->d_buf[0], (void *)h->d_solid, (void *)h->d_sbits,
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
                     (void *)h->d_forces, (void *)h->d_near, (void *)h->d_ring,
-                    (void *)h->d_prod, (void *)h->d_ops, h->d_stage, (void *)h->d_vgrid})
+                    (void *)h->d_prod, (void *)h->d_ops, h->d_stage, (void *)h->d_vgrid,
+                    (void *)h->d_near2, (void *)h->d_pcnt})
         if (p) (void)hipFree(p);
     if (h->h_stage) (void)hipHostFree(h->h_stage);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);

@@ -954,6 +954,59 @@ __global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, const float
     }
 }
 
+// Near-wall cells of each plane z0 + blockIdx.x (one block per plane).
+__global__ void __launch_bounds__(256) k_near_count(const uint32_t *nbits, int64_t pw, int z0, int *cnt)
+{
+    const uint32_t *pb = nbits + (int64_t)(z0 + (int)blockIdx.x) * pw;
+    int s = 0;
+    for (int64_t w = threadIdx.x; w < pw; w += blockDim.x) s += __popc(pb[w]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    __shared__ int part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// The near-wall list of planes z0 + blockIdx.x (one block per plane), in
+// cell order: plane p starts at head + cnt[0] + ... + cnt[p-1]; each thread
+// takes a run of the plane's words, placed by a block scan of their counts.
+__global__ void __launch_bounds__(256) k_near_fill(const uint32_t *nbits, int64_t pw, int nwx, int nx, int ny, int z0,
+                                                   const int *cnt, int64_t head, uint32_t *out)
+{
+    const int p = (int)blockIdx.x, z = z0 + p, t = (int)threadIdx.x;
+    const uint32_t *pb = nbits + (int64_t)z * pw;
+    __shared__ int64_t base;
+    __shared__ int sc[256];
+    if (t == 0) {
+        int64_t b = head;
+        for (int k = 0; k < p; k++) b += cnt[k];
+        base = b;
+    }
+    const int64_t per = (pw + 255) / 256;
+    const int64_t wa = std::min<int64_t>(pw, t * per), wb = std::min<int64_t>(pw, wa + per);
+    int s = 0;
+    for (int64_t w = wa; w < wb; w++) s += __popc(pb[w]);
+    sc[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = t >= off ? sc[t - off] : 0;
+        __syncthreads();
+        sc[t] += v;
+        __syncthreads();
+    }
+    int64_t o = base + sc[t] - s;
+    for (int64_t w = wa; w < wb; w++) {
+        uint32_t bits = pb[w];
+        const int wx = (int)(w % nwx), y = (int)(w / nwx);
+        while (bits) {
+            const int b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            out[o++] = (uint32_t)(wx * 32 + b + (int64_t)nx * (y + (int64_t)ny * z));
+        }
+    }
+}
+
 // ECMAScript ToInt32 of a double (the `|0` operator).
 __device__ __forceinline__ int js_toint32(double d)
 {
@@ -1200,67 +1253,69 @@ int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1)
     if (h->near_full || h->near_z.size() != (size_t)h->nz + 1) { z0 = 0; z1 = h->nz - 1; }
     const int64_t pw = (int64_t)h->nwx * h->ny;  // words per plane
     const int64_t w0 = pw * z0, w1 = pw * (z1 + 1);
-    if (w1 > w0) {
-        hipLaunchKernelGGL(k_masks, dim3((unsigned)(((w1 - w0) * 32 + 255) / 256)), dim3(256), 0, h->stream,
-                           h->d_solid, h->d_wgt, h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx, w0, w1);
-        WT_HIP(hipGetLastError());
+    const int P = z1 - z0 + 1;
+    if (P <= 0) return 0;
+    hipLaunchKernelGGL(k_masks, dim3((unsigned)(((w1 - w0) * 32 + 255) / 256)), dim3(256), 0, h->stream,
+                       h->d_solid, h->d_wgt, h->d_sbits, h->d_nbits, h->nx, h->ny, h->nz, h->nwx, w0, w1);
+    WT_HIP(hipGetLastError());
+    // near-wall cells per plane of the range
+    if (!h->d_pcnt && hipMalloc((void **)&h->d_pcnt, sizeof(int) * (size_t)h->nz) != hipSuccess) {
+        h->d_pcnt = nullptr;
+        return wt_fail(WT_E_NOMEM, "hipMalloc of the near-wall plane counts failed");
     }
-    // near-wall cells of planes z0..z1 in cell order
-    std::vector<uint32_t> bits((size_t)std::max<int64_t>(0, w1 - w0));
-    if (w1 > w0) {
-        WT_HIP(hipMemcpyAsync(bits.data(), h->d_nbits + w0, sizeof(uint32_t) * (size_t)(w1 - w0), hipMemcpyDeviceToHost,
-                              h->stream));
-        WT_HIP(hipStreamSynchronize(h->stream));
-    }
-    std::vector<uint32_t> mid;
-    std::vector<int64_t> mid_z;  // start of plane z0 + k within mid
-    for (int z = z0; z <= z1; z++) {
-        mid_z.push_back((int64_t)mid.size());
-        const uint32_t *pb = bits.data() + (size_t)(z - z0) * (size_t)pw;
-        for (int y = 0; y < h->ny; y++)
-            for (int wx = 0; wx < h->nwx; wx++) {
-                uint32_t w = pb[(size_t)wx + (size_t)h->nwx * y];
-                while (w) {
-                    const int b = __builtin_ctz(w);
-                    w &= w - 1;
-                    mid.push_back((uint32_t)(wx * 32 + b + (int64_t)h->nx * (y + (int64_t)h->ny * z)));
-                }
-            }
-    }
-    // splice: the planes before z0 and after z1 keep their cells
+    hipLaunchKernelGGL(k_near_count, dim3((unsigned)P), dim3(256), 0, h->stream, (const uint32_t *)h->d_nbits, pw, z0,
+                       h->d_pcnt);
+    WT_HIP(hipGetLastError());
+    std::vector<int> cnt((size_t)P);
+    WT_HIP(hipMemcpyAsync(cnt.data(), h->d_pcnt, sizeof(int) * (size_t)P, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipStreamSynchronize(h->stream));
+    // new layout: the planes before z0 and after z1 keep their cells
     std::vector<int64_t> oz = h->near_z;
     if (oz.size() != (size_t)h->nz + 1) oz.assign((size_t)h->nz + 1, 0);
     const int64_t head = oz[(size_t)z0], tail0 = oz[(size_t)z1 + 1], told = oz[(size_t)h->nz];
-    std::vector<uint32_t> list;
-    list.reserve((size_t)(head + (int64_t)mid.size() + (told - tail0)));
-    list.insert(list.end(), h->h_near.begin(), h->h_near.begin() + head);
-    list.insert(list.end(), mid.begin(), mid.end());
-    list.insert(list.end(), h->h_near.begin() + tail0, h->h_near.begin() + told);
-    const int64_t shift = head + (int64_t)mid.size() - tail0;
+    int64_t mid = 0;
+    for (int c : cnt) mid += c;
+    const int64_t total = head + mid + (told - tail0);
+    h->near_full = true;  // until the new list is in place
+    uint32_t *src = h->d_near, *dst = h->d_near2;
+    uint32_t *grown[2] = {nullptr, nullptr};
+    if (total > h->near_cap) {
+        const int64_t cap = total + total / 4 + 1024;
+        for (auto &g : grown)
+            if (hipMalloc((void **)&g, sizeof(uint32_t) * (size_t)cap) != hipSuccess) {
+                if (grown[0]) (void)hipFree(grown[0]);
+                if (grown[1]) (void)hipFree(grown[1]);
+                return wt_fail(WT_E_NOMEM, "hipMalloc of the near-wall list (%lld cells) failed", (long long)cap);
+            }
+        dst = grown[0];
+        h->near_cap = cap;
+    }
+    if (head > 0)
+        WT_HIP(hipMemcpyAsync(dst, src, sizeof(uint32_t) * (size_t)head, hipMemcpyDeviceToDevice, h->stream));
+    if (mid > 0) {
+        hipLaunchKernelGGL(k_near_fill, dim3((unsigned)P), dim3(256), 0, h->stream, (const uint32_t *)h->d_nbits, pw,
+                           h->nwx, h->nx, h->ny, z0, (const int *)h->d_pcnt, head, dst);
+        WT_HIP(hipGetLastError());
+    }
+    if (told > tail0)
+        WT_HIP(hipMemcpyAsync(dst + head + mid, src + tail0, sizeof(uint32_t) * (size_t)(told - tail0),
+                              hipMemcpyDeviceToDevice, h->stream));
+    if (grown[0]) {
+        WT_HIP(hipStreamSynchronize(h->stream));
+        if (h->d_near) WT_HIP(hipFree(h->d_near));
+        if (h->d_near2) WT_HIP(hipFree(h->d_near2));
+        h->d_near = grown[0];
+        h->d_near2 = grown[1];
+    } else {
+        std::swap(h->d_near, h->d_near2);
+    }
     h->near_z.assign((size_t)h->nz + 1, 0);
+    int64_t acc = head;
     for (int z = 0; z <= h->nz; z++) {
         if (z < z0) h->near_z[(size_t)z] = oz[(size_t)z];
-        else if (z <= z1) h->near_z[(size_t)z] = head + mid_z[(size_t)(z - z0)];
-        else h->near_z[(size_t)z] = oz[(size_t)z] + shift;
+        else if (z <= z1) { h->near_z[(size_t)z] = acc; acc += cnt[(size_t)(z - z0)]; }
+        else h->near_z[(size_t)z] = oz[(size_t)z] + (head + mid - tail0);
     }
-    h->h_near.swap(list);
-    h->near_full = true;  // until the device copy below is in place
-    const int64_t cnt = (int64_t)h->h_near.size();
-    if (cnt > h->near_cap) {
-        if (h->d_near) WT_HIP(hipFree(h->d_near));
-        h->d_near = nullptr;
-        h->near_cap = 0;
-        if (hipMalloc((void **)&h->d_near, sizeof(uint32_t) * (size_t)cnt) != hipSuccess) {
-            h->d_near = nullptr;
-            h->near_z.assign((size_t)h->nz + 1, 0);  // no near-wall cells until a full rebuild
-            h->h_near.clear();
-            h->near_full = true;
-            return wt_fail(WT_E_NOMEM, "hipMalloc of the near-wall list (%lld cells) failed", (long long)cnt);
-        }
-        h->near_cap = cnt;
-    }
-    if (cnt > 0)
-        WT_HIP(hipMemcpy(h->d_near, h->h_near.data(), sizeof(uint32_t) * (size_t)cnt, hipMemcpyHostToDevice));
     h->near_full = false;
     return 0;
 }

@@ -151,9 +151,12 @@ struct wt_lattice {
     // device copy of a voxel grid for the grid-backed body lookup, grown on demand
     uint8_t *d_vgrid = nullptr;
     size_t vgrid_cap = 0;
-    // host copy of the near-wall list (d_near), so a plane range is rebuilt
-    // and spliced in; near_full: the next rebuild covers every plane
-    std::vector<uint32_t> h_near;
+    // a plane range of the near-wall list is rebuilt on the device into the
+    // second buffer (d_near2, same capacity), around the unchanged planes
+    // copied from d_near; the two are then swapped. d_pcnt: per-plane counts
+    // (nz). near_full: the next rebuild covers every plane
+    uint32_t *d_near2 = nullptr;
+    int *d_pcnt = nullptr;
     bool near_full = true;
 };
 
