@@ -3,7 +3,8 @@ roofline, fp32 against binary16-offset storage, plus the adjoint/forward cost.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2]
                     [--no-cpu] [--no-f16] [--no-precision] [--no-adjoint]
-                    [--no-exact] [--no-micro] [--no-c1] [--no-c2] [--no-scale]
+                    [--no-exact] [--no-micro] [--no-c1] [--no-c2] [--no-drag]
+                    [--no-scale]
                     [--fd-cells 64]
 
 Throughput workload (default): D3Q19 TRT (Lambda = 3/16) with Smagorinsky LES
@@ -483,6 +484,35 @@ def run_c1(steps=2000):
     return out
 
 
+# -------------------------------------------------------------------- drag
+def run_drag(moves=12):
+    """Dragging the body: the 512x256x256 lattice with the mesh body (x2,
+    grid-backed lookup), moved by one cell along x per re-voxelize, as the
+    app re-voxelizes on the next frame after a drag moves the body
+    (js/main.js:478-485,1387-1394); wall time of each voxelize call, median
+    over the moves after the first two."""
+    from windtunnel_amd.lbm import LBM, Voxel
+    wl = WORKLOADS["c3"]
+    vox = Voxel.build(body_mesh(), 200)
+    L = LBM(*wl["dims"])
+    set_flow(L, wl)
+    L.reset()
+    b = np.array(vox["bbox"])
+    ts = []
+    for k in range(moves):
+        pos = np.array([0.3 * L.nx + k, L.ny / 2, L.nz / 2])
+        lo, hi = pos + wl["scale"] * b[:3] - 2, pos + wl["scale"] * b[3:] + 2
+        t0 = time.perf_counter()
+        L.voxelize_grid(vox, np.eye(3).reshape(-1), pos, wl["scale"],
+                        dict(x0=lo[0], y0=lo[1], z0=lo[2], x1=hi[0], y1=hi[1], z1=hi[2]))
+        ts.append(time.perf_counter() - t0)
+    out = {"grid": "512x256x256", "moves": moves, "solid_cells": int(L.solidCells),
+           "voxelize_ms_median": round(1e3 * float(np.median(ts[2:])), 3),
+           "voxelize_ms_first": round(1e3 * ts[0], 3)}
+    L.close()
+    return out
+
+
 # ---------------------------------------------------------------------- c2
 C2_FIXTURE = os.path.join(ROOT, "tests", "golden", "fixtures", "lbm_c2.json")
 
@@ -651,6 +681,7 @@ def main():
     ap.add_argument("--no-scale", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-drag", action="store_true")
     ap.add_argument("--no-micro", action="store_true")
     ap.add_argument("--fd-cells", type=int, default=64, help="central-difference cells of the adjoint leg (0: none)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -727,6 +758,10 @@ def main():
         log("c2 256x128x128 TRT+LES wake of the mesh body, 5000 steps: binary64, float32 and binary16 paths")
         out["c2"] = run_c2()
         log(f"c2: {json.dumps(out['c2'])}")
+    if not a.no_drag:
+        log("drag: re-voxelize the 512x256x256 body moved by one cell")
+        out["drag"] = run_drag()
+        log(f"drag: {json.dumps(out['drag'])}")
     if not a.no_scale:
         log("scale 640x320x320 f16, level-7 leaky icosphere, 3000 steps + adjoint")
         out["scale"] = run_scale()
