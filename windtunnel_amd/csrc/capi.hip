@@ -329,6 +329,19 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
             for (int y = ob[1]; y <= ob[4]; y++) {
                 const int64_t r = nx * ((int64_t)y + (int64_t)ny * z);
                 uint8_t *srow = solid + r, *orow = ops + r;
+                if (h->uw_cells == 0) {
+                    // no wall velocity to reset: a branch-free pass over the row,
+                    // the changed box taken as the box's extent along x
+                    uint8_t any = 0;
+                    for (int x = ob[0]; x <= ob[3]; x++) {
+                        const uint8_t m = srow[x] == wt::SOLID_OBJ;
+                        srow[x] = m ? (uint8_t)wt::FLUID : srow[x];
+                        orow[x] |= m;
+                        any |= m;
+                    }
+                    if (any) { touch(cb, ob[0], y, z); touch(cb, ob[3], y, z); }
+                    continue;
+                }
                 int xa = nx, xb = -1;  // first and last cleared cell of the row
                 for (int x = ob[0]; x <= ob[3]; x++) {
                     if (srow[x] != wt::SOLID_OBJ) continue;
@@ -371,23 +384,40 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
                 for (int t = w - 1; t >= xa; t--)
                     if (mrow[t]) { xb = t; break; }
                 const int64_t r = (int64_t)nx * (y + (int64_t)ny * z);
-                for (int t = xa; t <= xb; t++) {
-                    if (!mrow[t]) continue;
-                    const int x = x0 + t;
-                    const int64_t i = r + x;
-                    solid[i] = wt::SOLID_OBJ;
-                    ops[i] |= 2;
-                    float wx = 0, wy = 0, wz = 0;
-                    if (mrow[t] == 2) {
-                        wx = wv[3 * kw]; wy = wv[3 * kw + 1]; wz = wv[3 * kw + 2];
-                        kw++;
-                        mov = 1;
+                if (h->uw_cells == 0 && !std::memchr(mrow + xa, 2, (size_t)(xb - xa + 1))) {
+                    // no wall velocities anywhere: a branch-free pass over the row
+                    uint8_t *srow = solid + r + x0, *orow = ops + r + x0, *prow = plan.data() + (size_t)nx * z + x0;
+                    int16_t *lrow = low.data() + (size_t)nx * z + x0;
+                    const int16_t yy = (int16_t)y;
+                    int64_t c = 0;
+                    for (int t = xa; t <= xb; t++) {
+                        const uint8_t v = mrow[t] != 0;
+                        srow[t] = v ? (uint8_t)wt::SOLID_OBJ : srow[t];
+                        orow[t] |= (uint8_t)(v << 1);
+                        prow[t] |= v;
+                        lrow[t] = (v && yy < lrow[t]) ? yy : lrow[t];
+                        c += v;
                     }
-                    set_uw(i, x, y, z, wx, wy, wz);
-                    const size_t pk = x + (size_t)nx * z;
-                    plan[pk] = 1;
-                    if (y < low[pk]) low[pk] = (int16_t)y;
-                    cells++;
+                    cells += c;
+                } else {
+                    for (int t = xa; t <= xb; t++) {
+                        if (!mrow[t]) continue;
+                        const int x = x0 + t;
+                        const int64_t i = r + x;
+                        solid[i] = wt::SOLID_OBJ;
+                        ops[i] |= 2;
+                        float wx = 0, wy = 0, wz = 0;
+                        if (mrow[t] == 2) {
+                            wx = wv[3 * kw]; wy = wv[3 * kw + 1]; wz = wv[3 * kw + 2];
+                            kw++;
+                            mov = 1;
+                        }
+                        set_uw(i, x, y, z, wx, wy, wz);
+                        const size_t pk = x + (size_t)nx * z;
+                        plan[pk] = 1;
+                        if (y < low[pk]) low[pk] = (int16_t)y;
+                        cells++;
+                    }
                 }
                 col[y + (size_t)ny * z] = 1;
                 touch(cb, x0 + xa, y, z); touch(cb, x0 + xb, y, z);
