@@ -566,7 +566,7 @@ void wto_grid_mask(const uint8_t *grid, const int gdim[3], const double gmin[3],
 
 /* ===================================================== gray bounce-back
  *
- * No reference counterpart (a north-star addition): a cell j with weight
+ * No reference counterpart (an addition of this library): a cell j with weight
  * w_j in (0,1) stays a fluid cell, but every link leaving it is bounced back
  * with weight w_j at the downstream cell n = j + c_q:
  *   f_q(n) = (1 - w_j) g_q(j) + w_j g_qbar(n),   F -= w_j * 2 g_qbar(n) c_q.

@@ -1,5 +1,5 @@
 // Discrete adjoint of the gray-bounce-back step (no reference counterpart:
-// a north-star addition) with sqrt(N) checkpointing.
+// an addition of this library) with sqrt(N) checkpointing.
 //
 // Forward step k maps the state s_k (post-collision populations, AB layout)
 // to s_{k+1} and yields the force F_k. The objective is J = sum_k Fbar_k.F_k.
