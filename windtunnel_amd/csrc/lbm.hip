@@ -404,9 +404,8 @@ __device__ __forceinline__ void reduce_partials(const double *p, int64_t count, 
 // Resident waves per SIMD of the step kernel, by storage type (measured on
 // MI355X at 512x256x256): binary16 moves half the bytes per load, so it
 // needs 8 waves (at most 64 VGPRs) in flight to cover HBM latency; 4- and
-// 8-byte storage run best with one 1024-thread block per CU (4 waves per
-// SIMD; two blocks per CU cost ~7% at fp32). Build switches WT_F16_WAVES /
-// WT_WAVES.
+// 8-byte storage run best with 4 waves per SIMD (twice as many cost ~7% at
+// fp32). Build switches WT_F16_WAVES / WT_WAVES.
 #ifndef WT_F16_WAVES
 #define WT_F16_WAVES 8
 #endif
