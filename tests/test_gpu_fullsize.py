@@ -1,15 +1,17 @@
 """Parity at the throughput workload's own size: the 512x256x256 TRT+LES
 lattice (tau 0.505, u0 0.08, Cs 0.1) around the ~20k-triangle mesh body
 scaled 2x, voxelized on the GPU. In binary64 arithmetic (float storage, the
-production 64-cell blocks) two steps must leave the populations, fields and
-solid/near maps equal to the oracle's (the C restatement of js/lbm.js) bit
-for bit, and the forces within 1e-11 (the device sums per-workgroup
-partials, the reference cell by cell). The float32 and binary16 paths must
-stay within 1e-5 and 2e-3 of the binary64 forces over the same steps."""
+production 64-cell blocks, two buffers and in place) two steps must leave
+the populations, fields and solid/near maps equal to the oracle's (the C
+restatement of js/lbm.js) bit for bit, and the forces within 1e-11 (the
+device sums per-workgroup partials, the reference cell by cell). The
+float32 and binary16 paths must stay within 1e-5 and 2e-3 of the binary64
+forces over the same steps."""
 import numpy as np
 import pytest
 
 from oracle.oracle import OracleLBM
+from test_gpu_lbm import stored_mask
 from windtunnel_amd.lbm import LBM, Voxel
 from windtunnel_amd.synth import ellipsoid_box
 
@@ -58,8 +60,18 @@ def test_full_size_two_steps():
     F = L.fields()
     for k in ("rho", "ux", "uy", "uz"):
         assert np.array_equal(F[k], O.array(k)), k
-    del O
     L.close()
+    print("[full] in place (esoteric pull), binary64 arithmetic", flush=True)
+    E = lattice(vox, arith="f64", layout="esoteric")
+    fe = E.step(2)
+    np.testing.assert_allclose(fe, fo, rtol=1e-11, atol=1e-14)
+    FE = E.fields()
+    for k in ("rho", "ux", "uy", "uz"):
+        assert np.array_equal(FE[k], O.array(k)), k
+    m = stored_mask(DIMS, O.array("solid"))
+    assert np.array_equal(E.f[m], O.array("f")[m])
+    E.close()
+    del O, m
     for storage, tol in (("f32", 1e-5), ("f16", 2e-3)):
         print(f"[full] {storage} storage, float arithmetic", flush=True)
         G = lattice(vox, storage=storage)
