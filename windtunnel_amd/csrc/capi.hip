@@ -74,7 +74,7 @@ void release(wt_lattice *h)
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
                     (void *)h->d_forces, (void *)h->d_near, (void *)h->d_ring,
                     (void *)h->d_prod, (void *)h->d_ops, h->d_stage, (void *)h->d_vgrid,
-                    (void *)h->d_near2, (void *)h->d_pcnt})
+                    (void *)h->d_near2, (void *)h->d_pcnt, h->d_rvwork})
         if (p) (void)hipFree(p);
     if (h->h_stage) (void)hipHostFree(h->h_stage);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
@@ -486,6 +486,90 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     return 0;
 }
 
+// wt_voxelize for a body already rasterised into a device mask over box,
+// when no wall velocity is set anywhere and the road stays as it is: the
+// diff against the previous body, the ops, the projections and the lowest
+// cells are computed on the device, and only a few scalars, a histogram and
+// the changed box of the solid map (for the host mirror) come back. The
+// result equals wt_voxelize's on the same mask.
+static int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_body_stats *out)
+{
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    const int x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
+    const bool empty = x1 < x0 || y1 < y0 || z1 < z0;
+    if (!empty && (x0 < 1 || y0 < 1 || z0 < 1 || x1 > nx - 2 || y1 > ny - 2 || z1 > nz - 2))
+        return wt_fail(WT_E_ARG, "box [%d,%d,%d]-[%d,%d,%d] is not within the interior", x0, y0, z0, x1, y1, z1);
+    const int64_t n = h->n, plane = (int64_t)nx * ny;
+    const int *ob = h->obj_box;
+    const bool oempty = ob[3] < ob[0] || ob[4] < ob[1] || ob[5] < ob[2];
+    int mbox[6] = {1, 1, 1, 0, 0, 0}, ub[6] = {1, 1, 1, 0, 0, 0};
+    if (!empty) std::copy(box, box + 6, mbox);
+    if (!empty && !oempty)
+        for (int k = 0; k < 3; k++) { ub[k] = std::min(box[k], ob[k]); ub[k + 3] = std::max(box[k + 3], ob[k + 3]); }
+    else if (!empty) std::copy(box, box + 6, ub);
+    else if (!oempty) std::copy(ob, ob + 6, ub);
+    if (!h->d_ops) {
+        if (int rc = alloc((void **)&h->d_ops, (size_t)n)) return rc;
+        WT_HIP(hipMemset(h->d_ops, 0, (size_t)n));
+    }
+    const size_t work = ((size_t)ny * nz + 255) / 256 * 256 + ((size_t)nx * nz + 255) / 256 * 256 +
+                        sizeof(int) * ((size_t)nx * nz + 16 + (size_t)ny);
+    if (work > h->rvwork_cap) {
+        if (h->d_rvwork) (void)hipFree(h->d_rvwork);
+        h->d_rvwork = nullptr;
+        h->rvwork_cap = 0;
+        if (int rc = alloc(&h->d_rvwork, work)) return rc;
+        h->rvwork_cap = work;
+    }
+    const int big = 0x7fffffff;
+    int res[16] = {0, big, big, big, -1, -1, -1, big, big, big, -1, -1, -1, 0, 0, 0};
+    std::vector<int> hist((size_t)ny, 0);
+    if (int rc = lbm_revoxelize_device(h, ub, mbox, d_mask, h->d_rvwork, res, hist.data())) return rc;
+    // js/lbm.js:240-246: projections and the median gap under the body
+    const int64_t cells = res[0];
+    h->body.frontalArea = std::max<int64_t>(1, res[13]);
+    h->body.planformArea = std::max<int64_t>(1, res[14]);
+    h->body.solidCells = cells;
+    int64_t total = 0;
+    for (int c : hist) total += c;
+    int ride = 0;
+    if (total > 0) {
+        const int64_t mid = total >> 1;
+        int64_t acc = 0;
+        for (int g = 0; g < ny; g++) {
+            acc += hist[(size_t)g];
+            if (acc > mid) { ride = g; break; }
+        }
+    }
+    h->body.rideHeight = ride;
+    h->p.movingWalls = 0;
+    if (cells > 0) std::copy(res + 7, res + 13, h->obj_box);
+    else { const int e[6] = {nx, ny, nz, -1, -1, -1}; std::copy(e, e + 6, h->obj_box); }
+    const int *cb = res + 1;
+    if (cb[5] >= cb[2]) {
+        if (int rc = lbm_apply_voxelize(h, h->d_ops, plane * cb[2], plane * (cb[5] + 1))) return rc;
+        if (int rc = lbm_rebuild_masks_range(h, std::max(0, cb[2] - 1), std::min(nz - 1, cb[5] + 1))) return rc;
+        // the host mirror of the solid map over the changed box
+        const size_t cn = (size_t)(cb[3] - cb[0] + 1) * (size_t)(cb[4] - cb[1] + 1) * (size_t)(cb[5] - cb[2] + 1);
+        if (int rc = stage_reserve(h, cn)) return rc;
+        if (int rc = lbm_gather_box(h, h->d_solid, h->d_stage, 1, cb)) return rc;
+        WT_HIP(hipMemcpyAsync(h->h_stage, h->d_stage, cn, hipMemcpyDeviceToHost, h->stream));
+        WT_HIP(hipStreamSynchronize(h->stream));
+        const size_t row = (size_t)(cb[3] - cb[0] + 1);
+        const uint8_t *src = h->h_stage;
+        for (int z = cb[2]; z <= cb[5]; z++)
+            for (int y = cb[1]; y <= cb[4]; y++, src += row)
+                std::memcpy(h->h_solid.data() + cb[0] + nx * ((int64_t)y + (int64_t)ny * z), src, row);
+    }
+    WT_HIP(hipStreamSynchronize(h->stream));
+    if (out) {
+        out->frontalArea = h->body.frontalArea; out->planformArea = h->body.planformArea;
+        out->solidCells = h->body.solidCells; out->rideHeight = h->body.rideHeight;
+        out->movingWalls = 0; out->ground = h->ground;
+    }
+    return 0;
+}
+
 int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, const double M[9],
                      const double pos[3], double scale, const int box[6], int ground, double roadVel,
                      wt_body_stats *out)
@@ -512,6 +596,13 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
         if (hipMemcpyAsync(d_grid, grid, gbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "grid upload failed");
         if (!rc) rc = lbm_grid_mask(h, d_grid, vm, M, pos, scale, box, d_mask);
+        // no wall velocity anywhere and the road as it is: the whole update on
+        // the device (env WT_VOXELIZE_HOST=1 keeps the host bookkeeping)
+        const double rv = (roadVel != roadVel || roadVel == 0) ? 0 : roadVel;
+        if (!rc && h->uw_cells == 0 && rv == 0 && (!!ground) == h->ground && !std::getenv("WT_VOXELIZE_HOST")) {
+            h->epoch++;
+            return voxelize_device(h, box, d_mask, out);
+        }
         if (!rc && hipMemcpyAsync(mask.data(), d_mask, (size_t)cnt, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "mask download failed");
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "grid lookup failed");

@@ -1331,6 +1331,179 @@ __global__ void __launch_bounds__(256) k_scatter_box(const E *src, E *dst, int b
     dst[(bx0 + x) + (int64_t)nx * ((by0 + y) + (int64_t)ny * (bz0 + z))] = src[t];
 }
 
+// Device re-voxelize of a body given as a device mask (no wall velocities,
+// no road change: the caller checks). One thread per cell of the union of
+// the previous body's box and the new mask's box: the previous body's cells
+// become fluid (op 1), the mask's cells body (op 2), as the host bookkeeping
+// does (js/lbm.js:194-246). The changed box, the new body's box and cell
+// count are reduced per workgroup into res; the projections go to col / plan
+// (bytes) and the lowest cell of each (x, z) column to low.
+struct RevoxArgs {
+    uint8_t *solid, *ops;
+    const uint8_t *mask;
+    int nx, ny, nz;
+    int ux, uy, uz, uw, uh, ud;       // union box origin and extents
+    int mx0, my0, mz0, mw, mh, md;    // mask box origin and extents (md 0: no mask)
+    int *res;  // 0 cells, 1-6 changed box, 7-12 new body box, 13 frontal, 14 planform
+    uint8_t *col, *plan;
+    int *low, *hist;
+};
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_add(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_revox(RevoxArgs a)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t cnt = (int64_t)a.uw * a.uh * a.ud;
+    const int big = 0x7fffffff;
+    int v[13] = {0, big, big, big, -1, -1, -1, big, big, big, -1, -1, -1};
+    if (t < cnt) {
+        const int x = a.ux + (int)(t % a.uw);
+        const int64_t r = t / a.uw;
+        const int y = a.uy + (int)(r % a.uh), z = a.uz + (int)(r / a.uh);
+        const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
+        const int mx = x - a.mx0, my = y - a.my0, mz = z - a.mz0;
+        const bool in_new = a.md > 0 && mx >= 0 && mx < a.mw && my >= 0 && my < a.mh && mz >= 0 && mz < a.md &&
+                            a.mask[mx + (int64_t)a.mw * (my + (int64_t)a.mh * mz)] != 0;
+        const bool cleared = a.solid[i] == SOLID_OBJ;
+        if (cleared || in_new) {
+            a.ops[i] = (uint8_t)((cleared ? 1 : 0) | (in_new ? 2 : 0));
+            a.solid[i] = in_new ? (uint8_t)SOLID_OBJ : (uint8_t)FLUID;
+            v[1] = x; v[2] = y; v[3] = z; v[4] = x; v[5] = y; v[6] = z;
+        }
+        if (in_new) {
+            v[0] = 1;
+            v[7] = x; v[8] = y; v[9] = z; v[10] = x; v[11] = y; v[12] = z;
+            a.col[y + (int64_t)a.ny * z] = 1;
+            a.plan[x + (int64_t)a.nx * z] = 1;
+            atomicMin(&a.low[x + (int64_t)a.nx * z], y);
+        }
+    }
+    // workgroup reduction, then one atomic per value
+    __shared__ int part[4][13];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int rv[13];
+    rv[0] = wave_add(v[0]);
+#pragma unroll
+    for (int k = 1; k < 13; k++) rv[k] = (k - 1) % 6 < 3 ? wave_min(v[k]) : wave_max(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 13; k++) part[wv][k] = rv[k];
+    __syncthreads();
+    if (threadIdx.x < 13) {
+        const int k = threadIdx.x;
+        int w = part[0][k];
+        for (int j = 1; j < 4; j++) {
+            const int u = part[j][k];
+            w = k == 0 ? w + u : ((k - 1) % 6 < 3 ? min(w, u) : max(w, u));
+        }
+        if (k == 0) { if (w) atomicAdd(&a.res[0], w); }
+        else if ((k - 1) % 6 < 3) { if (w != big) atomicMin(&a.res[k], w); }
+        else if (w >= 0) atomicMax(&a.res[k], w);
+    }
+}
+
+// Projections of the new body: frontal area (y, z columns), planform area
+// (x, z columns) and the histogram of its lowest cells' gaps above the floor
+// (y - 1 > 0), from which the host takes the median (the ride height).
+__global__ void __launch_bounds__(256) k_revox_stats(RevoxArgs a)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int c = 0, p = 0;
+    if (t < (int64_t)a.ny * a.nz) c = a.col[t] != 0;
+    if (t < (int64_t)a.nx * a.nz && a.plan[t]) {
+        p = 1;
+        const int g = a.low[t] - 1;
+        if (g > 0) atomicAdd(&a.hist[g], 1);
+    }
+    c = wave_add(c);
+    p = wave_add(p);
+    if ((threadIdx.x & 63) == 0) {
+        if (c) atomicAdd(&a.res[13], c);
+        if (p) atomicAdd(&a.res[14], p);
+    }
+}
+
+template <class E>
+__global__ void __launch_bounds__(256) k_gather_box(const E *src, E *dst, int bx0, int by0, int bz0, int w, int hgt,
+                                                    int64_t cnt, int nx, int ny)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int x = (int)(t % w);
+    const int64_t r = t / w;
+    const int y = (int)(r % hgt), z = (int)(r / hgt);
+    dst[t] = src[(bx0 + x) + (int64_t)nx * ((by0 + y) + (int64_t)ny * (bz0 + z))];
+}
+
+int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], const uint8_t *d_mask, void *d_work,
+                          int *res, int *hist)
+{
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    uint8_t *col = (uint8_t *)d_work, *plan = col + ((size_t)ny * nz + 255) / 256 * 256;
+    int *low = (int *)(plan + ((size_t)nx * nz + 255) / 256 * 256);
+    int *dres = low + (size_t)nx * nz, *dhist = dres + 16;
+    WT_HIP(hipMemsetAsync(col, 0, (size_t)ny * nz, h->stream));
+    WT_HIP(hipMemsetAsync(plan, 0, (size_t)nx * nz, h->stream));
+    WT_HIP(hipMemsetAsync(low, 0x7f, sizeof(int) * (size_t)nx * nz, h->stream));
+    WT_HIP(hipMemsetAsync(dhist, 0, sizeof(int) * (size_t)ny, h->stream));
+    WT_HIP(hipMemcpyAsync(dres, res, sizeof(int) * 16, hipMemcpyHostToDevice, h->stream));
+    RevoxArgs a{};
+    a.solid = h->d_solid; a.ops = h->d_ops; a.mask = d_mask;
+    a.nx = nx; a.ny = ny; a.nz = nz;
+    a.ux = ubox[0]; a.uy = ubox[1]; a.uz = ubox[2];
+    a.uw = ubox[3] - ubox[0] + 1; a.uh = ubox[4] - ubox[1] + 1; a.ud = ubox[5] - ubox[2] + 1;
+    a.mx0 = mbox[0]; a.my0 = mbox[1]; a.mz0 = mbox[2];
+    const bool mask_empty = mbox[3] < mbox[0] || mbox[4] < mbox[1] || mbox[5] < mbox[2];
+    a.mw = mask_empty ? 0 : mbox[3] - mbox[0] + 1;
+    a.mh = mask_empty ? 0 : mbox[4] - mbox[1] + 1;
+    a.md = mask_empty ? 0 : mbox[5] - mbox[2] + 1;
+    a.res = dres; a.col = col; a.plan = plan; a.low = low; a.hist = dhist;
+    if (a.uw > 0 && a.uh > 0 && a.ud > 0) {
+        const int64_t cnt = (int64_t)a.uw * a.uh * a.ud;
+        hipLaunchKernelGGL(k_revox, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, a);
+        WT_HIP(hipGetLastError());
+    }
+    const int64_t sc = std::max<int64_t>((int64_t)ny * nz, (int64_t)nx * nz);
+    hipLaunchKernelGGL(k_revox_stats, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, h->stream, a);
+    WT_HIP(hipGetLastError());
+    WT_HIP(hipMemcpyAsync(res, dres, sizeof(int) * 16, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipMemcpyAsync(hist, dhist, sizeof(int) * (size_t)ny, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int lbm_gather_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6])
+{
+    const int w = box[3] - box[0] + 1, hgt = box[4] - box[1] + 1, d = box[5] - box[2] + 1;
+    if (w <= 0 || hgt <= 0 || d <= 0) return 0;
+    if (box[0] < 0 || box[1] < 0 || box[2] < 0 || box[3] >= h->nx || box[4] >= h->ny || box[5] >= h->nz)
+        return wt_fail(WT_E_ARG, "gather box outside the lattice");
+    if (elem != 1) return wt_fail(WT_E_ARG, "gather element of %d bytes", elem);
+    const int64_t cnt = (int64_t)w * hgt * d;
+    hipLaunchKernelGGL(k_gather_box<uint8_t>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream,
+                       (const uint8_t *)src, (uint8_t *)dst, box[0], box[1], box[2], w, hgt, cnt, h->nx, h->ny);
+    WT_HIP(hipGetLastError());
+    return 0;
+}
+
 int lbm_scatter_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6])
 {
     const int w = box[3] - box[0] + 1, hgt = box[4] - box[1] + 1, d = box[5] - box[2] + 1;

@@ -151,6 +151,9 @@ struct wt_lattice {
     // device copy of a voxel grid for the grid-backed body lookup, grown on demand
     uint8_t *d_vgrid = nullptr;
     size_t vgrid_cap = 0;
+    // device re-voxelize work space (projections, lowest cells, results)
+    void *d_rvwork = nullptr;
+    size_t rvwork_cap = 0;
     // a plane range of the near-wall list is rebuilt on the device into the
     // second buffer (d_near2, same capacity), around the unchanged planes
     // copied from d_near; the two are then swapped. d_pcnt: per-plane counts
@@ -182,6 +185,17 @@ int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1);
 // x, y, z order at src (elem bytes each, elem 1 or 4) written into the
 // lattice-shaped array dst
 int lbm_scatter_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6]);
+// the reverse: box cells of the lattice-shaped byte array src packed into dst
+int lbm_gather_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6]);
+// Re-voxelize on the device from a device mask over mbox (no wall
+// velocities, no road change): ubox is the union of the previous body's box
+// and mbox; d_work holds ny*nz and nx*nz bytes (each rounded up to 256) and
+// then nx*nz + 16 + ny ints. res (16 ints, in/out) and hist (ny ints, out) come back on
+// the host: res[0] body cells, [1..6] changed box, [7..12] new body box,
+// [13] frontal area, [14] planform area; hist[g]: (x, z) columns whose
+// lowest body cell is at y = g + 1.
+int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], const uint8_t *d_mask, void *d_work,
+                          int *res, int *hist);
 int lbm_channel(wt_lattice *h, bool macro);
 // Launches one step over z rows [bz0,bz1). Its force (when force_out is
 // given) is reduced into force_out by the NEXT step launch or by
