@@ -196,6 +196,14 @@ extern "C" int wt_voxelize_parts(wt_lattice *h, const wt_part *parts, int nparts
             rc = wt_fail(WT_E_DEVICE, "wheel upload failed");
         if (!rc) hipLaunchKernelGGL(k_parts_mask, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, a, d_mask, d_wv);
         if (!rc && hipGetLastError() != hipSuccess) rc = wt_fail(WT_E_DEVICE, "body kernel launch failed");
+        if (!rc && a.nwheels == 0 && voxelize_device_applies(h, ground, roadVel)) {
+            // no wall velocities: the whole update on the device
+            rc = voxelize_device(h, box, d_mask, out);
+            (void)hipFree(d_mask);
+            (void)hipFree(d_wv);
+            (void)hipFree(d_consts);
+            return rc;
+        }
         if (!rc && hipMemcpyAsync(mask.data(), d_mask, (size_t)cnt, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "mask download failed");
         if (!rc && a.nwheels) {

@@ -486,14 +486,23 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     return 0;
 }
 
+}  // extern "C"
+
 // wt_voxelize for a body already rasterised into a device mask over box,
 // when no wall velocity is set anywhere and the road stays as it is: the
 // diff against the previous body, the ops, the projections and the lowest
 // cells are computed on the device, and only a few scalars, a histogram and
 // the changed box of the solid map (for the host mirror) come back. The
 // result equals wt_voxelize's on the same mask.
-static int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_body_stats *out)
+bool voxelize_device_applies(const wt_lattice *h, int ground, double roadVel)
 {
+    const double rv = (roadVel != roadVel || roadVel == 0) ? 0 : roadVel;
+    return h->uw_cells == 0 && rv == 0 && (!!ground) == h->ground && !std::getenv("WT_VOXELIZE_HOST");
+}
+
+int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_body_stats *out)
+{
+    h->epoch++;
     const int nx = h->nx, ny = h->ny, nz = h->nz;
     const int x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
     const bool empty = x1 < x0 || y1 < y0 || z1 < z0;
@@ -570,6 +579,8 @@ static int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mas
     return 0;
 }
 
+extern "C" {
+
 int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, const double M[9],
                      const double pos[3], double scale, const int box[6], int ground, double roadVel,
                      wt_body_stats *out)
@@ -598,11 +609,7 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
         if (!rc) rc = lbm_grid_mask(h, d_grid, vm, M, pos, scale, box, d_mask);
         // no wall velocity anywhere and the road as it is: the whole update on
         // the device (env WT_VOXELIZE_HOST=1 keeps the host bookkeeping)
-        const double rv = (roadVel != roadVel || roadVel == 0) ? 0 : roadVel;
-        if (!rc && h->uw_cells == 0 && rv == 0 && (!!ground) == h->ground && !std::getenv("WT_VOXELIZE_HOST")) {
-            h->epoch++;
-            return voxelize_device(h, box, d_mask, out);
-        }
+        if (!rc && voxelize_device_applies(h, ground, roadVel)) return voxelize_device(h, box, d_mask, out);
         if (!rc && hipMemcpyAsync(mask.data(), d_mask, (size_t)cnt, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "mask download failed");
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "grid lookup failed");

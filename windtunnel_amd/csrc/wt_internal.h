@@ -185,6 +185,11 @@ int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1);
 // x, y, z order at src (elem bytes each, elem 1 or 4) written into the
 // lattice-shaped array dst
 int lbm_scatter_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6]);
+// wt_voxelize from a body mask already on the device (capi.hip): applies
+// when no cell has a wall velocity and the road stays as it is, and then
+// gives wt_voxelize's result on the same mask
+bool voxelize_device_applies(const wt_lattice *h, int ground, double roadVel);
+int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_body_stats *out);
 // the reverse: box cells of the lattice-shaped byte array src packed into dst
 int lbm_gather_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6]);
 // Re-voxelize on the device from a device mask over mbox (no wall
