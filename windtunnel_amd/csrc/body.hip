@@ -196,9 +196,9 @@ extern "C" int wt_voxelize_parts(wt_lattice *h, const wt_part *parts, int nparts
             rc = wt_fail(WT_E_DEVICE, "wheel upload failed");
         if (!rc) hipLaunchKernelGGL(k_parts_mask, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, a, d_mask, d_wv);
         if (!rc && hipGetLastError() != hipSuccess) rc = wt_fail(WT_E_DEVICE, "body kernel launch failed");
-        if (!rc && a.nwheels == 0 && voxelize_device_applies(h, ground, roadVel)) {
-            // no wall velocities: the whole update on the device
-            rc = voxelize_device(h, box, d_mask, out);
+        if (!rc && voxelize_device_applies(h, ground)) {
+            // the road's solid state as it is: the whole update on the device
+            rc = voxelize_device(h, box, d_mask, a.nwheels ? d_wv : nullptr, ground, roadVel, out);
             (void)hipFree(d_mask);
             (void)hipFree(d_wv);
             (void)hipFree(d_consts);

@@ -488,19 +488,20 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
 
 }  // extern "C"
 
-// wt_voxelize for a body already rasterised into a device mask over box,
-// when no wall velocity is set anywhere and the road stays as it is: the
-// diff against the previous body, the ops, the projections and the lowest
-// cells are computed on the device, and only a few scalars, a histogram and
-// the changed box of the solid map (for the host mirror) come back. The
-// result equals wt_voxelize's on the same mask.
-bool voxelize_device_applies(const wt_lattice *h, int ground, double roadVel)
+// wt_voxelize for a body already rasterised into a device mask over box
+// (d_wv: the wall velocities of its mask-2 cells, 3 per box cell, or null),
+// when the road's solid state stays as it is: the diff against the previous
+// body, the ops, the wall velocities, the projections and the lowest cells
+// are computed on the device, and only a few scalars, a histogram and the
+// changed boxes of the solid map and wall velocities (for the host mirrors)
+// come back. The result equals wt_voxelize's on the same mask.
+bool voxelize_device_applies(const wt_lattice *h, int ground)
 {
-    const double rv = (roadVel != roadVel || roadVel == 0) ? 0 : roadVel;
-    return h->uw_cells == 0 && rv == 0 && (!!ground) == h->ground && !std::getenv("WT_VOXELIZE_HOST");
+    return (!!ground) == h->ground && !std::getenv("WT_VOXELIZE_HOST");
 }
 
-int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_body_stats *out)
+int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, const float *d_wv, int ground,
+                    double roadVel, wt_body_stats *out)
 {
     h->epoch++;
     const int nx = h->nx, ny = h->ny, nz = h->nz;
@@ -509,6 +510,7 @@ int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_b
     if (!empty && (x0 < 1 || y0 < 1 || z0 < 1 || x1 > nx - 2 || y1 > ny - 2 || z1 > nz - 2))
         return wt_fail(WT_E_ARG, "box [%d,%d,%d]-[%d,%d,%d] is not within the interior", x0, y0, z0, x1, y1, z1);
     const int64_t n = h->n, plane = (int64_t)nx * ny;
+    const double rv = (roadVel != roadVel || roadVel == 0) ? 0 : roadVel;
     const int *ob = h->obj_box;
     const bool oempty = ob[3] < ob[0] || ob[4] < ob[1] || ob[5] < ob[2];
     int mbox[6] = {1, 1, 1, 0, 0, 0}, ub[6] = {1, 1, 1, 0, 0, 0};
@@ -522,7 +524,7 @@ int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_b
         WT_HIP(hipMemset(h->d_ops, 0, (size_t)n));
     }
     const size_t work = ((size_t)ny * nz + 255) / 256 * 256 + ((size_t)nx * nz + 255) / 256 * 256 +
-                        sizeof(int) * ((size_t)nx * nz + 16 + (size_t)ny);
+                        sizeof(int) * ((size_t)nx * nz + RV_N + (size_t)ny);
     if (work > h->rvwork_cap) {
         if (h->d_rvwork) (void)hipFree(h->d_rvwork);
         h->d_rvwork = nullptr;
@@ -531,13 +533,18 @@ int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_b
         h->rvwork_cap = work;
     }
     const int big = 0x7fffffff;
-    int res[16] = {0, big, big, big, -1, -1, -1, big, big, big, -1, -1, -1, 0, 0, 0};
+    int res[RV_N] = {0};
+    for (int k = 0; k < 6; k++) {
+        const int init = k < 3 ? big : -1;
+        res[RV_CB + k] = init; res[RV_NB + k] = init; res[RV_UB + k] = init;
+    }
     std::vector<int> hist((size_t)ny, 0);
-    if (int rc = lbm_revoxelize_device(h, ub, mbox, d_mask, h->d_rvwork, res, hist.data())) return rc;
+    const float road = ground ? (float)rv : 0.0f;
+    if (int rc = lbm_revoxelize_device(h, ub, mbox, d_mask, d_wv, road, h->d_rvwork, res, hist.data())) return rc;
     // js/lbm.js:240-246: projections and the median gap under the body
-    const int64_t cells = res[0];
-    h->body.frontalArea = std::max<int64_t>(1, res[13]);
-    h->body.planformArea = std::max<int64_t>(1, res[14]);
+    const int64_t cells = res[RV_CELLS];
+    h->body.frontalArea = std::max<int64_t>(1, res[RV_FRONTAL]);
+    h->body.planformArea = std::max<int64_t>(1, res[RV_PLAN]);
     h->body.solidCells = cells;
     int64_t total = 0;
     for (int c : hist) total += c;
@@ -551,30 +558,47 @@ int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_b
         }
     }
     h->body.rideHeight = ride;
-    h->p.movingWalls = 0;
-    if (cells > 0) std::copy(res + 7, res + 13, h->obj_box);
+    const int mov = (rv != 0 || res[RV_WHEELS] > 0) ? 1 : 0;
+    h->p.movingWalls = mov;
+    h->uw_cells += res[RV_NZ];
+    if (cells > 0) std::copy(res + RV_NB, res + RV_NB + 6, h->obj_box);
     else { const int e[6] = {nx, ny, nz, -1, -1, -1}; std::copy(e, e + 6, h->obj_box); }
-    const int *cb = res + 1;
-    if (cb[5] >= cb[2]) {
+    const int *cb = res + RV_CB, *wb = res + RV_UB;
+    const bool cdirty = cb[5] >= cb[2], wdirty = wb[5] >= wb[2];
+    if (cdirty) {
         if (int rc = lbm_apply_voxelize(h, h->d_ops, plane * cb[2], plane * (cb[5] + 1))) return rc;
         if (int rc = lbm_rebuild_masks_range(h, std::max(0, cb[2] - 1), std::min(nz - 1, cb[5] + 1))) return rc;
-        // the host mirror of the solid map over the changed box
-        const size_t cn = (size_t)(cb[3] - cb[0] + 1) * (size_t)(cb[4] - cb[1] + 1) * (size_t)(cb[5] - cb[2] + 1);
-        if (int rc = stage_reserve(h, cn)) return rc;
-        if (int rc = lbm_gather_box(h, h->d_solid, h->d_stage, 1, cb)) return rc;
-        WT_HIP(hipMemcpyAsync(h->h_stage, h->d_stage, cn, hipMemcpyDeviceToHost, h->stream));
+    }
+    // the host mirrors over the changed boxes
+    auto cells_of = [](const int *b) {
+        return (size_t)(b[3] - b[0] + 1) * (size_t)(b[4] - b[1] + 1) * (size_t)(b[5] - b[2] + 1);
+    };
+    const size_t cn = cdirty ? cells_of(cb) : 0, wn = wdirty ? cells_of(wb) : 0;
+    const size_t wbase = (cn + 15) / 16 * 16;
+    if (cn + wn > 0) {
+        if (int rc = stage_reserve(h, wbase + 12 * wn)) return rc;
+        if (cdirty) {
+            if (int rc = lbm_gather_box(h, h->d_solid, h->d_stage, 1, cb)) return rc;
+        }
+        for (int c = 0; wdirty && c < 3; c++)
+            if (int rc = lbm_gather_box(h, h->d_uw + c * n, (uint8_t *)h->d_stage + wbase + c * 4 * wn, 4, wb)) return rc;
+        WT_HIP(hipMemcpyAsync(h->h_stage, h->d_stage, wbase + 12 * wn, hipMemcpyDeviceToHost, h->stream));
         WT_HIP(hipStreamSynchronize(h->stream));
-        const size_t row = (size_t)(cb[3] - cb[0] + 1);
-        const uint8_t *src = h->h_stage;
-        for (int z = cb[2]; z <= cb[5]; z++)
-            for (int y = cb[1]; y <= cb[4]; y++, src += row)
-                std::memcpy(h->h_solid.data() + cb[0] + nx * ((int64_t)y + (int64_t)ny * z), src, row);
+        auto scatter = [&](const uint8_t *src, uint8_t *dst, const int *b, int elem) {
+            const size_t row = (size_t)(b[3] - b[0] + 1) * (size_t)elem;
+            for (int z = b[2]; z <= b[5]; z++)
+                for (int y = b[1]; y <= b[4]; y++, src += row)
+                    std::memcpy(dst + (size_t)(b[0] + nx * ((int64_t)y + (int64_t)ny * z)) * (size_t)elem, src, row);
+        };
+        if (cdirty) scatter(h->h_stage, h->h_solid.data(), cb, 1);
+        for (int c = 0; wdirty && c < 3; c++)
+            scatter(h->h_stage + wbase + c * 4 * wn, (uint8_t *)(h->h_uw.data() + c * n), wb, 4);
     }
     WT_HIP(hipStreamSynchronize(h->stream));
     if (out) {
         out->frontalArea = h->body.frontalArea; out->planformArea = h->body.planformArea;
         out->solidCells = h->body.solidCells; out->rideHeight = h->body.rideHeight;
-        out->movingWalls = 0; out->ground = h->ground;
+        out->movingWalls = mov; out->ground = h->ground;
     }
     return 0;
 }
@@ -607,9 +631,9 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
         if (hipMemcpyAsync(d_grid, grid, gbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "grid upload failed");
         if (!rc) rc = lbm_grid_mask(h, d_grid, vm, M, pos, scale, box, d_mask);
-        // no wall velocity anywhere and the road as it is: the whole update on
-        // the device (env WT_VOXELIZE_HOST=1 keeps the host bookkeeping)
-        if (!rc && voxelize_device_applies(h, ground, roadVel)) return voxelize_device(h, box, d_mask, out);
+        // the road's solid state as it is: the whole update on the device
+        // (env WT_VOXELIZE_HOST=1 keeps the host bookkeeping)
+        if (!rc && voxelize_device_applies(h, ground)) return voxelize_device(h, box, d_mask, nullptr, ground, roadVel, out);
         if (!rc && hipMemcpyAsync(mask.data(), d_mask, (size_t)cnt, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "mask download failed");
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "grid lookup failed");

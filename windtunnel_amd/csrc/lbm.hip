@@ -1331,20 +1331,26 @@ __global__ void __launch_bounds__(256) k_scatter_box(const E *src, E *dst, int b
     dst[(bx0 + x) + (int64_t)nx * ((by0 + y) + (int64_t)ny * (bz0 + z))] = src[t];
 }
 
-// Device re-voxelize of a body given as a device mask (no wall velocities,
-// no road change: the caller checks). One thread per cell of the union of
+// Device re-voxelize of a body given as a device mask (the road's solid
+// state unchanged: the caller checks). One thread per cell of the union of
 // the previous body's box and the new mask's box: the previous body's cells
-// become fluid (op 1), the mask's cells body (op 2), as the host bookkeeping
-// does (js/lbm.js:194-246). The changed box, the new body's box and cell
-// count are reduced per workgroup into res; the projections go to col / plan
-// (bytes) and the lowest cell of each (x, z) column to low.
+// become fluid (op 1), the mask's cells body (op 2), and their wall
+// velocities are set (mask 2: the per-cell velocity, otherwise zero), as the
+// host bookkeeping does (js/lbm.js:194-246). Reduced per workgroup into res:
+// body cells, the changed box, the new body's box, wheel cells, the change of
+// the count of cells with a nonzero wall velocity and the box of changed wall
+// velocities. The projections go to col / plan (bytes) and the lowest cell of
+// each (x, z) column to low.
 struct RevoxArgs {
     uint8_t *solid, *ops;
     const uint8_t *mask;
+    const float *wv;                  // 3 per mask cell (mask 2 cells), or null
+    float *uwx, *uwy, *uwz;
     int nx, ny, nz;
     int ux, uy, uz, uw, uh, ud;       // union box origin and extents
     int mx0, my0, mz0, mw, mh, md;    // mask box origin and extents (md 0: no mask)
-    int *res;  // 0 cells, 1-6 changed box, 7-12 new body box, 13 frontal, 14 planform
+    float road;                       // the road row's x velocity (ground), else 0
+    int *res;  // REVOX_* slots
     uint8_t *col, *plan;
     int *low, *hist;
 };
@@ -1368,56 +1374,109 @@ __device__ __forceinline__ int wave_add(int v)
     return v;
 }
 
+// the 21 reduced values of a thread: kind 0 sum, 1 min, 2 max; slot in res
+__device__ constexpr int rv_kind(int k)
+{
+    return (k == 0 || k == 13 || k == 14) ? 0
+                                          : ((k >= 1 && k <= 3) || (k >= 7 && k <= 9) || (k >= 15 && k <= 17)) ? 1 : 2;
+}
+__device__ constexpr int rv_slot(int k) { return k < 13 ? k : (k == 13 ? RV_WHEELS : (k == 14 ? RV_NZ : RV_UB + (k - 15))); }
+
+// wall velocity of cell i set to (vx, vy, vz) where it differs; records the
+// change in v (nonzero-count delta at 14, changed box at 15-20)
+__device__ __forceinline__ void revox_set_uw(const RevoxArgs &a, int64_t i, int x, int y, int z, float vx, float vy,
+                                             float vz, int (&v)[21])
+{
+    const float ox = a.uwx[i], oy = a.uwy[i], oz = a.uwz[i];
+    if (ox != vx || oy != vy || oz != vz) {
+        a.uwx[i] = vx; a.uwy[i] = vy; a.uwz[i] = vz;
+        v[14] += (int)(vx != 0 || vy != 0 || vz != 0) - (int)(ox != 0 || oy != 0 || oz != 0);
+        v[15] = min(v[15], x); v[16] = min(v[16], y); v[17] = min(v[17], z);
+        v[18] = max(v[18], x); v[19] = max(v[19], y); v[20] = max(v[20], z);
+    }
+}
+
+__device__ __forceinline__ void revox_reduce(const RevoxArgs &a, int (&v)[21])
+{
+    const int big = 0x7fffffff;
+    __shared__ int part[4][21];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 21; k++) {
+        const int r = rv_kind(k) == 0 ? wave_add(v[k]) : (rv_kind(k) == 1 ? wave_min(v[k]) : wave_max(v[k]));
+        if (lane == 0) part[wv][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 21) {
+        const int k = threadIdx.x, kind = rv_kind(k);
+        int w = part[0][k];
+        for (int j = 1; j < 4; j++) {
+            const int u = part[j][k];
+            w = kind == 0 ? w + u : (kind == 1 ? min(w, u) : max(w, u));
+        }
+        int *dst = a.res + rv_slot(k);
+        if (kind == 0) { if (w) atomicAdd(dst, w); }
+        else if (kind == 1) { if (w != big) atomicMin(dst, w); }
+        else if (w >= 0) atomicMax(dst, w);
+    }
+}
+
+__device__ __forceinline__ void revox_init(int (&v)[21])
+{
+    const int big = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < 21; k++) v[k] = rv_kind(k) == 0 ? 0 : (rv_kind(k) == 1 ? big : -1);
+}
+
 __global__ void __launch_bounds__(256) k_revox(RevoxArgs a)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t cnt = (int64_t)a.uw * a.uh * a.ud;
-    const int big = 0x7fffffff;
-    int v[13] = {0, big, big, big, -1, -1, -1, big, big, big, -1, -1, -1};
+    int v[21];
+    revox_init(v);
     if (t < cnt) {
         const int x = a.ux + (int)(t % a.uw);
         const int64_t r = t / a.uw;
         const int y = a.uy + (int)(r % a.uh), z = a.uz + (int)(r / a.uh);
         const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
         const int mx = x - a.mx0, my = y - a.my0, mz = z - a.mz0;
-        const bool in_new = a.md > 0 && mx >= 0 && mx < a.mw && my >= 0 && my < a.mh && mz >= 0 && mz < a.md &&
-                            a.mask[mx + (int64_t)a.mw * (my + (int64_t)a.mh * mz)] != 0;
+        const bool in_box = a.md > 0 && mx >= 0 && mx < a.mw && my >= 0 && my < a.mh && mz >= 0 && mz < a.md;
+        const int64_t k = in_box ? mx + (int64_t)a.mw * (my + (int64_t)a.mh * mz) : 0;
+        const int mv = in_box ? a.mask[k] : 0;
+        const bool in_new = mv != 0;
         const bool cleared = a.solid[i] == SOLID_OBJ;
         if (cleared || in_new) {
             a.ops[i] = (uint8_t)((cleared ? 1 : 0) | (in_new ? 2 : 0));
             a.solid[i] = in_new ? (uint8_t)SOLID_OBJ : (uint8_t)FLUID;
             v[1] = x; v[2] = y; v[3] = z; v[4] = x; v[5] = y; v[6] = z;
+            float vx = 0, vy = 0, vz = 0;
+            if (mv == 2 && a.wv) { vx = a.wv[3 * k]; vy = a.wv[3 * k + 1]; vz = a.wv[3 * k + 2]; }
+            revox_set_uw(a, i, x, y, z, vx, vy, vz, v);
         }
         if (in_new) {
             v[0] = 1;
+            v[13] = mv == 2;
             v[7] = x; v[8] = y; v[9] = z; v[10] = x; v[11] = y; v[12] = z;
             a.col[y + (int64_t)a.ny * z] = 1;
             a.plan[x + (int64_t)a.nx * z] = 1;
             atomicMin(&a.low[x + (int64_t)a.nx * z], y);
         }
     }
-    // workgroup reduction, then one atomic per value
-    __shared__ int part[4][13];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int rv[13];
-    rv[0] = wave_add(v[0]);
-#pragma unroll
-    for (int k = 1; k < 13; k++) rv[k] = (k - 1) % 6 < 3 ? wave_min(v[k]) : wave_max(v[k]);
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < 13; k++) part[wv][k] = rv[k];
-    __syncthreads();
-    if (threadIdx.x < 13) {
-        const int k = threadIdx.x;
-        int w = part[0][k];
-        for (int j = 1; j < 4; j++) {
-            const int u = part[j][k];
-            w = k == 0 ? w + u : ((k - 1) % 6 < 3 ? min(w, u) : max(w, u));
-        }
-        if (k == 0) { if (w) atomicAdd(&a.res[0], w); }
-        else if ((k - 1) % 6 < 3) { if (w != big) atomicMin(&a.res[k], w); }
-        else if (w >= 0) atomicMax(&a.res[k], w);
+    revox_reduce(a, v);
+}
+
+// The road row (y = 0, js/lbm.js:204-214): its wall velocity (a.road, 0, 0)
+// wherever it differs.
+__global__ void __launch_bounds__(256) k_revox_road(RevoxArgs a)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int v[21];
+    revox_init(v);
+    if (t < (int64_t)a.nx * a.nz) {
+        const int x = (int)(t % a.nx), z = (int)(t / a.nx);
+        revox_set_uw(a, x + (int64_t)a.nx * a.ny * z, x, 0, z, a.road, 0.0f, 0.0f, v);
     }
+    revox_reduce(a, v);
 }
 
 // Projections of the new body: frontal area (y, z columns), planform area
@@ -1436,8 +1495,8 @@ __global__ void __launch_bounds__(256) k_revox_stats(RevoxArgs a)
     c = wave_add(c);
     p = wave_add(p);
     if ((threadIdx.x & 63) == 0) {
-        if (c) atomicAdd(&a.res[13], c);
-        if (p) atomicAdd(&a.res[14], p);
+        if (c) atomicAdd(&a.res[RV_FRONTAL], c);
+        if (p) atomicAdd(&a.res[RV_PLAN], p);
     }
 }
 
@@ -1453,20 +1512,21 @@ __global__ void __launch_bounds__(256) k_gather_box(const E *src, E *dst, int bx
     dst[t] = src[(bx0 + x) + (int64_t)nx * ((by0 + y) + (int64_t)ny * (bz0 + z))];
 }
 
-int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], const uint8_t *d_mask, void *d_work,
-                          int *res, int *hist)
+int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], const uint8_t *d_mask, const float *d_wv,
+                          float road, void *d_work, int *res, int *hist)
 {
     const int nx = h->nx, ny = h->ny, nz = h->nz;
     uint8_t *col = (uint8_t *)d_work, *plan = col + ((size_t)ny * nz + 255) / 256 * 256;
     int *low = (int *)(plan + ((size_t)nx * nz + 255) / 256 * 256);
-    int *dres = low + (size_t)nx * nz, *dhist = dres + 16;
+    int *dres = low + (size_t)nx * nz, *dhist = dres + RV_N;
     WT_HIP(hipMemsetAsync(col, 0, (size_t)ny * nz, h->stream));
     WT_HIP(hipMemsetAsync(plan, 0, (size_t)nx * nz, h->stream));
     WT_HIP(hipMemsetAsync(low, 0x7f, sizeof(int) * (size_t)nx * nz, h->stream));
     WT_HIP(hipMemsetAsync(dhist, 0, sizeof(int) * (size_t)ny, h->stream));
-    WT_HIP(hipMemcpyAsync(dres, res, sizeof(int) * 16, hipMemcpyHostToDevice, h->stream));
+    WT_HIP(hipMemcpyAsync(dres, res, sizeof(int) * RV_N, hipMemcpyHostToDevice, h->stream));
     RevoxArgs a{};
-    a.solid = h->d_solid; a.ops = h->d_ops; a.mask = d_mask;
+    a.solid = h->d_solid; a.ops = h->d_ops; a.mask = d_mask; a.wv = d_wv;
+    a.uwx = h->d_uw; a.uwy = h->d_uw + h->n; a.uwz = h->d_uw + 2 * h->n;
     a.nx = nx; a.ny = ny; a.nz = nz;
     a.ux = ubox[0]; a.uy = ubox[1]; a.uz = ubox[2];
     a.uw = ubox[3] - ubox[0] + 1; a.uh = ubox[4] - ubox[1] + 1; a.ud = ubox[5] - ubox[2] + 1;
@@ -1475,16 +1535,20 @@ int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], c
     a.mw = mask_empty ? 0 : mbox[3] - mbox[0] + 1;
     a.mh = mask_empty ? 0 : mbox[4] - mbox[1] + 1;
     a.md = mask_empty ? 0 : mbox[5] - mbox[2] + 1;
+    a.road = road;
     a.res = dres; a.col = col; a.plan = plan; a.low = low; a.hist = dhist;
     if (a.uw > 0 && a.uh > 0 && a.ud > 0) {
         const int64_t cnt = (int64_t)a.uw * a.uh * a.ud;
         hipLaunchKernelGGL(k_revox, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, a);
         WT_HIP(hipGetLastError());
     }
+    const int64_t rc = (int64_t)nx * nz;
+    hipLaunchKernelGGL(k_revox_road, dim3((unsigned)((rc + 255) / 256)), dim3(256), 0, h->stream, a);
+    WT_HIP(hipGetLastError());
     const int64_t sc = std::max<int64_t>((int64_t)ny * nz, (int64_t)nx * nz);
     hipLaunchKernelGGL(k_revox_stats, dim3((unsigned)((sc + 255) / 256)), dim3(256), 0, h->stream, a);
     WT_HIP(hipGetLastError());
-    WT_HIP(hipMemcpyAsync(res, dres, sizeof(int) * 16, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipMemcpyAsync(res, dres, sizeof(int) * RV_N, hipMemcpyDeviceToHost, h->stream));
     WT_HIP(hipMemcpyAsync(hist, dhist, sizeof(int) * (size_t)ny, hipMemcpyDeviceToHost, h->stream));
     WT_HIP(hipStreamSynchronize(h->stream));
     return 0;
@@ -1496,10 +1560,16 @@ int lbm_gather_box(wt_lattice *h, const void *src, void *dst, int elem, const in
     if (w <= 0 || hgt <= 0 || d <= 0) return 0;
     if (box[0] < 0 || box[1] < 0 || box[2] < 0 || box[3] >= h->nx || box[4] >= h->ny || box[5] >= h->nz)
         return wt_fail(WT_E_ARG, "gather box outside the lattice");
-    if (elem != 1) return wt_fail(WT_E_ARG, "gather element of %d bytes", elem);
     const int64_t cnt = (int64_t)w * hgt * d;
-    hipLaunchKernelGGL(k_gather_box<uint8_t>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream,
-                       (const uint8_t *)src, (uint8_t *)dst, box[0], box[1], box[2], w, hgt, cnt, h->nx, h->ny);
+    const dim3 grid((unsigned)((cnt + 255) / 256));
+    if (elem == 1)
+        hipLaunchKernelGGL(k_gather_box<uint8_t>, grid, dim3(256), 0, h->stream, (const uint8_t *)src, (uint8_t *)dst,
+                           box[0], box[1], box[2], w, hgt, cnt, h->nx, h->ny);
+    else if (elem == 4)
+        hipLaunchKernelGGL(k_gather_box<uint32_t>, grid, dim3(256), 0, h->stream, (const uint32_t *)src,
+                           (uint32_t *)dst, box[0], box[1], box[2], w, hgt, cnt, h->nx, h->ny);
+    else
+        return wt_fail(WT_E_ARG, "gather element of %d bytes", elem);
     WT_HIP(hipGetLastError());
     return 0;
 }

@@ -186,21 +186,30 @@ int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1);
 // lattice-shaped array dst
 int lbm_scatter_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6]);
 // wt_voxelize from a body mask already on the device (capi.hip): applies
-// when no cell has a wall velocity and the road stays as it is, and then
-// gives wt_voxelize's result on the same mask
-bool voxelize_device_applies(const wt_lattice *h, int ground, double roadVel);
-int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, wt_body_stats *out);
+// when the road's solid state stays as it is, and then gives wt_voxelize's
+// result on the same mask. d_wv: 3 floats per mask cell (its mask-2 cells'
+// wall velocities), or null.
+bool voxelize_device_applies(const wt_lattice *h, int ground);
+int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, const float *d_wv, int ground,
+                    double roadVel, wt_body_stats *out);
 // the reverse: box cells of the lattice-shaped byte array src packed into dst
 int lbm_gather_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6]);
-// Re-voxelize on the device from a device mask over mbox (no wall
-// velocities, no road change): ubox is the union of the previous body's box
-// and mbox; d_work holds ny*nz and nx*nz bytes (each rounded up to 256) and
-// then nx*nz + 16 + ny ints. res (16 ints, in/out) and hist (ny ints, out) come back on
-// the host: res[0] body cells, [1..6] changed box, [7..12] new body box,
-// [13] frontal area, [14] planform area; hist[g]: (x, z) columns whose
-// lowest body cell is at y = g + 1.
-int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], const uint8_t *d_mask, void *d_work,
-                          int *res, int *hist);
+// result slots of lbm_revoxelize_device: body cells, changed box (6), new
+// body box (6), frontal and planform areas, wheel cells, change of the count
+// of nonzero wall velocities, changed wall-velocity box (6)
+enum { RV_CELLS = 0, RV_CB = 1, RV_NB = 7, RV_FRONTAL = 13, RV_PLAN = 14, RV_WHEELS = 15, RV_NZ = 16, RV_UB = 17,
+       RV_N = 24 };
+// Re-voxelize on the device from a device mask over mbox (the road's solid
+// state unchanged): ubox is the union of the previous body's box and mbox;
+// d_wv as for voxelize_device; road: the road row's x wall velocity. d_work
+// holds ny*nz and nx*nz bytes (each rounded up to 256) and then nx*nz + 24 +
+// ny ints. res (24 ints, in/out; the RV_* slots of lbm.hip: body cells,
+// changed box, new body box, frontal and planform areas, wheel cells, change
+// of the nonzero wall-velocity count, changed wall-velocity box) and hist
+// (ny ints: (x, z) columns whose lowest body cell is at y = g + 1) come back
+// on the host.
+int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], const uint8_t *d_mask, const float *d_wv,
+                          float road, void *d_work, int *res, int *hist);
 int lbm_channel(wt_lattice *h, bool macro);
 // Launches one step over z rows [bz0,bz1). Its force (when force_out is
 // given) is reduced into force_out by the NEXT step launch or by
