@@ -294,6 +294,19 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     if (!empty && !wv) {
         const int64_t cnt = (int64_t)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
         if (std::memchr(mask, 2, (size_t)cnt)) return wt_fail(WT_E_ARG, "mask holds wall-velocity cells but wv is null");
+        if (voxelize_device_applies(h, ground)) {
+            // the road's solid state as it is: the mask goes up and the
+            // whole update runs on the device
+            if ((size_t)cnt > h->vgrid_cap) {
+                if (h->d_vgrid) (void)hipFree(h->d_vgrid);
+                h->d_vgrid = nullptr;
+                h->vgrid_cap = 0;
+                if (int rc = alloc((void **)&h->d_vgrid, (size_t)cnt)) return rc;
+                h->vgrid_cap = (size_t)cnt;
+            }
+            WT_HIP(hipMemcpyAsync(h->d_vgrid, mask, (size_t)cnt, hipMemcpyHostToDevice, h->stream));
+            return voxelize_device(h, box, h->d_vgrid, nullptr, ground, roadVel, out);
+        }
     }
     const int64_t n = h->n, plane = (int64_t)nx * ny;
     if (h->h_ops.empty()) h->h_ops.assign((size_t)n, 0);
