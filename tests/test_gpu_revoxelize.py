@@ -1,9 +1,10 @@
 """A sequence of re-voxelizes on one lattice (body moved, ground and road
 velocity toggled, wall-velocity cells added and removed, an empty box)
 against the oracle, whose voxelize scans the whole lattice as
-js/lbm.js:191-266 does: the device keeps only the previous body's box and
-uploads only the planes that changed, and must end in the same state bit
-for bit (populations, fields, solid and near maps, wall velocities, body
+js/lbm.js:191-266 does: the library keeps only the previous body's box and
+touches only the changed box (on the device, or through the host mirrors
+when the ground plane is switched), and must end in the same state bit for
+bit (populations, fields, solid and near maps, wall velocities, body
 statistics) after every call and after steps in between."""
 import numpy as np
 import pytest
