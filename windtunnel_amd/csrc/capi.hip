@@ -295,8 +295,7 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
         const int64_t cnt = (int64_t)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
         if (std::memchr(mask, 2, (size_t)cnt)) return wt_fail(WT_E_ARG, "mask holds wall-velocity cells but wv is null");
         if (voxelize_device_applies(h, ground)) {
-            // the road's solid state as it is: the mask goes up and the
-            // whole update runs on the device
+            // the mask goes up and the whole update runs on the device
             if ((size_t)cnt > h->vgrid_cap) {
                 if (h->d_vgrid) (void)hipFree(h->d_vgrid);
                 h->d_vgrid = nullptr;
@@ -502,15 +501,15 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
 }  // extern "C"
 
 // wt_voxelize for a body already rasterised into a device mask over box
-// (d_wv: the wall velocities of its mask-2 cells, 3 per box cell, or null),
-// when the road's solid state stays as it is: the diff against the previous
-// body, the ops, the wall velocities, the projections and the lowest cells
-// are computed on the device, and only a few scalars, a histogram and the
-// changed boxes of the solid map and wall velocities (for the host mirrors)
-// come back. The result equals wt_voxelize's on the same mask.
-bool voxelize_device_applies(const wt_lattice *h, int ground)
+// (d_wv: the wall velocities of its mask-2 cells, 3 per box cell, or null):
+// the diff against the previous body, the road row, the ops, the wall
+// velocities, the projections and the lowest cells are computed on the
+// device, and only a few scalars, a histogram and the changed boxes of the
+// solid map and wall velocities (for the host mirrors) come back. The result
+// equals wt_voxelize's on the same mask.
+bool voxelize_device_applies(const wt_lattice *, int)
 {
-    return (!!ground) == h->ground && !std::getenv("WT_VOXELIZE_HOST");
+    return !std::getenv("WT_VOXELIZE_HOST");
 }
 
 int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, const float *d_wv, int ground,
@@ -553,7 +552,9 @@ int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, cons
     }
     std::vector<int> hist((size_t)ny, 0);
     const float road = ground ? (float)rv : 0.0f;
-    if (int rc = lbm_revoxelize_device(h, ub, mbox, d_mask, d_wv, road, h->d_rvwork, res, hist.data())) return rc;
+    h->ground = !!ground;
+    if (int rc = lbm_revoxelize_device(h, ub, mbox, d_mask, d_wv, h->ground, road, h->d_rvwork, res, hist.data()))
+        return rc;
     // js/lbm.js:240-246: projections and the median gap under the body
     const int64_t cells = res[RV_CELLS];
     h->body.frontalArea = std::max<int64_t>(1, res[RV_FRONTAL]);
@@ -644,8 +645,8 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
         if (hipMemcpyAsync(d_grid, grid, gbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "grid upload failed");
         if (!rc) rc = lbm_grid_mask(h, d_grid, vm, M, pos, scale, box, d_mask);
-        // the road's solid state as it is: the whole update on the device
-        // (env WT_VOXELIZE_HOST=1 keeps the host bookkeeping)
+        // the whole update on the device (env WT_VOXELIZE_HOST=1 keeps the
+        // host bookkeeping)
         if (!rc && voxelize_device_applies(h, ground)) return voxelize_device(h, box, d_mask, nullptr, ground, roadVel, out);
         if (!rc && hipMemcpyAsync(mask.data(), d_mask, (size_t)cnt, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "mask download failed");

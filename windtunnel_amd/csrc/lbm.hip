@@ -1350,6 +1350,7 @@ struct RevoxArgs {
     int ux, uy, uz, uw, uh, ud;       // union box origin and extents
     int mx0, my0, mz0, mw, mh, md;    // mask box origin and extents (md 0: no mask)
     float road;                       // the road row's x velocity (ground), else 0
+    int ground;                       // the road row is wall
     int *res;  // REVOX_* slots
     uint8_t *col, *plan;
     int *low, *hist;
@@ -1465,8 +1466,8 @@ __global__ void __launch_bounds__(256) k_revox(RevoxArgs a)
     revox_reduce(a, v);
 }
 
-// The road row (y = 0, js/lbm.js:204-214): its wall velocity (a.road, 0, 0)
-// wherever it differs.
+// The road row (y = 0, js/lbm.js:204-214): wall (a.ground) or fluid (re-seeded:
+// op 1), and its wall velocity (a.road, 0, 0), wherever they differ.
 __global__ void __launch_bounds__(256) k_revox_road(RevoxArgs a)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1474,7 +1475,14 @@ __global__ void __launch_bounds__(256) k_revox_road(RevoxArgs a)
     revox_init(v);
     if (t < (int64_t)a.nx * a.nz) {
         const int x = (int)(t % a.nx), z = (int)(t / a.nx);
-        revox_set_uw(a, x + (int64_t)a.nx * a.ny * z, x, 0, z, a.road, 0.0f, 0.0f, v);
+        const int64_t i = x + (int64_t)a.nx * a.ny * z;
+        const uint8_t gy = a.ground ? (uint8_t)SOLID_WALL : (uint8_t)FLUID;
+        if (a.solid[i] != gy) {
+            a.solid[i] = gy;
+            if (gy == FLUID) a.ops[i] = 1;
+            v[1] = x; v[2] = 0; v[3] = z; v[4] = x; v[5] = 0; v[6] = z;
+        }
+        revox_set_uw(a, i, x, 0, z, a.road, 0.0f, 0.0f, v);
     }
     revox_reduce(a, v);
 }
@@ -1513,7 +1521,7 @@ __global__ void __launch_bounds__(256) k_gather_box(const E *src, E *dst, int bx
 }
 
 int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], const uint8_t *d_mask, const float *d_wv,
-                          float road, void *d_work, int *res, int *hist)
+                          int ground, float road, void *d_work, int *res, int *hist)
 {
     const int nx = h->nx, ny = h->ny, nz = h->nz;
     uint8_t *col = (uint8_t *)d_work, *plan = col + ((size_t)ny * nz + 255) / 256 * 256;
@@ -1536,6 +1544,7 @@ int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], c
     a.mh = mask_empty ? 0 : mbox[4] - mbox[1] + 1;
     a.md = mask_empty ? 0 : mbox[5] - mbox[2] + 1;
     a.road = road;
+    a.ground = ground;
     a.res = dres; a.col = col; a.plan = plan; a.low = low; a.hist = dhist;
     if (a.uw > 0 && a.uh > 0 && a.ud > 0) {
         const int64_t cnt = (int64_t)a.uw * a.uh * a.ud;

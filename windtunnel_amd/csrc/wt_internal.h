@@ -185,10 +185,10 @@ int lbm_rebuild_masks_range(wt_lattice *h, int z0, int z1);
 // x, y, z order at src (elem bytes each, elem 1 or 4) written into the
 // lattice-shaped array dst
 int lbm_scatter_box(wt_lattice *h, const void *src, void *dst, int elem, const int box[6]);
-// wt_voxelize from a body mask already on the device (capi.hip): applies
-// when the road's solid state stays as it is, and then gives wt_voxelize's
-// result on the same mask. d_wv: 3 floats per mask cell (its mask-2 cells'
-// wall velocities), or null.
+// wt_voxelize from a body mask already on the device (capi.hip), with
+// wt_voxelize's result on the same mask. d_wv: 3 floats per mask cell (its
+// mask-2 cells' wall velocities), or null. voxelize_device_applies: false
+// only when the env WT_VOXELIZE_HOST asks for the host bookkeeping.
 bool voxelize_device_applies(const wt_lattice *h, int ground);
 int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, const float *d_wv, int ground,
                     double roadVel, wt_body_stats *out);
@@ -199,9 +199,9 @@ int lbm_gather_box(wt_lattice *h, const void *src, void *dst, int elem, const in
 // of nonzero wall velocities, changed wall-velocity box (6)
 enum { RV_CELLS = 0, RV_CB = 1, RV_NB = 7, RV_FRONTAL = 13, RV_PLAN = 14, RV_WHEELS = 15, RV_NZ = 16, RV_UB = 17,
        RV_N = 24 };
-// Re-voxelize on the device from a device mask over mbox (the road's solid
-// state unchanged): ubox is the union of the previous body's box and mbox;
-// d_wv as for voxelize_device; road: the road row's x wall velocity. d_work
+// Re-voxelize on the device from a device mask over mbox: ubox is the union
+// of the previous body's box and mbox; d_wv as for voxelize_device; ground:
+// the road row is wall; road: its x wall velocity. d_work
 // holds ny*nz and nx*nz bytes (each rounded up to 256) and then nx*nz + 24 +
 // ny ints. res (24 ints, in/out; the RV_* slots of lbm.hip: body cells,
 // changed box, new body box, frontal and planform areas, wheel cells, change
@@ -209,7 +209,7 @@ enum { RV_CELLS = 0, RV_CB = 1, RV_NB = 7, RV_FRONTAL = 13, RV_PLAN = 14, RV_WHE
 // (ny ints: (x, z) columns whose lowest body cell is at y = g + 1) come back
 // on the host.
 int lbm_revoxelize_device(wt_lattice *h, const int ubox[6], const int mbox[6], const uint8_t *d_mask, const float *d_wv,
-                          float road, void *d_work, int *res, int *hist);
+                          int ground, float road, void *d_work, int *res, int *hist);
 int lbm_channel(wt_lattice *h, bool macro);
 // Launches one step over z rows [bz0,bz1). Its force (when force_out is
 // given) is reduced into force_out by the NEXT step launch or by
