@@ -196,7 +196,7 @@ extern "C" int wt_voxelize_parts(wt_lattice *h, const wt_part *parts, int nparts
             rc = wt_fail(WT_E_DEVICE, "wheel upload failed");
         if (!rc) hipLaunchKernelGGL(k_parts_mask, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, a, d_mask, d_wv);
         if (!rc && hipGetLastError() != hipSuccess) rc = wt_fail(WT_E_DEVICE, "body kernel launch failed");
-        if (!rc && voxelize_device_applies(h, ground)) {
+        if (!rc && voxelize_device_applies()) {
             // the whole update on the device
             rc = voxelize_device(h, box, d_mask, a.nwheels ? d_wv : nullptr, ground, roadVel, out);
             (void)hipFree(d_mask);

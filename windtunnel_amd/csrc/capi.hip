@@ -294,7 +294,7 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
     if (!empty && !wv) {
         const int64_t cnt = (int64_t)(x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1);
         if (std::memchr(mask, 2, (size_t)cnt)) return wt_fail(WT_E_ARG, "mask holds wall-velocity cells but wv is null");
-        if (voxelize_device_applies(h, ground)) {
+        if (voxelize_device_applies()) {
             // the mask goes up and the whole update runs on the device
             if ((size_t)cnt > h->vgrid_cap) {
                 if (h->d_vgrid) (void)hipFree(h->d_vgrid);
@@ -507,7 +507,7 @@ int wt_voxelize(wt_lattice *h, const int box[6], const uint8_t *mask, const floa
 // device, and only a few scalars, a histogram and the changed boxes of the
 // solid map and wall velocities (for the host mirrors) come back. The result
 // equals wt_voxelize's on the same mask.
-bool voxelize_device_applies(const wt_lattice *, int)
+bool voxelize_device_applies()
 {
     return !std::getenv("WT_VOXELIZE_HOST");
 }
@@ -647,7 +647,7 @@ int wt_voxelize_grid(wt_lattice *h, const uint8_t *grid, const wt_voxmeta *vm, c
         if (!rc) rc = lbm_grid_mask(h, d_grid, vm, M, pos, scale, box, d_mask);
         // the whole update on the device (env WT_VOXELIZE_HOST=1 keeps the
         // host bookkeeping)
-        if (!rc && voxelize_device_applies(h, ground)) return voxelize_device(h, box, d_mask, nullptr, ground, roadVel, out);
+        if (!rc && voxelize_device_applies()) return voxelize_device(h, box, d_mask, nullptr, ground, roadVel, out);
         if (!rc && hipMemcpyAsync(mask.data(), d_mask, (size_t)cnt, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
             rc = wt_fail(WT_E_DEVICE, "mask download failed");
         if (!rc && hipStreamSynchronize(h->stream) != hipSuccess) rc = wt_fail(WT_E_DEVICE, "grid lookup failed");

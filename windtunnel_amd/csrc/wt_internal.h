@@ -189,7 +189,7 @@ int lbm_scatter_box(wt_lattice *h, const void *src, void *dst, int elem, const i
 // wt_voxelize's result on the same mask. d_wv: 3 floats per mask cell (its
 // mask-2 cells' wall velocities), or null. voxelize_device_applies: false
 // only when the env WT_VOXELIZE_HOST asks for the host bookkeeping.
-bool voxelize_device_applies(const wt_lattice *h, int ground);
+bool voxelize_device_applies();
 int voxelize_device(wt_lattice *h, const int box[6], const uint8_t *d_mask, const float *d_wv, int ground,
                     double roadVel, wt_body_stats *out);
 // the reverse: box cells of the lattice-shaped byte array src packed into dst
