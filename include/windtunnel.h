@@ -310,6 +310,12 @@ int wt_adjoint_forward(wt_lattice *h, int nsteps, int interval, double *forces);
 int wt_adjoint_backward(wt_lattice *h, int nsteps, const double *Fbar, double *dJdw, double *reverse_ms);
 int wt_adjoint_release(wt_lattice *h);
 
+/* Identity of the recorded forward pass (no reference counterpart): every
+ * wt_adjoint_forward on a lattice gets the next id (1, 2, ...), so a caller
+ * that holds several graphs can tell that its record was replaced by a later
+ * forward before it calls wt_adjoint_backward. WT_E_STATE when none. */
+int wt_adjoint_record(const wt_lattice *h, int64_t *id);
+
 /* Voxel.bounds (js/voxel.js:32-42) over len floats. */
 int wt_voxel_bounds(const float *P, int64_t len, double out[6]);
 

@@ -18,19 +18,21 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libwt_oracle.so")
 
-_lib = None
+_libs = {}
 
 
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(path: str | None = None):
+    """The oracle library (or a variant of it built elsewhere, e.g. with a
+    mistake planted by a test)."""
+    key = path or LIB_PATH
+    if key not in _libs:
+        if path is None and not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(key)
         P, I, D, V = C.c_void_p, C.c_int, C.c_double, None
         sig = {
             "wto_create": (P, [I, I, I, I]),
@@ -60,8 +62,8 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+        _libs[key] = L
+    return _libs[key]
 
 
 def _ptr(a: np.ndarray):
@@ -75,10 +77,11 @@ ARRAYS = {"f": 0, "g": 1, "solid": 2, "near": 3, "ux": 4, "uy": 5, "uz": 6, "rho
 class OracleLBM:
     """The reference ``LBM`` object restated on the CPU (js/lbm.js:51-546)."""
 
-    def __init__(self, nx: int, ny: int, nz: int, prec: int = 4):
+    def __init__(self, nx: int, ny: int, nz: int, prec: int = 4, libpath: str | None = None):
         self.nx, self.ny, self.nz, self.prec = nx, ny, nz, prec
         self.n = nx * ny * nz
-        self._h = lib().wto_create(nx, ny, nz, prec)
+        self._L = lib(libpath)
+        self._h = self._L.wto_create(nx, ny, nz, prec)
         if not self._h:
             raise ValueError("bad lattice dimensions")
         self._params = dict(tau=0.5035, les=True, csm=0.14, trt=True, **{"lambda": 0.25},
@@ -86,7 +89,7 @@ class OracleLBM:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().wto_destroy(self._h)
+            self._L.wto_destroy(self._h)
             self._h = None
 
     # -- parameters -----------------------------------------------------
@@ -96,7 +99,7 @@ class OracleLBM:
             if k not in p:
                 raise KeyError(k)
             p[k] = v
-        lib().wto_set_params(self._h, float(p["tau"]), int(bool(p["les"])), float(p["csm"]),
+        self._L.wto_set_params(self._h, float(p["tau"]), int(bool(p["les"])), float(p["csm"]),
                              int(bool(p["trt"])), float(p["lambda"]), float(p["u0"]),
                              int(bool(p["movingWalls"])))
 
@@ -108,12 +111,12 @@ class OracleLBM:
         equilibrium outflow; applied after the reference boundaries)."""
         if mode not in ("reference", "channel"):
             raise ValueError(mode)
-        lib().wto_set_channel(self._h, int(mode == "channel"))
+        self._L.wto_set_channel(self._h, int(mode == "channel"))
 
     # -- state ----------------------------------------------------------
     def array(self, name: str) -> np.ndarray:
         which = ARRAYS[name]
-        p = lib().wto_array(self._h, which)
+        p = self._L.wto_array(self._h, which)
         if name in ("solid", "near"):
             dt, cnt = np.uint8, self.n
         elif name.startswith("uw"):
@@ -128,7 +131,7 @@ class OracleLBM:
     def state(self) -> dict:
         s = np.zeros(8, dtype=np.int64)
         f = np.zeros(3)
-        lib().wto_state(self._h, _ptr(s), _ptr(f))
+        self._L.wto_state(self._h, _ptr(s), _ptr(f))
         keys = ["parity", "steps", "frontalArea", "planformArea", "solidCells", "rideHeight",
                 "movingWalls", "ground"]
         d = {k: int(v) for k, v in zip(keys, s)}
@@ -137,14 +140,14 @@ class OracleLBM:
 
     # -- operations ------------------------------------------------------
     def reset(self):
-        lib().wto_reset(self._h)
+        self._L.wto_reset(self._h)
 
     def voxelize_mask(self, box, mask, wv=None, ground=False, roadVel=0.0) -> dict:
         box = np.ascontiguousarray(box, dtype=np.int32)
         mask = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
         wv = np.ascontiguousarray(wv if wv is not None else np.zeros(0), dtype=np.float32)
         stats = np.zeros(5, dtype=np.int64)
-        lib().wto_voxelize(self._h, _ptr(box), _ptr(mask), _ptr(wv), int(bool(ground)),
+        self._L.wto_voxelize(self._h, _ptr(box), _ptr(mask), _ptr(wv), int(bool(ground)),
                            float(roadVel), _ptr(stats))
         self._params["movingWalls"] = bool(stats[4])
         return dict(frontalArea=int(stats[0]), planformArea=int(stats[1]),
@@ -152,28 +155,28 @@ class OracleLBM:
 
     def step(self, n: int = 1) -> np.ndarray:
         f = np.zeros((n, 3))
-        lib().wto_step(self._h, n, _ptr(f))
+        self._L.wto_step(self._h, n, _ptr(f))
         return f
 
     def step_pool(self, n: int, nworkers: int, threaded: bool = False) -> np.ndarray:
         f = np.zeros((n, 3))
-        rc = lib().wto_step_pool(self._h, n, nworkers, int(threaded), _ptr(f))
+        rc = self._L.wto_step_pool(self._h, n, nworkers, int(threaded), _ptr(f))
         if rc:
             raise ValueError("bad worker count")
         return f
 
     def sample(self, x, y, z):
         out = np.zeros(3)
-        lib().wto_sample(self._h, float(x), float(y), float(z), _ptr(out))
+        self._L.wto_sample(self._h, float(x), float(y), float(z), _ptr(out))
         return out
 
     def set_gray(self, w):
         """Gray bounce-back weights per cell (0 fluid, (0,1) gray); None clears."""
         if w is None:
-            rc = lib().wto_set_gray(self._h, None)
+            rc = self._L.wto_set_gray(self._h, None)
         else:
             self._w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
-            rc = lib().wto_set_gray(self._h, _ptr(self._w))
+            rc = self._L.wto_set_gray(self._h, _ptr(self._w))
         if rc:
             raise ValueError("gray weight >= 1 on a fluid cell")
 
@@ -182,13 +185,13 @@ class OracleLBM:
         Fbar = np.ascontiguousarray(Fbar, dtype=np.float64).reshape(nsteps, 3)
         g = np.zeros(self.n)
         f = np.zeros((nsteps, 3))
-        rc = lib().wto_adjoint(self._h, int(nsteps), _ptr(Fbar), _ptr(g), _ptr(f))
+        rc = self._L.wto_adjoint(self._h, int(nsteps), _ptr(Fbar), _ptr(g), _ptr(f))
         if rc:
             raise ValueError("adjoint needs FP64 storage and no moving walls")
         return f, g
 
     def is_solid(self, x, y, z) -> bool:
-        return bool(lib().wto_is_solid(self._h, float(x), float(y), float(z)))
+        return bool(self._L.wto_is_solid(self._h, float(x), float(y), float(z)))
 
 
 class VoxMeta(C.Structure):

@@ -19,6 +19,13 @@
  */
 #include <math.h>
 #include <pthread.h>
+
+/* Mistakes a test plants on purpose to show that the gray bounce-back
+ * checks catch them (1: force sign, 2: one-sided link weight, 3: mixing
+ * weights swapped); 0, the only value the Makefile builds, plants none. */
+#ifndef WTO_PLANT_GRAY
+#define WTO_PLANT_GRAY 0
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -95,6 +102,7 @@ static void rebuild_near(wto_lbm *L)
             long i = 1 + (long)nx * (y + (long)ny * z);
             for (int x = 1; x < nx - 1; x++, i++) {
                 if (L->solid[i]) continue;
+                if (L->wgt && L->wgt[i] > 0) { L->near[i] = 1; continue; }  /* gray cell: every link is gray */
                 for (int q = 1; q < 19; q++) {
                     const long j = i - L->off[q];
                     if (L->solid[j] || (L->wgt && L->wgt[j] > 0)) { L->near[i] = 1; break; }
@@ -566,12 +574,18 @@ void wto_grid_mask(const uint8_t *grid, const int gdim[3], const double gmin[3],
 
 /* ===================================================== gray bounce-back
  *
- * No reference counterpart (an addition of this library): a cell j with weight
- * w_j in (0,1) stays a fluid cell, but every link leaving it is bounced back
- * with weight w_j at the downstream cell n = j + c_q:
- *   f_q(n) = (1 - w_j) g_q(j) + w_j g_qbar(n),   F -= w_j * 2 g_qbar(n) c_q.
- * w = 0 is plain streaming and w = 1 is a SOLID_OBJ cell (selected, not
- * blended), so 0/1 weights reduce exactly to the reference. */
+ * No reference counterpart (an addition of this library). A cell with weight
+ * w in (0,1) stays a fluid cell and every link (i, j = i - c_q) between two
+ * fluid cells transmits the fraction t = (1 - w_i)(1 - w_j) of what streams
+ * along it; the rest is bounced back halfway, as at a wall:
+ *   f_q(i) = t g_q(j) + (1 - t) g_qbar(i),   F -= (1 - t) 2 g_qbar(i) c_q.
+ * Cell j takes the same t for the opposite link, so each pair of links
+ * conserves mass exactly, and F is exactly the momentum the fluid loses
+ * (partial bounce-back on links; the transmission of a link is the product
+ * of what its two end cells let through). w = 0 on both ends is plain
+ * streaming and w = 1 is a SOLID_OBJ cell (selected, not blended), so 0/1
+ * weights reduce exactly to the reference. In a uniform gray medium the
+ * momentum of a uniform flow decays by the factor 2t - 1 per step. */
 
 /* Weights per cell; 0 or NaN = fluid, (0,1) = gray. Cells that are solid keep
  * their solid type; a weight >= 1 on a fluid cell is rejected (voxelize it
@@ -604,15 +618,23 @@ static int computed(const wto_lbm *L, long i)
     return x >= 1 && x <= L->nx - 2 && y >= 1 && y <= L->ny - 2 && z >= 1 && z <= L->nz - 2 && !L->solid[i];
 }
 
-static double weight_eff(const wto_lbm *L, long j)
+/* gray weight of cell m (0 off the gray cells) */
+static double wcell(const wto_lbm *L, long m)
 {
-    if (L->solid[j]) return 1;
-    return L->wgt ? (double)L->wgt[j] : 0;
+    return L->wgt ? (double)L->wgt[m] : 0;
 }
 
-static int force_link(const wto_lbm *L, long j)
+/* the link (i <- j) is gray: neither end solid, one of them gray (the rest
+ * population, j = i, is no link) */
+static int gray_link(const wto_lbm *L, long i, long j)
 {
-    return L->solid[j] == SOLID_OBJ || (!L->solid[j] && L->wgt && L->wgt[j] > 0);
+    return i != j && !L->solid[j] && L->wgt && (L->wgt[i] > 0 || L->wgt[j] > 0);
+}
+
+/* transmission of the link (i <- j) */
+static double trans(const wto_lbm *L, long i, long j)
+{
+    return (1 - wcell(L, i)) * (1 - wcell(L, j));
 }
 
 /* pre-collision populations of computed cell i from state S (double storage) */
@@ -621,10 +643,11 @@ static void gather_f64(const wto_lbm *L, const double *S, long i, double f[19])
     const long n = L->n;
     for (int q = 0; q < 19; q++) {
         const long j = i - L->off[q];
-        const double w = weight_eff(L, j);
-        if (w == 1) f[q] = S[OPP[q] * n + i];
-        else if (w > 0) f[q] = (1 - w) * S[q * n + j] + w * S[OPP[q] * n + i];
-        else f[q] = S[q * n + j];
+        if (L->solid[j]) f[q] = S[OPP[q] * n + i];
+        else if (gray_link(L, i, j)) {
+            const double t = trans(L, i, j);
+            f[q] = t * S[q * n + j] + (1 - t) * S[OPP[q] * n + i];
+        } else f[q] = S[q * n + j];
     }
 }
 
@@ -710,7 +733,14 @@ void wto_collide_adj(const double f[19], const double lam[19], double tau0, doub
 
 /* Run N steps from the current state and return dJ/dw for every gray cell
  * (0 elsewhere). Fbar[3k..3k+2] = dJ/dF_k. forces (may be NULL) gets F_k.
- * FP64 storage only; moving walls unsupported. */
+ * FP64 storage only; moving walls unsupported.
+ *
+ * mu = dJ/d(pre-collision populations of step k), lam = dJ/d(state s_k that
+ * step k pulls from). A gray link (i <- j) along q contributes to dJ/dW of
+ * its bounced fraction W = 1 - t
+ *   G = mu_q(i) (s_qbar(i) - s_q(j)) - 2 s_qbar(i) c_q . Fbar_k,
+ * and dW/dw_i = 1 - w_j, dW/dw_j = 1 - w_i: a gray cell collects G of the 18
+ * links it receives on and of the 18 links it sends on. */
 int wto_adjoint(wto_lbm *L, int N, const double *Fbar, double *dJdw, double *forces)
 {
     if (L->prec != 8 || L->movingWalls || N < 1) return -1;
@@ -728,35 +758,45 @@ int wto_adjoint(wto_lbm *L, int N, const double *Fbar, double *dJdw, double *for
         const double *Sk = S + (long)k * sz, *Fb = Fbar + 3 * k;
         for (long m = 0; m < n; m++) {
             const int cm = computed(L, m);
-            const double wm = weight_eff(L, m);
             for (int q = 0; q < 19; q++) {
                 double v = 0;
                 const long n1 = m + L->off[q];
-                if (n1 >= 0 && n1 < n && computed(L, n1) && wm < 1) v += (1 - wm) * mu[q * n + n1];
+                const int in = n1 >= 0 && n1 < n;
+                /* s_q(m) streams into n1 */
+                if (in && computed(L, n1) && !L->solid[m])
+                    v += (gray_link(L, n1, m) ? trans(L, n1, m) : 1) * mu[q * n + n1];
+                /* s_q(m) is bounced back at m: m pulls along qbar from j = n1 */
                 if (cm) {
-                    const long j = m + L->off[q];
-                    const double wj = weight_eff(L, j);
-                    if (wj > 0) {
-                        const int qb = OPP[q];
-                        const double cF = force_link(L, j) ? EX[qb] * Fb[0] + EY[qb] * Fb[1] + EZ[qb] * Fb[2] : 0;
-                        v += wj * (mu[qb * n + m] - 2 * cF);
-                    }
+                    const long j = n1;
+                    const int qb = OPP[q];
+                    const double cF = EX[qb] * Fb[0] + EY[qb] * Fb[1] + EZ[qb] * Fb[2];
+                    if (L->solid[j]) v += mu[qb * n + m] - (L->solid[j] == SOLID_OBJ ? 2 * cF : 0);
+                    else if (gray_link(L, m, j)) v += (1 - trans(L, m, j)) * (mu[qb * n + m] - 2 * cF);
                 }
                 lam[q * n + m] = v;
             }
         }
         if (L->wgt) {
-            for (long j = 0; j < n; j++) {
-                if (!(L->wgt[j] > 0) || L->solid[j]) continue;
-                double g = 0;
+            for (long g = 0; g < n; g++) {
+                if (!(L->wgt[g] > 0) || L->solid[g]) continue;
+                double d = 0;
                 for (int q = 0; q < 19; q++) {
-                    const long n1 = j + L->off[q];
-                    if (n1 < 0 || n1 >= n || !computed(L, n1)) continue;
-                    const double fd = Sk[OPP[q] * n + n1];
-                    g += mu[q * n + n1] * (fd - Sk[q * n + j]) -
-                         2 * fd * (EX[q] * Fb[0] + EY[q] * Fb[1] + EZ[q] * Fb[2]);
+                    const double cF = EX[q] * Fb[0] + EY[q] * Fb[1] + EZ[q] * Fb[2];
+                    if (q == 0) continue;
+                    /* g receives along q from j */
+                    const long j = g - L->off[q];
+                    if (!L->solid[j]) {
+                        const double fd = Sk[OPP[q] * n + g];
+                        d += (1 - wcell(L, j)) * (mu[q * n + g] * (fd - Sk[q * n + j]) - 2 * fd * cF);
+                    }
+                    /* n1 receives along q from g */
+                    const long n1 = g + L->off[q];
+                    if (n1 >= 0 && n1 < n && computed(L, n1)) {
+                        const double fd = Sk[OPP[q] * n + n1];
+                        d += (1 - wcell(L, n1)) * (mu[q * n + n1] * (fd - Sk[q * n + g]) - 2 * fd * cF);
+                    }
                 }
-                dJdw[j] += g;
+                dJdw[g] += d;
             }
         }
         if (k == 0) break;

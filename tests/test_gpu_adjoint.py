@@ -174,3 +174,58 @@ def test_fp16_adjoint_tracks_fp64():
     print(f"[adjoint f16] rel-L2 vs FP64: {errs}", flush=True)
     assert errs["f32"] < 1e-3
     assert errs["f16"] < 3e-3
+
+
+def test_second_graph_on_one_lattice_refuses_stale_backward():
+    """Two gray_forces calls on one lattice before backward: the lattice holds
+    only the second recorded pass, so the first graph's backward raises
+    instead of returning the second graph's gradient; the second works."""
+    import torch
+
+    from windtunnel_amd.autograd import gray_forces
+
+    N = 12
+    C = setup(LBM(*DIMS), DIMS, W, True, True)
+    w = torch.tensor(np.where(W >= 1, 0, W), dtype=torch.float32, requires_grad=True)
+    J1 = (gray_forces(w, C, N)[:, 0]).sum()
+    r1 = C.adjoint_record()
+    J2 = (gray_forces(w, C, N)[:, 0]).sum()
+    assert C.adjoint_record() == r1 + 1
+    with pytest.raises(RuntimeError, match="replaced"):
+        J1.backward()
+    J2.backward()
+    assert torch.isfinite(w.grad).all() and w.grad.abs().max() > 0
+    C.close()
+
+
+def test_weight_field_is_freed_with_the_lattice():
+    """Creating, gray-weighting and destroying lattices in a loop does not
+    leak device memory (the weight field goes with the lattice)."""
+    from windtunnel_amd.lbm import device_memory
+
+    dims = (128, 64, 64)
+    w = np.full(dims[0] * dims[1] * dims[2], 0.2, np.float32)
+
+    def cycle():
+        L = LBM(*dims)
+        L.reset()
+        L.set_gray(w)
+        L.close()
+
+    cycle()
+    free0 = device_memory()[0]
+    for _ in range(4):
+        cycle()
+    assert device_memory()[0] >= free0 - (1 << 20)
+
+
+def test_slab_count_limit_is_an_error():
+    """More slabs than the device force buffer holds per step is refused
+    before any launch."""
+    L = LBM(3, 3, 4200)
+    L.reset()
+    with pytest.raises(ValueError, match="above the limit"):
+        L.step_slabs(1, 4197)
+    f = L.step_slabs(1, 4096)
+    assert f.shape[0] == 1 and np.all(np.isfinite(f))
+    L.close()

@@ -533,6 +533,20 @@ static napi_value js_adjoint_release(napi_env env, napi_callback_info info)
     return NULL;
 }
 
+/* adjointRecord(h) -> id of the recorded forward pass */
+static napi_value js_adjoint_record(napi_env env, napi_callback_info info)
+{
+    napi_value a[1];
+    args(env, info, 1, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int64_t id = 0;
+    WT(env, wt_adjoint_record(h, &id));
+    napi_value v;
+    napi_create_double(env, (double)id, &v);
+    return v;
+}
+
 static napi_value js_set_parity(napi_env env, napi_callback_info info)
 {
     napi_value a[2];
@@ -851,7 +865,7 @@ static napi_value init(napi_env env, napi_value exports)
         {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
         {"getPops", js_get_pops}, {"setPops", js_set_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory}, {"setBoundaryMode", js_set_boundary_mode}, {"slice", js_slice}, {"sampleMany", js_sample_many}, {"meshParse", js_mesh_parse},
         {"voxelizeParts", js_voxelize_parts}, {"getWalls", js_get_walls}, {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},
-        {"adjointRelease", js_adjoint_release},
+        {"adjointRelease", js_adjoint_release}, {"adjointRecord", js_adjoint_record},
         {"voxelBounds", js_voxel_bounds}, {"voxelPlan", js_voxel_plan}, {"voxelBuild", js_voxel_build}};
     for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
         napi_value f;

@@ -33,12 +33,28 @@ def sources() -> list:
 
 
 def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
-    """Compile the library; `defines` are extra -D switches (tuning builds)."""
+    """Compile the library; `defines` are extra -D switches (tuning builds).
+    Each source compiles to an object of its own, in parallel, then one link."""
+    import concurrent.futures as cf
+    import tempfile
     tmp = out + ".tmp"
-    cmd = [HIPCC, *FLAGS, *[f"-D{d}" for d in defines], "-o", tmp, *sources()]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    dflags = [f"-D{d}" for d in defines]
+    with tempfile.TemporaryDirectory(prefix="wt_build_") as obj:
+        def compile_one(src):
+            o = os.path.join(obj, os.path.basename(src) + ".o")
+            cmd = [HIPCC, *[f for f in FLAGS if f != "-shared"], *dflags, "-c", "-o", o, src]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+            return o
+        srcs = sources()
+        workers = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+        with cf.ThreadPoolExecutor(workers) as ex:
+            objs = list(ex.map(compile_one, srcs))
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
     os.replace(tmp, out)
     return out
 

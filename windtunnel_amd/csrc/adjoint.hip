@@ -3,13 +3,16 @@
 //
 // Forward step k maps the state s_k (post-collision populations, AB layout)
 // to s_{k+1} and yields the force F_k. The objective is J = sum_k Fbar_k.F_k.
-// With mu_k the adjoint of the pre-collision populations f_k of step k and
-// lambda_k the adjoint of s_k, one reverse iteration k is ONE fused kernel:
+// A link (i <- j = i - c_q) between fluid cells transmits t_ij =
+// (1 - w_i)(1 - w_j) of the pulled value and bounces W_ij = 1 - t_ij back
+// (W = 1 from a solid cell; t = 1 between two cells of weight 0). With mu_k
+// the adjoint of the pre-collision populations f_k of step k and lambda_k
+// the adjoint of s_k, one reverse iteration k is ONE fused kernel:
 //
-//   lambda_k(m)[q] = (1 - w_m) mu_k[q](m + c_q)                 (streaming)
-//                  + w_j (mu_k[qbar](m) - 2 c_qbar . Fbar_k)    (bounce-back, j = m + c_q)
-//   dJ/dw_m       += sum_q mu_k[q](m + c_q) (s_k[qbar](m + c_q) - s_k[q](m))
-//                                          - 2 s_k[qbar](m + c_q) c_q . Fbar_k     (gray m)
+//   lambda_k(m)[q] = t_{m+c_q,m} mu_k[q](m + c_q)                     (streaming)
+//                  + W_{m,j} (mu_k[qbar](m) - 2 c_qbar . Fbar_k)       (bounce-back, j = m + c_q)
+//   dJ/dw_m       += sum over the links (i <- j) with m = i or m = j of
+//                    (1 - w_other) [mu_k[q](i) (s_k[qbar](i) - s_k[q](j)) - 2 s_k[qbar](i) c_q . Fbar_k]
 //   mu_{k-1}(m)    = (d collide / d f)^T at f_{k-1}(m), applied to lambda_k(m)
 //                    (plus lambda_k of the outlet cell for x = nx-2: the outlet
 //                    copies that cell, js/lbm.js:504-511)
@@ -163,7 +166,6 @@ template <class S, class M, class T, bool LES, bool TRT>
 __device__ __forceinline__ void adj_cold(const AdjArgs<S, M> &a, int x, int y, int z)
 {
     const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
-    const bool nr = bitset(a.nbits, a.nwx, a.ny, x, y, z);
     const float wi = a.wgt ? a.wgt[i] : 0.0f;
     T mun[Q], muh[Q];  // mu_k at i + c_q, and at i itself
     int sj[Q];         // solid state of i + c_q
@@ -176,36 +178,44 @@ __device__ __forceinline__ void adj_cold(const AdjArgs<S, M> &a, int x, int y, i
         sj[q] = a.solid[n1];
         wj[q] = a.wgt ? a.wgt[n1] : 0.0f;
     }
-    // lambda_k(i): gray-aware streaming, plus bounce-back links of a near cell
+    // lambda_k(i): gray-aware streaming, plus the bounce-back links of i
+    // (from j = i + c_q along qbar)
     T lam[Q];
-    const T wm = (T)wi;
+    const T ti = T(1) - (T)wi;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        T v = (T(1) - wm) * mun[q];
-        const double w = sj[q] ? 1.0 : (double)wj[q];
-        if (nr && w > 0) {
+        const bool gl = q && !sj[q] && (wi > 0.0f || wj[q] > 0.0f);  // gray link either way
+        const T t = gl ? ti * (T(1) - (T)wj[q]) : T(1);
+        T v = sj[q] ? T(0) : t * mun[q];
+        if (q && (sj[q] || gl)) {
             const int qb = OPP[q];
-            const bool flink = sj[q] == wt::SOLID_OBJ || (sj[q] == 0 && wj[q] > 0.0f);
+            const bool flink = sj[q] == wt::SOLID_OBJ || gl;
             const double cF = flink ? EX[qb] * a.Fb[0] + EY[qb] * a.Fb[1] + EZ[qb] * a.Fb[2] : 0.0;
-            v += (T)w * (muh[qb] - T(2) * (T)cF);
+            v += (sj[q] ? T(1) : T(1) - t) * (muh[qb] - T(2) * (T)cF);
         }
         lam[q] = v;
     }
     if (wi > 0.0f) {
-        // dJ/dw_i over the links leaving i into computed cells
-        double sk[Q], fdn[Q];
+        // dJ/dw_i over the 18 links i receives on (from i - c_q) and the 18
+        // it sends on (into i + c_q)
+        double sk[Q], fdn[Q], skj[Q];
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             sk[q] = sval<double>(a.Sk, pix(q, i, a.lay));
             fdn[q] = sval<double>(a.Sk, pix(OPP[q], i + off_of(q, a.nx, a.ny), a.lay));
+            skj[q] = sval<double>(a.Sk, pix(q, i - off_of(q, a.nx, a.ny), a.lay));
         }
         double g = 0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) {
+        for (int q = 1; q < Q; q++) {
+            const int qb = OPP[q];
+            const double cF = EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2];
+            // i receives along q from j = i - c_q = i + c_qbar
+            if (!sj[qb]) g += (1.0 - (double)wj[qb]) * ((double)muh[q] * (sk[qb] - skj[q]) - 2 * sk[qb] * cF);
+            // n1 = i + c_q receives along q from i
             const int X = x + EX[q], Y = y + EY[q], Z = z + EZ[q];
             const bool comp = X >= 1 && X <= a.nx - 2 && Y >= 1 && Y <= a.ny - 2 && Z >= 1 && Z <= a.nz - 2 && !sj[q];
-            if (comp)
-                g += (double)mun[q] * (fdn[q] - sk[q]) - 2 * fdn[q] * (EX[q] * a.Fb[0] + EY[q] * a.Fb[1] + EZ[q] * a.Fb[2]);
+            if (comp) g += (1.0 - (double)wj[q]) * ((double)mun[q] * (fdn[q] - sk[q]) - 2 * fdn[q] * cF);
         }
         a.dJdw[i] += g;
     }
@@ -227,10 +237,14 @@ __device__ __forceinline__ void adj_cold(const AdjArgs<S, M> &a, int x, int y, i
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int qb = OPP[q];
-        const double w = !nr ? 0.0 : (sj[qb] ? 1.0 : (double)wj[qb]);
-        if (w == 1) f[q] = sph[qb];
-        else if (w > 0) f[q] = (T(1) - (T)w) * spj[q] + (T)w * sph[qb];
-        else f[q] = spj[q];
+        if (q && sj[qb]) {
+            f[q] = sph[qb];
+        } else if (q && (wi > 0.0f || wj[qb] > 0.0f)) {
+            const T t = ti * (T(1) - (T)wj[qb]);
+            f[q] = t * spj[q] + (T(1) - t) * sph[qb];
+        } else {
+            f[q] = spj[q];
+        }
     }
     T mu[Q];
     adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
@@ -340,6 +354,7 @@ struct AdjRecord {
     size_t st = 0;             // bytes of one stored state
     int storage = 0, arith = 0;
     uint64_t epoch = 0;        // lattice configuration epoch at record time
+    int64_t id = 0;            // wt_adjoint_record
     void *ck = nullptr;        // nseg states: ck + seg*st = s_{seg*K - 1} (s_0 for seg 0)
     std::vector<double> forces;
     ~AdjRecord() { if (ck) (void)hipFree(ck); }
@@ -385,6 +400,7 @@ int record_forward(wt_lattice *h, int N, int K, double *forces)
     if (forces) std::copy(R->forces.begin(), R->forces.end(), forces);
     for (int k = 0; k < 3; k++) h->force[k] = R->forces[3 * (N - 1) + k];
     adj_release(h);
+    R->id = ++h->adj_seq;
     h->adj = R;
     return 0;
 }
@@ -612,6 +628,15 @@ int wt_adjoint_backward(wt_lattice *h, int nsteps, const double *Fbar, double *d
     if (h->storage == WT_STORE_F16) return reverse_t<__half, float, float>(h, *R, Fbar, dJdw, reverse_ms);
     if (h->arith == WT_ARITH_F64) return reverse_t<float, float, double>(h, *R, Fbar, dJdw, reverse_ms);
     return reverse_t<float, float, float>(h, *R, Fbar, dJdw, reverse_ms);
+}
+
+int wt_adjoint_record(const wt_lattice *h, int64_t *id)
+{
+    if (!h || !id) return wt_fail(WT_E_ARG, "null argument");
+    const auto *R = (const AdjRecord *)h->adj;
+    if (!R) return wt_fail(WT_E_STATE, "no recorded forward pass");
+    *id = R->id;
+    return 0;
 }
 
 int wt_adjoint_release(wt_lattice *h)

@@ -74,7 +74,7 @@ void release(wt_lattice *h)
                     (void *)h->d_nbits, (void *)h->d_uw, (void *)h->d_mac, (void *)h->d_partial,
                     (void *)h->d_forces, (void *)h->d_near, (void *)h->d_ring,
                     (void *)h->d_prod, (void *)h->d_ops, h->d_stage, (void *)h->d_vgrid,
-                    (void *)h->d_near2, (void *)h->d_pcnt, h->d_rvwork})
+                    (void *)h->d_near2, (void *)h->d_pcnt, h->d_rvwork, (void *)h->d_wgt})
         if (p) (void)hipFree(p);
     if (h->h_stage) (void)hipHostFree(h->h_stage);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
@@ -661,6 +661,9 @@ static int run_steps(wt_lattice *h, int nsteps, int nslabs, double *forces)
     WT_CHECK_H(h);
     if (nsteps < 0) return wt_fail(WT_E_ARG, "negative step count");
     if (nslabs < 1 || nslabs > h->nz - 2) return wt_fail(WT_E_ARG, "slab count %d out of range", nslabs);
+    // one step's slab forces must fit the device force buffer
+    if (nslabs > h->forces_cap)
+        return wt_fail(WT_E_ARG, "slab count %d above the limit of %d slabs", nslabs, h->forces_cap);
     const int64_t fs = fault_step();
     if (fs >= 0 && h->steps <= fs && fs < h->steps + nsteps)
         return wt_fail(WT_E_FAULT, "injected fault at step %lld", (long long)fs);

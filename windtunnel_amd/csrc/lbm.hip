@@ -414,9 +414,10 @@ __device__ __forceinline__ void reduce_partials(const double *p, int64_t count, 
 #endif
 template <class S> constexpr int step_waves() { return std::is_same<S, __half>::value ? WT_F16_WAVES : WT_WAVES; }
 
-// One near-wall cell (an interior fluid cell with a solid or gray upstream
-// neighbour): halfway bounce-back, moving wall, momentum exchange and gray
-// links, then the collision of the straight path.
+// One near-wall cell (an interior fluid cell with a solid upstream
+// neighbour, a gray cell or a neighbour of one): halfway bounce-back, moving
+// wall, momentum exchange and gray links, then the collision of the
+// straight path.
 template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO, bool WIDE>
 __device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, double &fx, double &fy, double &fz)
 {
@@ -437,7 +438,11 @@ __device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, 
     };
 
     // near-wall cell: halfway bounce-back, moving wall, momentum
-    // exchange (js/lbm.js:334-368)
+    // exchange (js/lbm.js:334-368). Gray links (no reference counterpart):
+    // the link from a fluid cell j transmits t = (1 - w_i)(1 - w_j) of the
+    // pulled value and bounces the rest back, F -= (1 - t) 2 g_qbar(i) c_q;
+    // j applies the same t to the opposite link, so the pair conserves mass.
+    const float wi = (!ESO && a.wgt) ? a.wgt[i] : 0.0f;
     T f[Q];
     if constexpr (!WIDE) {
 #pragma unroll
@@ -460,12 +465,11 @@ __device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, 
                     const double d = (double)(T(2) * fd + corr);
                     fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
                 }
-            } else if (!ESO && a.wgt && a.wgt[j] > 0.0f) {
-                // gray upstream cell: a fraction w_j of the link bounces back
-                const T wj = (T)a.wgt[j];
+            } else if (q && !ESO && a.wgt && (wi > 0.0f || a.wgt[j] > 0.0f)) {
+                const T t = (T(1) - (T)wi) * (T(1) - (T)a.wgt[j]);
                 const T fd = P::template ld<T>(a.F, pix(OPP[q], i, a.lay));
-                f[q] = (T(1) - wj) * P::template ld<T>(a.F, pix(q, j, a.lay)) + wj * fd;
-                const double d = (double)(wj * (T(2) * fd));
+                f[q] = t * P::template ld<T>(a.F, pix(q, j, a.lay)) + (T(1) - t) * fd;
+                const double d = (double)((T(1) - t) * (T(2) * fd));
                 fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
             } else {
                 const int64_t k = ESO ? slot_of(q, j, ph, a.lay, nx, ny) : pix(q, j, a.lay);
@@ -499,12 +503,12 @@ __device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, 
                     corr = T(6) * wq * (T)a.rho[i] * cu;
                 }
             }
-            const bool gray = !sj[q] && wj[q] > 0.0f;
-            const T w = (T)wj[q];
-            const T mix = (T(1) - w) * fp[q] + w * fb[q];
+            const bool gray = q && !sj[q] && (wi > 0.0f || wj[q] > 0.0f);
+            const T t = (T(1) - (T)wi) * (T(1) - (T)wj[q]);
+            const T mix = t * fp[q] + (T(1) - t) * fb[q];
             f[q] = sj[q] ? fb[q] + corr : (gray ? mix : fp[q]);
             const double d = sj[q] == SOLID_OBJ ? (double)(T(2) * fb[q] + corr)
-                                                : (gray ? (double)(w * (T(2) * fb[q])) : 0.0);
+                                                : (gray ? (double)((T(1) - t) * (T(2) * fb[q])) : 0.0);
             fx -= d * EX[q]; fy -= d * EY[q]; fz -= d * EZ[q];
         }
     }
@@ -692,8 +696,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
                 }
             }
         }
-        // near-wall cells (interior fluid with a solid or gray upstream
-        // neighbour) are stepped by the near-wall blocks
+        // near-wall cells (interior fluid with a solid upstream neighbour,
+        // gray cells and their neighbours) are stepped by the near-wall blocks
     }
 }
 
@@ -919,7 +923,8 @@ __global__ void __launch_bounds__(256) k_read_pops(const S *B, const uint8_t *so
 }
 
 // Bit-packed solid mask and near-wall mask (js/lbm.js:268-282): near = an
-// interior fluid cell with a solid cell among its 18 upstream neighbours.
+// interior fluid cell with a solid cell among its 18 upstream neighbours
+// (or, with gray weights, a gray cell or one next to a gray cell).
 __global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, const float *wgt, uint32_t *sbits,
                                                uint32_t *nbits, int nx, int ny, int nz, int nwx, int64_t w0,
                                                int64_t w1)
@@ -937,6 +942,7 @@ __global__ void __launch_bounds__(256) k_masks(const uint8_t *solid, const float
         const int64_t i = x + (int64_t)nx * (y + (int64_t)ny * z);
         s = solid[i] != 0;
         if (!s && x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && z >= 1 && z <= nz - 2) {
+            near = wgt && wgt[i] > 0.0f;  // a gray cell: every link of it is gray
 #pragma unroll
             for (int q = 1; q < Q; q++) {
                 const int64_t j = i - cell_off(q, nx, ny);

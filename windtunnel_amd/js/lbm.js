@@ -363,6 +363,11 @@ class LBM {
     return { dJdw: g, ms };
   }
 
+  /** Id of the recorded forward pass (each adjointForward takes the next one). */
+  adjointRecord() {
+    return native().adjointRecord(this._h);
+  }
+
   adjointRelease() {
     native().adjointRelease(this._h);
     this._adjSteps = 0;

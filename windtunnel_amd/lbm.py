@@ -104,6 +104,7 @@ SIGNATURES = {
     "wt_adjoint_forward": (_I, [_P, _I, _I, _P]),
     "wt_adjoint_backward": (_I, [_P, _I, _P, _P, _P]),
     "wt_adjoint_release": (_I, [_P]),
+    "wt_adjoint_record": (_I, [_P, _P]),
     "wt_voxel_bounds": (_I, [_P, _L, _P]),
     "wt_voxel_plan": (_I, [_P, _L, _I, _P]),
     "wt_voxel_build": (_I, [_P, _L, _I, _P, _P]),
@@ -538,6 +539,12 @@ class LBM:
         ms = C.c_double()
         _check(lib().wt_adjoint_backward(self._h, n, _ptr(Fbar), _ptr(g), C.byref(ms)))
         return g, ms.value
+
+    def adjoint_record(self) -> int:
+        """Id of the recorded forward pass (increases with every adjoint_forward)."""
+        v = C.c_int64(0)
+        _check(lib().wt_adjoint_record(self._h, C.byref(v)))
+        return int(v.value)
 
     def adjoint_release(self) -> None:
         _check(lib().wt_adjoint_release(self._h))
