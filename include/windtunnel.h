@@ -228,6 +228,18 @@ int wt_slice(wt_lattice *h, int axis, int k, int field, float *out, uint8_t *sol
  * range, into uvw[3p..3p+2]. */
 int wt_sample(wt_lattice *h, const double *xyz, int64_t count, double *uvw);
 
+/* wt_sample without any device allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture, tight serving loops (particles and probes sampled every
+ * frame) and harnesses that forbid hipMalloc. Same inputs, outputs and
+ * results bit for bit as wt_sample, but the device staging of the points
+ * and velocities is a fixed array of the library on each GPU, so count may
+ * be at most WT_SAMPLE_NOALLOC_MAX (WT_E_ARG beyond it). Calls that overlap
+ * on one GPU (from several threads) race on that array: serialise them.
+ * The lattice's device must be the current device (WT_E_STATE otherwise). */
+#define WT_SAMPLE_NOALLOC_MAX 262144
+int wt_sample_noalloc(wt_lattice *h, const double *xyz, int64_t count, double *uvw);
+
 /* MeshIO.parse (js/meshio.js:322-340 with its readers, :19-214): a model
  * file's bytes into a float32 triangle soup, 9 values per triangle in the
  * file's coordinates, for Voxel.build. The extension of `name` (or, when it

@@ -169,6 +169,10 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
 
     wt_lattice *h = new wt_lattice();
     h->nx = nx; h->ny = ny; h->nz = nz; h->n = (int64_t)nx * ny * nz;
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        delete h;
+        return wt_fail(WT_E_DEVICE, "hipGetDevice failed");
+    }
     h->storage = storage; h->arith = arith; h->layout = layout;
     h->pop_elem = storage == WT_STORE_F16 ? 2 : (storage == WT_STORE_F64 ? 8 : 4);
     h->mac_elem = storage == WT_STORE_F64 ? 8 : 4;

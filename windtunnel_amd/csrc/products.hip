@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace {
 
@@ -157,6 +158,49 @@ template <class MAC> int sample_t(wt_lattice *h, const double *xyz, int64_t coun
     return 0;
 }
 
+// ------------------------------------------------- no-allocation staging
+// The device staging of wt_sample_noalloc: WT_SAMPLE_NOALLOC_MAX points in
+// and as many velocities out (12 MiB), held by every GPU the library runs
+// on until the process ends. Host code never uses this name as an address:
+// each device's copy lives at its own address, looked up once per device
+// with hipGetSymbolAddress and kept in scratch_dev.
+static __device__ __align__(256) double scratch[6 * (size_t)WT_SAMPLE_NOALLOC_MAX];
+
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_dev[SCRATCH_DEVICES];  // address of `scratch` on each device, null until looked up
+
+// The device address of `scratch` on device dev, which must be current.
+int scratch_of(int dev, double **out)
+{
+    if (dev < 0 || dev >= SCRATCH_DEVICES) return wt_fail(WT_E_DEVICE, "device %d beyond the %d supported", dev, SCRATCH_DEVICES);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return wt_fail(WT_E_DEVICE, "hipGetSymbolAddress of the sample staging failed on device %d: %s", dev,
+                           hipGetErrorString(e));
+        scratch_dev[dev] = (double *)p;
+    }
+    *out = scratch_dev[dev];
+    return 0;
+}
+
+template <class MAC> int sample_noalloc_t(wt_lattice *h, double *stage, const double *xyz, int64_t count, double *uvw)
+{
+    const MAC *m = (const MAC *)h->d_mac;
+    const size_t b = sizeof(double) * 3 * (size_t)count;
+    double *dx = stage, *du = stage + 3 * (size_t)WT_SAMPLE_NOALLOC_MAX;
+    WT_HIP(hipMemcpyAsync(dx, xyz, b, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_sample<MAC>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, h->stream, m + h->n,
+                       m + 2 * h->n, m + 3 * h->n, (const double *)dx, du, count, h->nx, h->ny, h->nz, h->p.u0);
+    WT_HIP(hipGetLastError());
+    WT_HIP(hipMemcpyAsync(uvw, du, b, hipMemcpyDeviceToHost, h->stream));
+    WT_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -180,6 +224,24 @@ int wt_sample(wt_lattice *h, const double *xyz, int64_t count, double *uvw)
     if (count == 0) return 0;
     if (h->storage == WT_STORE_F64) return sample_t<double>(h, xyz, count, uvw);
     return sample_t<float>(h, xyz, count, uvw);
+}
+
+int wt_sample_noalloc(wt_lattice *h, const double *xyz, int64_t count, double *uvw)
+{
+    if (!h) return wt_fail(WT_E_ARG, "null lattice handle");
+    if (count < 0 || (count > 0 && (!xyz || !uvw))) return wt_fail(WT_E_ARG, "bad sample request");
+    if (count > WT_SAMPLE_NOALLOC_MAX)
+        return wt_fail(WT_E_ARG, "%lld points exceed the %d of the no-allocation sample", (long long)count,
+                       WT_SAMPLE_NOALLOC_MAX);
+    if (count == 0) return 0;
+    int dev = -1;
+    WT_HIP(hipGetDevice(&dev));
+    if (dev != h->device)
+        return wt_fail(WT_E_STATE, "the lattice lives on device %d, device %d is current", h->device, dev);
+    double *stage = nullptr;
+    if (int rc = scratch_of(dev, &stage)) return rc;
+    if (h->storage == WT_STORE_F64) return sample_noalloc_t<double>(h, stage, xyz, count, uvw);
+    return sample_noalloc_t<float>(h, stage, xyz, count, uvw);
 }
 
 }  // extern "C"

@@ -93,6 +93,7 @@ SIGNATURES = {
     "wt_set_pops": (_I, [_P, _P]),
     "wt_mesh_parse": (_I, [C.c_char_p, _P, C.c_int64, _P, _P]),
     "wt_sample": (_I, [_P, _P, C.c_int64, _P]),
+    "wt_sample_noalloc": (_I, [_P, _P, C.c_int64, _P]),
     "wt_set_timing": (_I, [_P, _I]),
     "wt_get_timing": (_I, [_P, _P, _P]),
     "wt_set_gray": (_I, [_P, _P]),
