@@ -124,8 +124,13 @@ template <class S> __device__ __forceinline__ uint32_t cell_bits(int64_t i, uint
     if constexpr (rounds_stochastically<S>()) return mix32((uint32_t)i ^ seed);
     return 0;
 }
-// the 13 rounding bits of direction q
-__device__ __forceinline__ uint32_t dir_bits(uint32_t r, int q) { return (r + (uint32_t)q * 0x9E3779B9u) >> 19; }
+// The 13 rounding bits of direction q: the cell's top 13 random bits XOR a
+// constant of the direction (the top 13 bits of q times the golden ratio
+// 2^32/phi, well spread over [0, 8192)). XOR with a constant is a bijection,
+// so each direction's bits are uniform; adding them to a float's bit
+// pattern compiles to one v_xad_u32 per direction.
+__host__ __device__ constexpr uint32_t dir_key(int q) { return ((uint32_t)q * 0x9E3779B9u) >> 19; }
+__device__ __forceinline__ uint32_t dir_bits(uint32_t r, int q) { return (r >> 19) ^ dir_key(q); }
 // Two values already in binary16 units (x F16_SCALE) to binary16 with one
 // conversion instruction: stochastically rounded with bits ra, rb (round
 // toward zero after the random bits are added), or to nearest.
@@ -136,6 +141,21 @@ __device__ __forceinline__ wt_half2 f16_pair(float a, float b, uint32_t ra, uint
         a = __uint_as_float(__float_as_uint(a) + ra);
         b = __uint_as_float(__float_as_uint(b) + rb);
         return __builtin_amdgcn_cvt_pkrtz(a, b);
+    } else {
+        return wt_half2{(__fp16)a, (__fp16)b};
+    }
+}
+// Two values in binary16 units to binary16 with stochastic rounding, the
+// bits of directions qa and qb taken from the cell's top 13 random bits r13
+// (r >> 19): dir_bits(r, q) added to the bit pattern as one v_xad_u32
+// ((r13 ^ key) + bits) per value.
+template <int QA, int QB> __device__ __forceinline__ wt_half2 f16_pair_dir(float a, float b, uint32_t r13)
+{
+    if constexpr (!WT_F16_NEAREST) {
+        uint32_t ua, ub;
+        asm("v_xad_u32 %0, %1, %2, %3" : "=v"(ua) : "v"(r13), "s"(dir_key(QA)), "v"(__float_as_uint(a)));
+        asm("v_xad_u32 %0, %1, %2, %3" : "=v"(ub) : "v"(r13), "s"(dir_key(QB)), "v"(__float_as_uint(b)));
+        return __builtin_amdgcn_cvt_pkrtz(__uint_as_float(ua), __uint_as_float(ub));
     } else {
         return wt_half2{(__fp16)a, (__fp16)b};
     }
@@ -320,18 +340,73 @@ __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq1
 // 3 W r cu, products are fused, and 1/x and sqrt use the hardware's
 // single-precision approximations (about 1 ulp). Same model, same results to
 // float rounding; the binary64 path above keeps the reference's order.
+// The pair terms are carried doubled (2 sp = s - 2 even, 2 sm = d - 6 W r cu,
+// an exact power-of-two scaling), which saves a multiply per pair.
 // SCL: f and g are in binary16 units (deviations x F16_SCALE, a power of
 // two), so binary16 loads and stores need no scaling; every product and sum
 // is the unscaled one times the scale, exactly. r and u are unscaled.
-template <bool LES, bool TRT, bool SCL = false>
-__device__ __forceinline__ void collide_fast(const float (&f)[Q], float tau0, float omega0, float sq18cs2, float lam,
+// IN: float, or the raw binary16 loads (H16, the bit patterns zero-extended
+// to 32 bits): every use of a loaded value is then an operand of a
+// mixed-precision FMA (v_fma_mix_f32: f16 operands are widened exactly and
+// the result is rounded once to f32, as a conversion and an add would give),
+// so the loads need no conversion instructions.
+struct H16 {
+    uint32_t bits;
+};
+// a + b, a - b, a + x and a * b + h for binary16 a, b, h and float x
+__device__ __forceinline__ float mix_hh(H16 a, H16 b)
+{
+    float d;
+    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(d) : "v"(a.bits), "v"(b.bits));
+    return d;
+}
+__device__ __forceinline__ float mix_hh_sub(H16 a, H16 b)
+{
+    float d;
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(d) : "v"(a.bits), "v"(b.bits));
+    return d;
+}
+__device__ __forceinline__ float mix_hf(H16 a, float x)
+{
+    float d;
+    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(a.bits), "v"(x));
+    return d;
+}
+__device__ __forceinline__ float mix_ffh(float a, float b, H16 h)
+{
+    float d;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(d) : "v"(a), "v"(b), "v"(h.bits));
+    return d;
+}
+__device__ __forceinline__ float add_in(float a, float b) { return a + b; }
+__device__ __forceinline__ float add_in(H16 a, H16 b) { return mix_hh(a, b); }
+__device__ __forceinline__ float sub_in(float a, float b) { return a - b; }
+__device__ __forceinline__ float sub_in(H16 a, H16 b) { return mix_hh_sub(a, b); }
+__device__ __forceinline__ float mix_hf_sub(H16 a, float x)
+{
+    float d;
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(a.bits), "v"(x));
+    return d;
+}
+__device__ __forceinline__ float add_in(float a, float x, int) { return a + x; }
+__device__ __forceinline__ float add_in(H16 a, float x, int) { return mix_hf(a, x); }
+__device__ __forceinline__ float sub_in(float a, float x, int) { return a - x; }
+__device__ __forceinline__ float sub_in(H16 a, float x, int) { return mix_hf_sub(a, x); }
+__device__ __forceinline__ float fma_in(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ float fma_in(float a, float b, H16 c) { return mix_ffh(a, b, c); }
+
+template <bool LES, bool TRT, bool SCL = false, class IN = float>
+__device__ __forceinline__ void collide_fast(const IN (&f)[Q], float tau0, float omega0, float sq18cs2, float lam,
                                              float (&g)[Q], float &r, float &vx, float &vy, float &vz)
 {
     constexpr float SC = SCL ? F16_SCALE : 1.0f;
     float s[9], d[9];
 #pragma unroll
-    for (int p = 0; p < 9; p++) { s[p] = f[2 * p + 1] + f[2 * p + 2]; d[p] = f[2 * p + 1] - f[2 * p + 2]; }
-    const float dr = f[0] + (((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])) + s[8]);
+    for (int p = 0; p < 9; p++) {
+        s[p] = add_in(f[2 * p + 1], f[2 * p + 2]);
+        d[p] = sub_in(f[2 * p + 1], f[2 * p + 2]);
+    }
+    const float dr = add_in(f[0], ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])) + s[8], 0);
     if constexpr (SCL) r = __builtin_fmaf(dr, F16_INV_SCALE, 1.0f);
     else r = 1.0f + dr;
     const float inv = __builtin_amdgcn_rcpf(r);
@@ -342,39 +417,41 @@ __device__ __forceinline__ void collide_fast(const float (&f)[Q], float tau0, fl
     vz = (d[2] + d[5] - d[6] + d[7] - d[8]) * invs;
     const float usq = __builtin_fmaf(vx, vx, __builtin_fmaf(vy, vy, vz * vz));
     const float A = __builtin_fmaf((-1.5f * SC) * r, usq, dr);
-    const float r45 = (4.5f * SC) * r;
     const float cu[9] = {vx, vy, vz, vx + vy, vx - vy, vx + vz, vx - vz, vy + vz, vy - vz};
     const float WF = 1.0f / 18, WE = 1.0f / 36;
-    const float oF = ((3.0f * WF) * SC) * r, oE = ((3.0f * WE) * SC) * r;
-    float sp[9], sm[9];  // (n_a + n_b)/2 and (n_a - n_b)/2 of each pair
+    // doubled equilibrium pair terms: 2 even = c1 cu^2 + c0, 2 odd = o2 cu
+    const float c1F = ((9.0f * WF) * SC) * r, c1E = ((9.0f * WE) * SC) * r;
+    const float c0F = (2.0f * WF) * A, c0E = (2.0f * WE) * A;
+    const float o2F = ((6.0f * WF) * SC) * r, o2E = ((6.0f * WE) * SC) * r;
+    float sp[9], sm[9];  // n_a + n_b and n_a - n_b of each pair (twice the half sums)
 #pragma unroll
     for (int p = 0; p < 9; p++) {
-        const float w = p < 3 ? WF : WE, o = p < 3 ? oF : oE;
-        const float even = w * __builtin_fmaf(r45 * cu[p], cu[p], A);
-        sp[p] = __builtin_fmaf(0.5f, s[p], -even);
-        sm[p] = __builtin_fmaf(0.5f, d[p], -o * cu[p]);
+        const float c1 = p < 3 ? c1F : c1E, c0 = p < 3 ? c0F : c0E, o2 = p < 3 ? o2F : o2E;
+        sp[p] = s[p] - __builtin_fmaf(c1 * cu[p], cu[p], c0);
+        sm[p] = __builtin_fmaf(-o2, cu[p], d[p]);
     }
-    const float n0 = __builtin_fmaf(-1.0f / 3, A, f[0]);
+    const float n0 = fma_in(-1.0f / 3, A, f[0]);
     float om = omega0, tauEff = tau0;
     if (LES) {
         const float pxx = sp[0] + sp[3] + sp[4] + sp[5] + sp[6];
         const float pyy = sp[1] + sp[3] + sp[4] + sp[7] + sp[8];
         const float pzz = sp[2] + sp[5] + sp[6] + sp[7] + sp[8];
         const float pxy = sp[3] - sp[4], pxz = sp[5] - sp[6], pyz = sp[7] - sp[8];
-        // the sums above are half the tensor components: mag = sqrt(8 (...))
+        // the sums above are the tensor components: mag = sqrt(2 (...))
         const float q2 = __builtin_fmaf(pxx, pxx, __builtin_fmaf(pyy, pyy, pzz * pzz)) +
                          2.0f * __builtin_fmaf(pxy, pxy, __builtin_fmaf(pxz, pxz, pyz * pyz));
-        const float mag = __builtin_amdgcn_sqrtf(8.0f * q2);
+        const float mag = __builtin_amdgcn_sqrtf(2.0f * q2);
         tauEff = 0.5f * (tau0 + __builtin_amdgcn_sqrtf(__builtin_fmaf(tau0, tau0, sq18cs2 * mag * invs)));
         om = __builtin_amdgcn_rcpf(tauEff);
     }
     const float omm = TRT ? __builtin_amdgcn_rcpf(__builtin_fmaf(lam, __builtin_amdgcn_rcpf(tauEff - 0.5f), 0.5f)) : om;
-    g[0] = __builtin_fmaf(-om, n0, f[0]);
+    const float omh = 0.5f * om, ommh = 0.5f * omm;
+    g[0] = fma_in(-om, n0, f[0]);
 #pragma unroll
     for (int p = 0; p < 9; p++) {
-        const float x = om * sp[p];
-        g[2 * p + 1] = __builtin_fmaf(-omm, sm[p], f[2 * p + 1] - x);
-        g[2 * p + 2] = __builtin_fmaf(omm, sm[p], f[2 * p + 2] - x);
+        const float x = omh * sp[p];
+        g[2 * p + 1] = sub_in(f[2 * p + 1], __builtin_fmaf(ommh, sm[p], x), 0);
+        g[2 * p + 2] = sub_in(f[2 * p + 2], __builtin_fmaf(-ommh, sm[p], x), 0);
     }
 }
 
@@ -611,18 +688,22 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
             // uniform base plus the lane term of x-1, x or x+1.
             const int64_t row = (int64_t)nx * (y + (int64_t)ny * z);
             const int64_t rb = (row >> a.lsh) * a.lstride;
+            // lane term; a row spans fewer than 2^24 cells and a block fewer
+            // than 2^24 elements, so the block offset is a 24-bit product
             auto lane = [&](int xs) {
-                return (uint32_t)(((xs >> a.lsh) * a.lstride + (xs & a.lmask)) * (int)sizeof(S));
+                return (uint32_t)((__umul24((unsigned)(xs >> a.lsh), (unsigned)a.lstride) + (xs & a.lmask)) * (int)sizeof(S));
             };
             const uint32_t o[3] = {lane(x - 1), lane(x), lane(x + 1)};
             T f[Q];
             T g[Q], r, vx, vy, vz;
             if constexpr (HALF_UNITS) {
-                // binary16 storage: collide in binary16 units, no scaling
+                // binary16 storage: collide in binary16 units, no scaling,
+                // straight from the raw loads
+                H16 h[Q];
 #pragma unroll
                 for (int q = 0; q < Q; q++)
-                    f[q] = __half2float(ld_pop((const __half *)((const char *)(a.lp[q] + rb) + o[1 + ld_shift<ESO>(q)]), 0));
-                collide_fast<LES, TRT, true>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r, vx, vy, vz);
+                    h[q].bits = ld_pop((const unsigned short *)((const char *)(a.lp[q] + rb) + o[1 + ld_shift<ESO>(q)]), 0);
+                collide_fast<LES, TRT, true, H16>(h, a.tauf, a.omf, a.sq18f, a.lamf, g, r, vx, vy, vz);
             } else {
 #pragma unroll
                 for (int q = 0; q < Q; q++) f[q] = ld_at<S, T>(a.lp[q] + rb, o[1 + ld_shift<ESO>(q)]);
@@ -638,13 +719,19 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
             auto store_all = [&](const uint32_t (&os)[3]) {
                 if constexpr (HALF_UNITS) {
                     auto at = [&](int q) { return (__half *)((char *)(a.sp[q] + rb) + os[1 + st_shift<ESO>(q)]); };
+                    const uint32_t r13 = rc >> 19;
                     st_h16(at(0), f16_pair(g[0], 0.0f, dir_bits(rc, 0), 0).x);
-#pragma unroll
-                    for (int q = 1; q < Q; q += 2) {
-                        const wt_half2 h = f16_pair(g[q], g[q + 1], dir_bits(rc, q), dir_bits(rc, q + 1));
+                    auto pair = [&](auto qa) {
+                        constexpr int q = decltype(qa)::value;
+                        const wt_half2 h = f16_pair_dir<q, q + 1>(g[q], g[q + 1], r13);
                         st_h16(at(q), h.x);
                         st_h16(at(q + 1), h.y);
-                    }
+                    };
+                    pair(std::integral_constant<int, 1>{}); pair(std::integral_constant<int, 3>{});
+                    pair(std::integral_constant<int, 5>{}); pair(std::integral_constant<int, 7>{});
+                    pair(std::integral_constant<int, 9>{}); pair(std::integral_constant<int, 11>{});
+                    pair(std::integral_constant<int, 13>{}); pair(std::integral_constant<int, 15>{});
+                    pair(std::integral_constant<int, 17>{});
                 } else {
 #pragma unroll
                     for (int q = 0; q < Q; q++) st_col_at<S, T>(a.sp[q] + rb, os[1 + st_shift<ESO>(q)], g[q], dir_bits(rc, q));
