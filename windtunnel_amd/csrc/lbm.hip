@@ -250,6 +250,11 @@ template <class S, class MAC> struct StepArgs {
     const S *lp[Q];
     S *sp[Q];
     int lsh, lstride, lmask;
+    // binary16 pair launch (k_step_pair): the load bases of the directions
+    // PAIR_DIRS, and each direction's equilibrium-plane value as a binary16
+    // bit pattern in both halves of a word
+    const S *fixp[10];
+    uint32_t eqw[Q];
 };
 
 // x shift of the cell whose slot holds the value a cell pulls along q (AB:
@@ -350,50 +355,58 @@ __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq1
 // mixed-precision FMA (v_fma_mix_f32: f16 operands are widened exactly and
 // the result is rounded once to f32, as a conversion and an add would give),
 // so the loads need no conversion instructions.
-struct H16 {
+// HI selects the half of the register a value sits in (0: low, 1: high),
+// so the two cells of a packed pair of binary16 loads read the same register.
+template <int HI> struct H16t {
     uint32_t bits;
 };
-// a + b, a - b, a + x and a * b + h for binary16 a, b, h and float x
-__device__ __forceinline__ float mix_hh(H16 a, H16 b)
+using H16 = H16t<0>;
+// a + b, a - b, a + x, a - x and a * b + h for binary16 a, b, h and float x
+template <int HI> __device__ __forceinline__ float mix_hh(H16t<HI> a, H16t<HI> b)
 {
     float d;
-    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(d) : "v"(a.bits), "v"(b.bits));
+    if constexpr (HI) asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(d) : "v"(a.bits), "v"(b.bits));
+    else asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(d) : "v"(a.bits), "v"(b.bits));
     return d;
 }
-__device__ __forceinline__ float mix_hh_sub(H16 a, H16 b)
+template <int HI> __device__ __forceinline__ float mix_hh_sub(H16t<HI> a, H16t<HI> b)
 {
     float d;
-    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(d) : "v"(a.bits), "v"(b.bits));
+    if constexpr (HI) asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(d) : "v"(a.bits), "v"(b.bits));
+    else asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(d) : "v"(a.bits), "v"(b.bits));
     return d;
 }
-__device__ __forceinline__ float mix_hf(H16 a, float x)
+template <int HI> __device__ __forceinline__ float mix_hf(H16t<HI> a, float x)
 {
     float d;
-    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(a.bits), "v"(x));
+    if constexpr (HI) asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(d) : "v"(a.bits), "v"(x));
+    else asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(a.bits), "v"(x));
     return d;
 }
-__device__ __forceinline__ float mix_ffh(float a, float b, H16 h)
+template <int HI> __device__ __forceinline__ float mix_hf_sub(H16t<HI> a, float x)
 {
     float d;
-    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(d) : "v"(a), "v"(b), "v"(h.bits));
+    if constexpr (HI) asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(d) : "v"(a.bits), "v"(x));
+    else asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(a.bits), "v"(x));
+    return d;
+}
+template <int HI> __device__ __forceinline__ float mix_ffh(float a, float b, H16t<HI> h)
+{
+    float d;
+    if constexpr (HI) asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(d) : "v"(a), "v"(b), "v"(h.bits));
+    else asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(d) : "v"(a), "v"(b), "v"(h.bits));
     return d;
 }
 __device__ __forceinline__ float add_in(float a, float b) { return a + b; }
-__device__ __forceinline__ float add_in(H16 a, H16 b) { return mix_hh(a, b); }
+template <int HI> __device__ __forceinline__ float add_in(H16t<HI> a, H16t<HI> b) { return mix_hh(a, b); }
 __device__ __forceinline__ float sub_in(float a, float b) { return a - b; }
-__device__ __forceinline__ float sub_in(H16 a, H16 b) { return mix_hh_sub(a, b); }
-__device__ __forceinline__ float mix_hf_sub(H16 a, float x)
-{
-    float d;
-    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(a.bits), "v"(x));
-    return d;
-}
+template <int HI> __device__ __forceinline__ float sub_in(H16t<HI> a, H16t<HI> b) { return mix_hh_sub(a, b); }
 __device__ __forceinline__ float add_in(float a, float x, int) { return a + x; }
-__device__ __forceinline__ float add_in(H16 a, float x, int) { return mix_hf(a, x); }
+template <int HI> __device__ __forceinline__ float add_in(H16t<HI> a, float x, int) { return mix_hf(a, x); }
 __device__ __forceinline__ float sub_in(float a, float x, int) { return a - x; }
-__device__ __forceinline__ float sub_in(H16 a, float x, int) { return mix_hf_sub(a, x); }
+template <int HI> __device__ __forceinline__ float sub_in(H16t<HI> a, float x, int) { return mix_hf_sub(a, x); }
 __device__ __forceinline__ float fma_in(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ float fma_in(float a, float b, H16 c) { return mix_ffh(a, b, c); }
+template <int HI> __device__ __forceinline__ float fma_in(float a, float b, H16t<HI> c) { return mix_ffh(a, b, c); }
 
 template <bool LES, bool TRT, bool SCL = false, class IN = float>
 __device__ __forceinline__ void collide_fast(const IN (&f)[Q], float tau0, float omega0, float sq18cs2, float lam,
@@ -632,37 +645,17 @@ __device__ __forceinline__ void block_force(double fx, double fy, double fz, dou
     }
 }
 
-template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO, int WAVES>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES))) k_step(StepArgs<S, MAC> a)
+// One cell of the streaming launch (x < nx - 1; x = nx - 1 is the outlet,
+// written by x = nx - 2): the straight path of an interior fluid cell, or
+// the equilibrium planes and the outlet copy of boundary and solid cells.
+// Near-wall cells are left to the near-wall blocks or launch.
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO>
+__device__ __forceinline__ void step_cell(const StepArgs<S, MAC> &a, int x, int y, int z)
 {
     using P = Pop<S>;
     // binary16 storage with the float collision: the straight path works in
     // binary16 units (no scaling at loads and stores)
     constexpr bool HALF_UNITS = std::is_same<S, __half>::value && std::is_same<T, float>::value && WT_FAST_COLLIDE;
-    // Four waves per SIMD at most: claiming v127 gives every wave a
-    // 128-register file, so at most four waves fit a SIMD.
-    if constexpr (WAVES <= 4) asm volatile("" ::: "v127");
-    if (a.prev_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 64)
-        reduce_partials(a.prev_partial, a.prev_count, a.prev_out);
-    if constexpr (fused_near<S>()) {
-        if ((int)blockIdx.z < a.near_planes) {
-            // The first near_planes z planes of the grid are near-wall
-            // blocks: one listed cell per thread. They are dispatched first,
-            // so their long dependent load chains overlap the streaming
-            // blocks; each writes its FP64 force partial.
-            const int64_t nb = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-            const int64_t k = nb * blockDim.x + threadIdx.x;
-            double fx = 0, fy = 0, fz = 0;
-            if (k < a.near_cnt)
-                near_cell<S, T, MAC, MACRO, LES, TRT, ESO, false>(a, (int64_t)a.near[a.near0 + k], fx, fy, fz);
-            block_force(fx, fy, fz, a.partial + nb * 3);
-            return;
-        }
-    }
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = a.revy ? a.ny - 1 - (int)blockIdx.y : (int)blockIdx.y;
-    const int zb = (int)blockIdx.z - a.near_planes, nzr = (int)gridDim.z - a.near_planes;
-    const int z = a.z0 + (a.revz ? nzr - 1 - zb : zb);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int nph = a.phase ^ 1;
 
@@ -788,6 +781,36 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
     }
 }
 
+template <class S, class T, class MAC, bool MACRO, bool LES, bool TRT, bool ESO, int WAVES>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES))) k_step(StepArgs<S, MAC> a)
+{
+    // Four waves per SIMD at most: claiming v127 gives every wave a
+    // 128-register file, so at most four waves fit a SIMD.
+    if constexpr (WAVES <= 4) asm volatile("" ::: "v127");
+    if (a.prev_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 64)
+        reduce_partials(a.prev_partial, a.prev_count, a.prev_out);
+    if constexpr (fused_near<S>()) {
+        if ((int)blockIdx.z < a.near_planes) {
+            // The first near_planes z planes of the grid are near-wall
+            // blocks: one listed cell per thread. They are dispatched first,
+            // so their long dependent load chains overlap the streaming
+            // blocks; each writes its FP64 force partial.
+            const int64_t nb = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+            const int64_t k = nb * blockDim.x + threadIdx.x;
+            double fx = 0, fy = 0, fz = 0;
+            if (k < a.near_cnt)
+                near_cell<S, T, MAC, MACRO, LES, TRT, ESO, false>(a, (int64_t)a.near[a.near0 + k], fx, fy, fz);
+            block_force(fx, fy, fz, a.partial + nb * 3);
+            return;
+        }
+    }
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = a.revy ? a.ny - 1 - (int)blockIdx.y : (int)blockIdx.y;
+    const int zb = (int)blockIdx.z - a.near_planes, nzr = (int)gridDim.z - a.near_planes;
+    const int z = a.z0 + (a.revz ? nzr - 1 - zb : zb);
+    step_cell<S, T, MAC, MACRO, LES, TRT, ESO>(a, x, y, z);
+}
+
 // The near-wall cells of one step as a launch of their own (binary16
 // storage): 256-thread blocks over the list (grid-stride when it is longer
 // than the partial buffer allows), the one-round-trip gather, one force
@@ -801,6 +824,181 @@ __global__ void __launch_bounds__(NEAR_THREADS) k_near(StepArgs<S, MAC> a)
          k += (int64_t)gridDim.x * blockDim.x)
         near_cell<S, T, MAC, MACRO, LES, TRT, ESO, true>(a, (int64_t)a.near[a.near0 + k], fx, fy, fz);
     block_force(fx, fy, fz, a.partial + (int64_t)blockIdx.x * 3);
+}
+
+// f(integral_constant<int, q>) for q = 0 .. 18
+#define WT_FOR_Q(f)                                                                                            \
+    f(std::integral_constant<int, 0>{}); f(std::integral_constant<int, 1>{}); f(std::integral_constant<int, 2>{}); \
+    f(std::integral_constant<int, 3>{}); f(std::integral_constant<int, 4>{}); f(std::integral_constant<int, 5>{}); \
+    f(std::integral_constant<int, 6>{}); f(std::integral_constant<int, 7>{}); f(std::integral_constant<int, 8>{}); \
+    f(std::integral_constant<int, 9>{}); f(std::integral_constant<int, 10>{}); f(std::integral_constant<int, 11>{}); \
+    f(std::integral_constant<int, 12>{}); f(std::integral_constant<int, 13>{}); f(std::integral_constant<int, 14>{}); \
+    f(std::integral_constant<int, 15>{}); f(std::integral_constant<int, 16>{}); f(std::integral_constant<int, 17>{}); \
+    f(std::integral_constant<int, 18>{});
+
+// ------------------------------------------------------- binary16 pairs
+// The binary16 step with two cells per thread (x, x + 1; x even), for two
+// buffers in 128-cell blocks. A wave then covers one block, and every
+// population a thread pulls or stores is one 4-byte word holding both
+// cells: half the memory instructions of one cell per thread, whose 2-byte
+// accesses keep the texture addresser busy (on MI355X it, not HBM, bounds
+// the one-cell binary16 step). The pulls along +x and -x are the word of
+// the same cells shifted across lanes (DPP wavefront shift by one lane);
+// the lane at the block's edge takes the missing half from the neighbouring
+// block, loaded once per wave by ten lanes (one per shifted direction) and
+// broadcast with readlane. Each cell's collision, rounding and results are
+// those of the one-cell path bit for bit: the same code, reading the low or
+// the high half of the shared registers.
+constexpr int PAIR_DIRS[10] = {1, 7, 9, 11, 13, 2, 8, 10, 12, 14};  // e_x = +1 (5), then -1 (5)
+// Four waves per SIMD (at most 128 VGPRs): two cells per thread fill them
+// with as many cells as eight one-cell waves, and the near-wall path fits
+// beside the streaming path without spilling, so the near-wall cells run
+// as the first blocks of the same launch.
+#ifndef WT_PAIR_WAVES
+#define WT_PAIR_WAVES 4
+#endif
+
+template <bool MACRO, bool LES, bool TRT>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WT_PAIR_WAVES)))
+k_step_pair(StepArgs<__half, float> a)
+{
+    using S = __half;
+    using MAC = float;
+    if (a.prev_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 64)
+        reduce_partials(a.prev_partial, a.prev_count, a.prev_out);
+    if ((int)blockIdx.z < a.near_planes) {
+        // near-wall blocks, dispatched first: two listed cells per thread,
+        // one force partial per block
+        const int64_t nb = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+        double fx = 0, fy = 0, fz = 0;
+#pragma unroll 1
+        for (int r = 0; r < 2; r++) {
+            const int64_t k = (2 * nb + r) * blockDim.x + threadIdx.x;
+            if (k < a.near_cnt)
+                near_cell<S, float, MAC, MACRO, LES, TRT, false, false>(a, (int64_t)a.near[a.near0 + k], fx, fy, fz);
+        }
+        block_force(fx, fy, fz, a.partial + nb * 3);
+        return;
+    }
+    const int x = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = a.revy ? a.ny - 1 - (int)blockIdx.y : (int)blockIdx.y;
+    const int zb = (int)blockIdx.z - a.near_planes, nzr = (int)gridDim.z - a.near_planes;
+    const int z = a.z0 + (a.revz ? nzr - 1 - zb : zb);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    if (y == 0 || y == ny - 1 || z == 0 || z == nz - 1) {
+        // equilibrium rows (whole workgroups): the one-cell code
+        step_cell<S, float, MAC, MACRO, LES, TRT, false>(a, x, y, z);
+        step_cell<S, float, MAC, MACRO, LES, TRT, false>(a, x + 1, y, z);
+        return;
+    }
+    const int lane = (int)(threadIdx.x & 63);
+    const int x0 = x - 2 * lane;  // first cell of the wave's block
+    const int64_t row = (int64_t)nx * (y + (int64_t)ny * z);
+    const int64_t rb = (row >> 7) * a.lstride;
+    // element of cell xs within its row (a row spans fewer than 2^24 cells)
+    auto elem = [&](int xs) { return __umul24((unsigned)(xs >> 7), (unsigned)a.lstride) + (unsigned)(xs & 127); };
+    const uint32_t eb = 2 * elem(x);  // byte offset of the word of cells (x, x + 1)
+    // every lane of the row pulls (the shifted words need their neighbours)
+    uint32_t w[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) w[q] = *(const uint32_t *)((const char *)(a.lp[q] + rb) + eb);
+    uint32_t fv;
+    {
+        // lanes 0..9: the word beyond the block's edge of direction
+        // PAIR_DIRS[lane] (before the block for e_x = +1, after it for -1;
+        // clamped into the row at its ends, where no cell uses it)
+        const int li = lane < 10 ? lane : 0;
+        const int xf = li < 5 ? (x0 >= 2 ? x0 - 2 : 0) : (x0 + 128 <= nx - 2 ? x0 + 128 : nx - 2);
+        const S *fp = a.fixp[0];  // a select chain: no dynamic index into the arguments
+#pragma unroll
+        for (int k = 1; k < 10; k++) fp = li == k ? a.fixp[k] : fp;
+        fv = *(const uint32_t *)((const char *)(fp + rb) + 2 * elem(xf));
+    }
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        const int q = PAIR_DIRS[k];
+        const uint32_t edge = (uint32_t)__builtin_amdgcn_readlane((int)fv, k);
+        if (k < 5) {
+            // pull from x - 1: (high half of the lane below, low half of this lane)
+            const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)w[q], 0x138, 0xf, 0xf, false);
+            w[q] = __builtin_amdgcn_alignbit(w[q], prev, 16);
+        } else {
+            // pull from x + 1: (high half of this lane, low half of the lane above)
+            const uint32_t next = (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)w[q], 0x130, 0xf, 0xf, false);
+            w[q] = __builtin_amdgcn_alignbit(next, w[q], 16);
+        }
+    }
+    const int64_t wi = (x >> 5) + (int64_t)a.nwx * (y + (int64_t)ny * z);
+    const uint32_t sw = a.sbits[wi] >> (x & 31), nw = a.nbits[wi] >> (x & 31);
+    const bool solA = sw & 1u, solB = (sw >> 1) & 1u, nrA = nw & 1u, nrB = (nw >> 1) & 1u;
+    const bool bndA = x == 0, outB = x + 1 == nx - 1;
+    // both cells collide (the values of boundary cells are replaced, those of
+    // solid and near-wall cells masked below: the near-wall blocks write
+    // near-wall cells)
+    H16t<0> hA[Q];
+    H16t<1> hB[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) { hA[q].bits = w[q]; hB[q].bits = w[q]; }
+    float gA[Q], gB[Q], rA, uxA, uyA, uzA, rB, uxB, uyB, uzB;
+    collide_fast<LES, TRT, true>(hA, a.tauf, a.omf, a.sq18f, a.lamf, gA, rA, uxA, uyA, uzA);
+    collide_fast<LES, TRT, true>(hB, a.tauf, a.omf, a.sq18f, a.lamf, gB, rB, uxB, uyB, uzB);
+    const int64_t iA = row + x;
+    const uint32_t r13A = cell_bits<S>(iA, a.seed) >> 19, r13B = cell_bits<S>(iA + 1, a.seed) >> 19;
+    // the word to store: low half cell A (the equilibrium value at x = 0),
+    // high half cell B (at x + 1 = nx - 1, the outlet: a copy of cell A)
+    const uint32_t sel = bndA ? 0x07060100u : (outB ? 0x05040504u : 0x07060504u);
+    // halves written here: every cell but a solid or near-wall one (the
+    // inlet cell x = 0 writes its equilibrium whatever it is); the outlet's
+    // when cell A is written (a near-wall x = nx - 2 cell writes the outlet
+    // from its near-wall block, a solid one copies itself into it in the
+    // one-cell code below)
+    const bool wA = bndA || (!solA && !nrA), wB = outB ? (!solA && !nrA) : (!solB && !nrB);
+    auto word = [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        uint32_t ua, ub;
+        if constexpr (!WT_F16_NEAREST) {
+            asm("v_xad_u32 %0, %1, %2, %3" : "=v"(ua) : "v"(r13A), "s"(dir_key(q)), "v"(__float_as_uint(gA[q])));
+            asm("v_xad_u32 %0, %1, %2, %3" : "=v"(ub) : "v"(r13B), "s"(dir_key(q)), "v"(__float_as_uint(gB[q])));
+        } else {
+            ua = __float_as_uint(gA[q]);
+            ub = __float_as_uint(gB[q]);
+        }
+        wt_half2 pk = WT_F16_NEAREST ? wt_half2{(__fp16)gA[q], (__fp16)gB[q]}
+                                     : __builtin_amdgcn_cvt_pkrtz(__uint_as_float(ua), __uint_as_float(ub));
+        uint32_t pw;
+        __builtin_memcpy(&pw, &pk, 4);
+        return __builtin_amdgcn_perm(pw, a.eqw[q], sel);
+    };
+    auto dst = [&](int q) { return (char *)(a.sp[q] + rb) + eb; };
+    if (__builtin_expect(!__any(solA || solB || nrA || nrB), 1)) {
+        auto st = [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            *(uint32_t *)dst(q) = word(qc);
+        };
+        WT_FOR_Q(st)
+    } else {
+        auto st = [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const uint32_t v = word(qc);
+            if (wA && wB) *(uint32_t *)dst(q) = v;
+            else if (wA) *(uint16_t *)dst(q) = (uint16_t)v;
+            else if (wB) *(uint16_t *)(dst(q) + 2) = (uint16_t)(v >> 16);
+        };
+        WT_FOR_Q(st)
+        if (solA && outB) step_cell<S, float, MAC, MACRO, LES, TRT, false>(a, x, y, z);
+    }
+    if (MACRO) {
+        // fields of the written straight and equilibrium cells (a near-wall
+        // cell's are the near-wall launch's)
+        if (bndA || (!solA && !nrA)) {
+            a.rho[iA] = bndA ? 1.0f : rA; a.ux[iA] = bndA ? (float)a.u0 : uxA;
+            a.uy[iA] = bndA ? 0.0f : uyA; a.uz[iA] = bndA ? 0.0f : uzA;
+        }
+        if (outB ? (!solA && !nrA) : (!solB && !nrB)) {
+            a.rho[iA + 1] = outB ? rA : rB; a.ux[iA + 1] = outB ? uxA : uxB;
+            a.uy[iA + 1] = outB ? uyA : uyB; a.uz[iA + 1] = outB ? uzA : uzB;
+        }
+    }
 }
 
 // A pending force reduction on its own (one wavefront).
@@ -1255,16 +1453,42 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     a.near0 = h->near_z[(size_t)bz0];
     a.near_cnt = h->near_z[(size_t)bz1] - a.near0;
     const int64_t row_blocks = (int64_t)gx * h->ny;
-    const int64_t near_blocks = fused_near<S>() ? (a.near_cnt + bx - 1) / bx
-                                                : std::min<int64_t>((a.near_cnt + NEAR_THREADS - 1) / NEAR_THREADS,
-                                                                    h->partial_cap);
+    int64_t near_blocks = fused_near<S>() ? (a.near_cnt + bx - 1) / bx
+                                          : std::min<int64_t>((a.near_cnt + NEAR_THREADS - 1) / NEAR_THREADS,
+                                                              h->partial_cap);
     a.near_planes = fused_near<S>() ? (int)((near_blocks + row_blocks - 1) / row_blocks) : 0;
-    const int64_t count = fused_near<S>() ? a.near_planes * row_blocks : near_blocks;
-    hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO, step_waves<S>()>),
-                       dim3(gx, h->ny, bz1 - bz0 + a.near_planes), dim3(bx), 0, h->stream, a);
+    bool paired = false;
+    if constexpr (std::is_same<S, __half>::value && std::is_same<T, float>::value && !ESO && WT_FAST_COLLIDE) {
+        // two cells per thread: 128-cell blocks, so a wave is one block
+        // 128-cell blocks: rows of whole blocks, nx % 128 == 0
+        const int tpr = h->bsh == 7 ? h->nx / 2 : 0;  // threads per row: a multiple of 64
+        int bxp = 1024;                                // the widest workgroup of whole waves dividing the row
+        while (bxp > 64 && tpr % bxp) bxp -= 64;
+        const int64_t prow = (int64_t)(tpr / bxp) * h->ny;
+        // near-wall blocks of 2 bxp listed cells: at most nz planes of them
+        const int64_t nblk = (a.near_cnt + 2 * bxp - 1) / (2 * bxp);
+        const int np = prow > 0 ? (int)((nblk + prow - 1) / prow) : 0;
+        if (h->pairs && tpr > 0 && tpr % 64 == 0 && np * prow <= h->partial_cap) {
+            for (int k = 0; k < 10; k++) a.fixp[k] = a.lp[PAIR_DIRS[k]];
+            for (int q = 0; q < Q; q++) {
+                const uint32_t b = __half_as_ushort(__float2half_rn((float)a.eqv[q] * F16_SCALE));
+                a.eqw[q] = b | (b << 16);
+            }
+            a.near_planes = np;
+            hipLaunchKernelGGL((k_step_pair<MACRO, LES, TRT>), dim3(tpr / bxp, h->ny, bz1 - bz0 + a.near_planes),
+                               dim3(bxp), 0, h->stream, a);
+            paired = true;
+            near_blocks = a.near_planes * prow;
+        }
+    }
+    if (!paired)
+        hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO, step_waves<S>()>),
+                           dim3(gx, h->ny, bz1 - bz0 + a.near_planes), dim3(bx), 0, h->stream, a);
     WT_HIP(hipGetLastError());
+    // force partials of this launch: one per near-wall block
+    const int64_t count = fused_near<S>() ? a.near_planes * row_blocks : near_blocks;
     if constexpr (!fused_near<S>()) {
-        if (near_blocks > 0) {
+        if (near_blocks > 0 && !paired) {
             hipLaunchKernelGGL((k_near<S, T, MAC, MACRO, LES, TRT, ESO>), dim3((unsigned)near_blocks),
                                dim3(NEAR_THREADS), 0, h->stream, a);
             WT_HIP(hipGetLastError());
