@@ -1,9 +1,10 @@
-"""The binary16 step with two cells per thread (two buffers, 128-cell
-blocks) against the one-cell step: the same populations and fields bit for
+"""The step with two cells per thread (binary16 and float32 storage, two
+buffers, rows of 128-cell multiples) against the one-cell step: the same populations and fields bit for
 bit and the same forces to the order of their sum, with a body (solid and near-wall cells, a solid cell in the
 column next to the outlet), a rolling-road floor, moving walls, the slab
-form and the steps that write the fields. WT_F16_PAIRS=0 selects the
-one-cell step when a lattice is created."""
+form and the steps that write the fields. WT_PAIRS (read when a lattice is
+created) selects the one-cell step (0) or two cells for float32 too (2; by
+default only binary16 takes two)."""
 import numpy as np
 import pytest
 
@@ -14,10 +15,10 @@ pytestmark = pytest.mark.gpu
 DIMS = (256, 24, 20)
 
 
-def run(monkeypatch, pairs, les, trt, ground):
-    monkeypatch.setenv("WT_F16_PAIRS", str(pairs))
-    L = LBM(*DIMS, storage="f16")
-    assert L.block_cells == 128
+def run(monkeypatch, storage, pairs, les, trt, ground):
+    monkeypatch.setenv("WT_PAIRS", str(pairs))
+    L = LBM(*DIMS, storage=storage)
+    assert L.block_cells == (128 if storage == "f16" else 64)
     L.set_params(tau=0.56, les=les, trt=trt, csm=0.1, u0=0.05, **{"lambda": 3 / 16})
     L.reset()
     nx, ny, nz = DIMS
@@ -38,11 +39,12 @@ def run(monkeypatch, pairs, les, trt, ground):
     return out
 
 
+@pytest.mark.parametrize("storage", ["f16", "f32"])
 @pytest.mark.parametrize("les,trt,ground", [(True, True, False), (False, False, True), (True, False, True)])
-def test_pairs_equal_one_cell_step(monkeypatch, les, trt, ground):
-    print(f"[f16 pairs] les={les} trt={trt} ground={ground}", flush=True)
-    one = run(monkeypatch, 0, les, trt, ground)
-    two = run(monkeypatch, 1, les, trt, ground)
+def test_pairs_equal_one_cell_step(monkeypatch, storage, les, trt, ground):
+    print(f"[pairs] {storage} les={les} trt={trt} ground={ground}", flush=True)
+    one = run(monkeypatch, storage, 0, les, trt, ground)
+    two = run(monkeypatch, storage, 2, les, trt, ground)
     for k in one:
         if k.startswith("f"):
             # the force partials of near-wall cells are summed per block: a

@@ -200,10 +200,11 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
     // tuning knob.
     h->sweep = layout == WT_LAYOUT_ESOTERIC ? 1 : 0;
     if (const char *e = std::getenv("WT_SWEEP")) h->sweep = std::atoi(e) & 3;
-    // binary16 with two buffers in 128-cell blocks steps two cells per
-    // thread (half the memory instructions); WT_F16_PAIRS=0 keeps one cell
-    // per thread (the same results bit for bit: a test and tuning knob)
-    if (const char *e = std::getenv("WT_F16_PAIRS")) h->pairs = std::atoi(e) != 0;
+    // binary16 storage with two buffers and rows of 128-cell multiples steps
+    // two cells per thread (half the memory instructions). WT_PAIRS=0 keeps
+    // one cell per thread, WT_PAIRS=2 takes two for float32 storage as well
+    // (the same populations and fields bit for bit: a test and tuning knob)
+    if (const char *e = std::getenv("WT_PAIRS")) h->pairs = std::atoi(e);
     // The step addresses a block-laid row as whole blocks, so the blocked
     // layout needs rows of a whole number of blocks; other widths use planes.
     if (h->bsh && nx % (1 << h->bsh)) h->bsh = 0;

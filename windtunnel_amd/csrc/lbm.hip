@@ -275,7 +275,8 @@ __device__ __forceinline__ bool bit(const uint32_t *bits, int nwx, int ny, int x
 #endif
 
 // Moments, deviation equilibrium, LES and TRT/BGK relaxation of one cell
-// (js/lbm.js:377-460) in the reference's expression order.
+// (js/lbm.js:377-460) in the reference's expression order (up to exact
+// factors of two, below).
 template <class T, bool LES, bool TRT>
 __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq18cs2, T lam, T (&g)[Q],
                                         T &r, T &vx, T &vy, T &vz)
@@ -328,12 +329,16 @@ __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq1
     const T omm = TRT ? T(1) / (T(0.5) + lam / (tauEff - T(0.5))) : om;
 
     g[0] = f0 - om * (f0 - e0);
+    // om * (0.5 (n_a + n_b)) rounds to exactly (0.5 om) * (n_a + n_b): a
+    // factor of two commutes with rounding (no value here is near the
+    // subnormal range), so the halves move into the rates, bit for bit
+    const T omh = T(0.5) * om, ommh = T(0.5) * omm;
     T sp, sm;
 #define WT_PAIR(a, b)                                \
-    sp = T(0.5) * (n##a + n##b);                     \
-    sm = T(0.5) * (n##a - n##b);                     \
-    g[a] = f##a - om * sp - omm * sm;                \
-    g[b] = f##b - om * sp + omm * sm;
+    sp = omh * (n##a + n##b);                        \
+    sm = ommh * (n##a - n##b);                       \
+    g[a] = f##a - sp - sm;                           \
+    g[b] = f##b - sp + sm;
     WT_PAIR(1, 2) WT_PAIR(3, 4) WT_PAIR(5, 6) WT_PAIR(7, 8) WT_PAIR(9, 10)
     WT_PAIR(11, 12) WT_PAIR(13, 14) WT_PAIR(15, 16) WT_PAIR(17, 18)
 #undef WT_PAIR
@@ -502,7 +507,17 @@ __device__ __forceinline__ void reduce_partials(const double *p, int64_t count, 
 #ifndef WT_WAVES
 #define WT_WAVES 4
 #endif
-template <class S> constexpr int step_waves() { return std::is_same<S, __half>::value ? WT_F16_WAVES : WT_WAVES; }
+// Float32 storage with binary64 arithmetic (the reference's own numbers):
+// build switch WT_EXACT_WAVES.
+#ifndef WT_EXACT_WAVES
+#define WT_EXACT_WAVES WT_WAVES
+#endif
+template <class S, class T> constexpr int step_waves()
+{
+    return std::is_same<S, __half>::value ? WT_F16_WAVES
+                                          : (std::is_same<S, float>::value && std::is_same<T, double>::value ? WT_EXACT_WAVES
+                                                                                                             : WT_WAVES);
+}
 
 // One near-wall cell (an interior fluid cell with a solid upstream
 // neighbour, a gray cell or a neighbour of one): halfway bounce-back, moving
@@ -626,7 +641,16 @@ __device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, 
 // path would spill, so it steps them in a launch of its own (k_near) after
 // the streaming kernel; wider storage runs them as the first blocks of the
 // streaming launch (no second launch, no gap between launches).
-template <class S> constexpr bool fused_near() { return !std::is_same<S, __half>::value; }
+// Build switch WT_EXACT_FUSED_NEAR=0: float32 storage with binary64
+// arithmetic steps its near-wall cells in a launch of their own.
+#ifndef WT_EXACT_FUSED_NEAR
+#define WT_EXACT_FUSED_NEAR 1
+#endif
+template <class S, class T> constexpr bool fused_near()
+{
+    return !std::is_same<S, __half>::value &&
+           (WT_EXACT_FUSED_NEAR || !(std::is_same<S, float>::value && std::is_same<T, double>::value));
+}
 
 // Block sum of the per-thread force: wave shuffles, then the waves' sums in
 // wave order, written by thread 0 to out[0..2].
@@ -789,7 +813,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES
     if constexpr (WAVES <= 4) asm volatile("" ::: "v127");
     if (a.prev_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 64)
         reduce_partials(a.prev_partial, a.prev_count, a.prev_out);
-    if constexpr (fused_near<S>()) {
+    if constexpr (fused_near<S, T>()) {
         if ((int)blockIdx.z < a.near_planes) {
             // The first near_planes z planes of the grid are near-wall
             // blocks: one listed cell per thread. They are dispatched first,
@@ -858,12 +882,16 @@ constexpr int PAIR_DIRS[10] = {1, 7, 9, 11, 13, 2, 8, 10, 12, 14};  // e_x = +1 
 #define WT_PAIR_WAVES 4
 #endif
 
-template <bool MACRO, bool LES, bool TRT>
+// Float32 storage the same way: a pair is two 4-byte values (one 8-byte
+// access), the x shifts move one of them across lanes, and the edge values
+// are single elements.
+template <class S, bool MACRO, bool LES, bool TRT>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WT_PAIR_WAVES)))
-k_step_pair(StepArgs<__half, float> a)
+k_step_pair(StepArgs<S, float> a)
 {
-    using S = __half;
+    constexpr bool HALF = std::is_same<S, __half>::value;
     using MAC = float;
+    using Word = typename std::conditional<HALF, uint32_t, float2>::type;  // the pair's two values
     if (a.prev_out && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x < 64)
         reduce_partials(a.prev_partial, a.prev_count, a.prev_out);
     if ((int)blockIdx.z < a.near_planes) {
@@ -892,40 +920,58 @@ k_step_pair(StepArgs<__half, float> a)
         return;
     }
     const int lane = (int)(threadIdx.x & 63);
-    const int x0 = x - 2 * lane;  // first cell of the wave's block
+    const int x0 = x - 2 * lane;  // first cell of the wave's 128
     const int64_t row = (int64_t)nx * (y + (int64_t)ny * z);
-    const int64_t rb = (row >> 7) * a.lstride;
+    const int64_t rb = (row >> a.lsh) * a.lstride;
     // element of cell xs within its row (a row spans fewer than 2^24 cells)
-    auto elem = [&](int xs) { return __umul24((unsigned)(xs >> 7), (unsigned)a.lstride) + (unsigned)(xs & 127); };
-    const uint32_t eb = 2 * elem(x);  // byte offset of the word of cells (x, x + 1)
-    // every lane of the row pulls (the shifted words need their neighbours)
-    uint32_t w[Q];
+    auto elem = [&](int xs) {
+        return __umul24((unsigned)(xs >> a.lsh), (unsigned)a.lstride) + (unsigned)(xs & a.lmask);
+    };
+    const uint32_t eb = (uint32_t)sizeof(S) * elem(x);  // byte offset of the pair (x, x + 1)
+    // every lane of the row pulls (the shifted pairs need their neighbours)
+    Word w[Q];
 #pragma unroll
-    for (int q = 0; q < Q; q++) w[q] = *(const uint32_t *)((const char *)(a.lp[q] + rb) + eb);
+    for (int q = 0; q < Q; q++) w[q] = *(const Word *)((const char *)(a.lp[q] + rb) + eb);
     uint32_t fv;
     {
-        // lanes 0..9: the word beyond the block's edge of direction
-        // PAIR_DIRS[lane] (before the block for e_x = +1, after it for -1;
-        // clamped into the row at its ends, where no cell uses it)
+        // lanes 0..9: the value beyond the wave's edge of direction
+        // PAIR_DIRS[lane] (before it for e_x = +1, after it for -1; clamped
+        // into the row at its ends, where no cell uses it); binary16 loads
+        // the aligned word that holds it
         const int li = lane < 10 ? lane : 0;
-        const int xf = li < 5 ? (x0 >= 2 ? x0 - 2 : 0) : (x0 + 128 <= nx - 2 ? x0 + 128 : nx - 2);
+        const int lo = HALF ? 2 : 1;
+        const int xf = li < 5 ? (x0 >= lo ? x0 - lo : 0) : (x0 + 128 <= nx - lo ? x0 + 128 : nx - lo);
         const S *fp = a.fixp[0];  // a select chain: no dynamic index into the arguments
 #pragma unroll
         for (int k = 1; k < 10; k++) fp = li == k ? a.fixp[k] : fp;
-        fv = *(const uint32_t *)((const char *)(fp + rb) + 2 * elem(xf));
+        fv = *(const uint32_t *)((const char *)(fp + rb) + sizeof(S) * elem(xf));
     }
 #pragma unroll
     for (int k = 0; k < 10; k++) {
         const int q = PAIR_DIRS[k];
-        const uint32_t edge = (uint32_t)__builtin_amdgcn_readlane((int)fv, k);
-        if (k < 5) {
-            // pull from x - 1: (high half of the lane below, low half of this lane)
-            const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)w[q], 0x138, 0xf, 0xf, false);
-            w[q] = __builtin_amdgcn_alignbit(w[q], prev, 16);
+        const int edge = __builtin_amdgcn_readlane((int)fv, k);
+        if constexpr (HALF) {
+            if (k < 5) {
+                // pull from x - 1: (high half of the lane below, low half of this lane)
+                const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(edge, (int)w[q], 0x138, 0xf, 0xf, false);
+                w[q] = __builtin_amdgcn_alignbit(w[q], prev, 16);
+            } else {
+                // pull from x + 1: (high half of this lane, low half of the lane above)
+                const uint32_t next = (uint32_t)__builtin_amdgcn_update_dpp(edge, (int)w[q], 0x130, 0xf, 0xf, false);
+                w[q] = __builtin_amdgcn_alignbit(next, w[q], 16);
+            }
         } else {
-            // pull from x + 1: (high half of this lane, low half of the lane above)
-            const uint32_t next = (uint32_t)__builtin_amdgcn_update_dpp((int)edge, (int)w[q], 0x130, 0xf, 0xf, false);
-            w[q] = __builtin_amdgcn_alignbit(next, w[q], 16);
+            if (k < 5) {
+                // pull from x - 1: (the lane below's x + 1, this lane's x)
+                const float prev =
+                    __int_as_float(__builtin_amdgcn_update_dpp(edge, __float_as_int(w[q].y), 0x138, 0xf, 0xf, false));
+                w[q] = make_float2(prev, w[q].x);
+            } else {
+                // pull from x + 1: (this lane's x + 1, the lane above's x)
+                const float next =
+                    __int_as_float(__builtin_amdgcn_update_dpp(edge, __float_as_int(w[q].x), 0x130, 0xf, 0xf, false));
+                w[q] = make_float2(w[q].y, next);
+            }
         }
     }
     const int64_t wi = (x >> 5) + (int64_t)a.nwx * (y + (int64_t)ny * z);
@@ -935,61 +981,80 @@ k_step_pair(StepArgs<__half, float> a)
     // both cells collide (the values of boundary cells are replaced, those of
     // solid and near-wall cells masked below: the near-wall blocks write
     // near-wall cells)
-    H16t<0> hA[Q];
-    H16t<1> hB[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) { hA[q].bits = w[q]; hB[q].bits = w[q]; }
     float gA[Q], gB[Q], rA, uxA, uyA, uzA, rB, uxB, uyB, uzB;
-    collide_fast<LES, TRT, true>(hA, a.tauf, a.omf, a.sq18f, a.lamf, gA, rA, uxA, uyA, uzA);
-    collide_fast<LES, TRT, true>(hB, a.tauf, a.omf, a.sq18f, a.lamf, gB, rB, uxB, uyB, uzB);
+    if constexpr (HALF) {
+        H16t<0> hA[Q];
+        H16t<1> hB[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) { hA[q].bits = w[q]; hB[q].bits = w[q]; }
+        collide_fast<LES, TRT, true>(hA, a.tauf, a.omf, a.sq18f, a.lamf, gA, rA, uxA, uyA, uzA);
+        collide_fast<LES, TRT, true>(hB, a.tauf, a.omf, a.sq18f, a.lamf, gB, rB, uxB, uyB, uzB);
+    } else {
+        float fA[Q], fB[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) { fA[q] = w[q].x; fB[q] = w[q].y; }
+        collide_fast<LES, TRT>(fA, a.tauf, a.omf, a.sq18f, a.lamf, gA, rA, uxA, uyA, uzA);
+        collide_fast<LES, TRT>(fB, a.tauf, a.omf, a.sq18f, a.lamf, gB, rB, uxB, uyB, uzB);
+    }
     const int64_t iA = row + x;
-    const uint32_t r13A = cell_bits<S>(iA, a.seed) >> 19, r13B = cell_bits<S>(iA + 1, a.seed) >> 19;
-    // the word to store: low half cell A (the equilibrium value at x = 0),
-    // high half cell B (at x + 1 = nx - 1, the outlet: a copy of cell A)
-    const uint32_t sel = bndA ? 0x07060100u : (outB ? 0x05040504u : 0x07060504u);
+    // the pair to store: cell A (the equilibrium value at x = 0) and cell B
+    // (at x + 1 = nx - 1, the outlet: a copy of cell A)
+    auto word = [&](auto qc) -> Word {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (HALF) {
+            const uint32_t r13A = cell_bits<S>(iA, a.seed) >> 19, r13B = cell_bits<S>(iA + 1, a.seed) >> 19;
+            const uint32_t sel = bndA ? 0x07060100u : (outB ? 0x05040504u : 0x07060504u);
+            uint32_t ua, ub;
+            if constexpr (!WT_F16_NEAREST) {
+                asm("v_xad_u32 %0, %1, %2, %3" : "=v"(ua) : "v"(r13A), "s"(dir_key(q)), "v"(__float_as_uint(gA[q])));
+                asm("v_xad_u32 %0, %1, %2, %3" : "=v"(ub) : "v"(r13B), "s"(dir_key(q)), "v"(__float_as_uint(gB[q])));
+            } else {
+                ua = __float_as_uint(gA[q]);
+                ub = __float_as_uint(gB[q]);
+            }
+            wt_half2 pk = WT_F16_NEAREST ? wt_half2{(__fp16)gA[q], (__fp16)gB[q]}
+                                         : __builtin_amdgcn_cvt_pkrtz(__uint_as_float(ua), __uint_as_float(ub));
+            uint32_t pw;
+            __builtin_memcpy(&pw, &pk, 4);
+            return __builtin_amdgcn_perm(pw, a.eqw[q], sel);
+        } else {
+            const float va = bndA ? (float)a.eqv[q] : gA[q];
+            return make_float2(va, outB ? va : gB[q]);
+        }
+    };
     // halves written here: every cell but a solid or near-wall one (the
     // inlet cell x = 0 writes its equilibrium whatever it is); the outlet's
     // when cell A is written (a near-wall x = nx - 2 cell writes the outlet
     // from its near-wall block, a solid one copies itself into it in the
     // one-cell code below)
     const bool wA = bndA || (!solA && !nrA), wB = outB ? (!solA && !nrA) : (!solB && !nrB);
-    auto word = [&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        uint32_t ua, ub;
-        if constexpr (!WT_F16_NEAREST) {
-            asm("v_xad_u32 %0, %1, %2, %3" : "=v"(ua) : "v"(r13A), "s"(dir_key(q)), "v"(__float_as_uint(gA[q])));
-            asm("v_xad_u32 %0, %1, %2, %3" : "=v"(ub) : "v"(r13B), "s"(dir_key(q)), "v"(__float_as_uint(gB[q])));
-        } else {
-            ua = __float_as_uint(gA[q]);
-            ub = __float_as_uint(gB[q]);
-        }
-        wt_half2 pk = WT_F16_NEAREST ? wt_half2{(__fp16)gA[q], (__fp16)gB[q]}
-                                     : __builtin_amdgcn_cvt_pkrtz(__uint_as_float(ua), __uint_as_float(ub));
-        uint32_t pw;
-        __builtin_memcpy(&pw, &pk, 4);
-        return __builtin_amdgcn_perm(pw, a.eqw[q], sel);
-    };
     auto dst = [&](int q) { return (char *)(a.sp[q] + rb) + eb; };
     if (__builtin_expect(!__any(solA || solB || nrA || nrB), 1)) {
         auto st = [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            *(uint32_t *)dst(q) = word(qc);
+            *(Word *)dst(q) = word(qc);
         };
         WT_FOR_Q(st)
     } else {
         auto st = [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const uint32_t v = word(qc);
-            if (wA && wB) *(uint32_t *)dst(q) = v;
-            else if (wA) *(uint16_t *)dst(q) = (uint16_t)v;
-            else if (wB) *(uint16_t *)(dst(q) + 2) = (uint16_t)(v >> 16);
+            const Word v = word(qc);
+            if (wA && wB) {
+                *(Word *)dst(q) = v;
+            } else if constexpr (HALF) {
+                if (wA) *(uint16_t *)dst(q) = (uint16_t)v;
+                else if (wB) *(uint16_t *)(dst(q) + 2) = (uint16_t)(v >> 16);
+            } else {
+                if (wA) *(float *)dst(q) = v.x;
+                else if (wB) *(float *)(dst(q) + 4) = v.y;
+            }
         };
         WT_FOR_Q(st)
         if (solA && outB) step_cell<S, float, MAC, MACRO, LES, TRT, false>(a, x, y, z);
     }
     if (MACRO) {
         // fields of the written straight and equilibrium cells (a near-wall
-        // cell's are the near-wall launch's)
+        // cell's are its near-wall block's)
         if (bndA || (!solA && !nrA)) {
             a.rho[iA] = bndA ? 1.0f : rA; a.ux[iA] = bndA ? (float)a.u0 : uxA;
             a.uy[iA] = bndA ? 0.0f : uyA; a.uz[iA] = bndA ? 0.0f : uzA;
@@ -1453,41 +1518,45 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     a.near0 = h->near_z[(size_t)bz0];
     a.near_cnt = h->near_z[(size_t)bz1] - a.near0;
     const int64_t row_blocks = (int64_t)gx * h->ny;
-    int64_t near_blocks = fused_near<S>() ? (a.near_cnt + bx - 1) / bx
+    int64_t near_blocks = fused_near<S, T>() ? (a.near_cnt + bx - 1) / bx
                                           : std::min<int64_t>((a.near_cnt + NEAR_THREADS - 1) / NEAR_THREADS,
                                                               h->partial_cap);
-    a.near_planes = fused_near<S>() ? (int)((near_blocks + row_blocks - 1) / row_blocks) : 0;
+    a.near_planes = fused_near<S, T>() ? (int)((near_blocks + row_blocks - 1) / row_blocks) : 0;
     bool paired = false;
-    if constexpr (std::is_same<S, __half>::value && std::is_same<T, float>::value && !ESO && WT_FAST_COLLIDE) {
-        // two cells per thread: 128-cell blocks, so a wave is one block
-        // 128-cell blocks: rows of whole blocks, nx % 128 == 0
-        const int tpr = h->bsh == 7 ? h->nx / 2 : 0;  // threads per row: a multiple of 64
+    if constexpr ((std::is_same<S, __half>::value || std::is_same<S, float>::value) && std::is_same<T, float>::value &&
+                  !ESO && WT_FAST_COLLIDE) {
+        // two cells per thread, a wave 128 cells: blocked rows of 128-cell multiples
+        const int tpr = h->bsh && h->nx % 128 == 0 ? h->nx / 2 : 0;  // threads per row: a multiple of 64
         int bxp = 1024;                                // the widest workgroup of whole waves dividing the row
         while (bxp > 64 && tpr % bxp) bxp -= 64;
         const int64_t prow = (int64_t)(tpr / bxp) * h->ny;
         // near-wall blocks of 2 bxp listed cells: at most nz planes of them
         const int64_t nblk = (a.near_cnt + 2 * bxp - 1) / (2 * bxp);
         const int np = prow > 0 ? (int)((nblk + prow - 1) / prow) : 0;
-        if (h->pairs && tpr > 0 && tpr % 64 == 0 && np * prow <= h->partial_cap) {
+        // binary16 by default; float32 only on request (WT_PAIRS=2): with
+        // 4-byte values, twice the cells in flight per wave made the float32
+        // step 11 % slower on MI355X (0.989 against 0.892 ms at 512x256x256)
+        const bool want = std::is_same<S, __half>::value ? h->pairs >= 1 : h->pairs >= 2;
+        if (want && tpr > 0 && tpr % 64 == 0 && np * prow <= h->partial_cap) {
             for (int k = 0; k < 10; k++) a.fixp[k] = a.lp[PAIR_DIRS[k]];
             for (int q = 0; q < Q; q++) {
                 const uint32_t b = __half_as_ushort(__float2half_rn((float)a.eqv[q] * F16_SCALE));
                 a.eqw[q] = b | (b << 16);
             }
             a.near_planes = np;
-            hipLaunchKernelGGL((k_step_pair<MACRO, LES, TRT>), dim3(tpr / bxp, h->ny, bz1 - bz0 + a.near_planes),
+            hipLaunchKernelGGL((k_step_pair<S, MACRO, LES, TRT>), dim3(tpr / bxp, h->ny, bz1 - bz0 + a.near_planes),
                                dim3(bxp), 0, h->stream, a);
             paired = true;
             near_blocks = a.near_planes * prow;
         }
     }
     if (!paired)
-        hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO, step_waves<S>()>),
+        hipLaunchKernelGGL((k_step<S, T, MAC, MACRO, LES, TRT, ESO, step_waves<S, T>()>),
                            dim3(gx, h->ny, bz1 - bz0 + a.near_planes), dim3(bx), 0, h->stream, a);
     WT_HIP(hipGetLastError());
     // force partials of this launch: one per near-wall block
-    const int64_t count = fused_near<S>() ? a.near_planes * row_blocks : near_blocks;
-    if constexpr (!fused_near<S>()) {
+    const int64_t count = paired ? near_blocks : fused_near<S, T>() ? a.near_planes * row_blocks : near_blocks;
+    if constexpr (!fused_near<S, T>()) {
         if (near_blocks > 0 && !paired) {
             hipLaunchKernelGGL((k_near<S, T, MAC, MACRO, LES, TRT, ESO>), dim3((unsigned)near_blocks),
                                dim3(NEAR_THREADS), 0, h->stream, a);

@@ -96,7 +96,7 @@ struct wt_lattice {
     void *adj = nullptr;                  // recorded adjoint forward pass (adjoint.hip), or null
     int64_t adj_seq = 0;                  // id of the last recorded forward pass
     int device = 0;                       // HIP device the lattice was created on
-    int pairs = 1;                        // binary16 step with two cells per thread (WT_F16_PAIRS=0: one)
+    int pairs = 1;                        // two cells per thread: 0 never, 1 binary16, 2 binary16 and float32
     uint64_t epoch = 0;                   // bumped by every change of parameters, body or weights
     void *d_mac = nullptr;        // rho, ux, uy, uz planes of n values
     double *d_partial = nullptr;  // per-block force partials, two buffers of partial_cap triples
