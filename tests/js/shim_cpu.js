@@ -23,7 +23,7 @@ const Voxel = require(path.join(JS, 'voxel.js'));
   const addon = require(path.join(JS, '..', 'wt_napi.node'));
   const exported = ('abiVersion create setParams getParams reset voxelize voxelizeGrid voxelizeParts step ' +
     'stepSlabs stepSlab boundaries setParity getState getFields getPops setPops getSolid getWalls timeSteps ' +
-    'deviceMemory setBoundaryMode slice sampleMany meshParse setGray adjointForward adjointBackward adjointRelease adjointRecord voxelBounds voxelPlan voxelBuild').split(' ');
+    'deviceMemory setBoundaryMode slice sampleMany meshParse setGray adjointForward adjointBackward adjointRelease adjointRecord voxelBounds voxelPlan voxelBuild stepSlabsAsync').split(' ');
   assert(Object.keys(addon).sort().join() === exported.sort().join(), 'addon exports: ' + Object.keys(addon).join(' '));
   err = null;
   try { addon.deviceMemory(); } catch (e) { err = e; }

@@ -5,7 +5,7 @@
 const path = require('path');
 const { load, arr, sha, assert, close, applyParams, replay } = require('./fixtures.js');
 const JS = path.join(__dirname, '..', '..', 'windtunnel_amd', 'js');
-const { LBM } = require(path.join(JS, 'lbm.js'));
+const { LBM, native } = require(path.join(JS, 'lbm.js'));
 const { Pool } = require(path.join(JS, 'pool.js'));
 const Voxel = require(path.join(JS, 'voxel.js'));
 
@@ -74,6 +74,39 @@ check('worker pool protocol reproduces lbm_pool3', async () => {
   pool.dispose();
   for (const k of ['rho', 'ux', 'uy', 'uz']) assert(sha(L[k]) === fx.pooled[k], k);
   assert(L.parity === fx.pooled.parity && L.steps === fx.pooled.steps, 'parity/steps');
+});
+
+check('Pool.step leaves the event loop running; its forces equal the synchronous slab step', async () => {
+  const mk = () => {
+    const L = new LBM(512, 128, 128);
+    L.tau = 0.56; L.u0 = 0.05;
+    L.reset();
+    L.voxelize((x, y, z) => (x - 100) ** 2 + (y - 64) ** 2 + (z - 64) ** 2 < 144,
+      { x0: 80, y0: 44, z0: 44, x1: 120, y1: 84, z1: 84 }, false);
+    return L;
+  };
+  const A = mk(), B = mk();
+  const pool = await new Pool(A, 3).start();
+  pool.syncParams(true);
+  let ticks = 0, running = true;
+  const tick = () => { ticks++; if (running) setImmediate(tick); };
+  setImmediate(tick);
+  const got = await pool.step(16);
+  running = false;
+  pool.dispose();
+  assert(ticks > 0, 'the event loop ran while the batch stepped (ticks ' + ticks + ')');
+  // the same batch, synchronously on the twin lattice
+  B._sync();
+  native().setParity(B._h, 0);
+  const f = new Float64Array(16 * 3 * 3);
+  native().stepSlabs(B._h, 16, 3, f);
+  for (let s = 0; s < 16; s++) {
+    for (let k = 0; k < 3; k++) {
+      const want = f[(s * 3) * 3 + k] + f[(s * 3 + 1) * 3 + k] + f[(s * 3 + 2) * 3 + k];
+      assert(got[s][k] === want, 'force step ' + s);
+    }
+  }
+  assert(sha(A.rho) === sha(B.rho) && sha(A.ux) === sha(B.ux), 'fields');
 });
 
 check('lbmWorker.js step loop through stepSlab/_boundaries reproduces lbm_pool3', () => {
@@ -226,6 +259,36 @@ check('Voxel.build is bit-exact (vox_box, vox_hitcap, vox_sphere)', () => {
   let msg = null;
   try { Voxel.build(new Float32Array(0)); } catch (e) { msg = e.message; }
   assert(msg === load('vox_errors').empty, 'error message');
+});
+
+check('ForceStats folds the drop-in\'s step forces into the app\'s run report', () => {
+  const { ForceStats } = require(path.join(JS, 'stats.js'));
+  const fx = load('lbm_bgk_sphere');
+  const L = new LBM(...fx.dims, { arith: 'f64' });
+  applyParams(L, fx.params);
+  L.reset();
+  replay(L, fx.voxelize);
+  const st = new ForceStats(5);
+  const F = [];
+  for (let s = 0; s < 80; s++) {
+    if (s === 20) st.setCapture(true, L.steps);
+    L.step();
+    F.push(Array.from(L.force));
+    st.accumulate(L.force, L.u0, L.frontalArea, L.steps);
+  }
+  st.setCapture(false, L.steps);
+  const csv = st.runCSV(L, { name: 'sphere', custom: false, credit: null, ref: 'frontal', scale: 16, ground: false,
+    lengthM: 1, windMs: 30, aoa: 0, yaw: 0 }, '2026-01-01T00:00:00.000Z').split('\n');
+  const row = (k) => csv.find((l) => l.startsWith(k + ',')).split(',');
+  assert(row('capture_from_step')[1] === '20' && row('capture_to_step')[1] === '80', 'capture window');
+  assert(row('samples')[1] === '60' && row('timesteps_total')[1] === '80', 'sample count');
+  // the captured samples are the pair averages of steps 21..80
+  const q = 0.5 * L.u0 * L.u0 * L.frontalArea;
+  let m = 0;
+  for (let s = 20; s < 80; s++) m += 0.5 * (F[s][0] + F[s - 1][0]) / q;
+  m /= 60;
+  const cd = Number(row('Cd')[1]);
+  assert(Math.abs(cd - m) < 1e-5, 'Cd ' + cd + ' vs ' + m);
 });
 
 check('injected device fault surfaces as an Error', () => {

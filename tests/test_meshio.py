@@ -16,6 +16,13 @@ from _util import load
 from windtunnel_amd import synth
 from windtunnel_amd.lbm import mesh_parse
 
+
+# every test here also runs in the GPU suite, on the shipped build of the GPU box
+@pytest.fixture(params=[pytest.param("cpu"), pytest.param("gpu", marks=pytest.mark.gpu)], autouse=True)
+def suite(request):
+    return request.param
+
+
 DIR = os.path.join(os.path.dirname(__file__), "golden", "fixtures", "meshio")
 CASES = load("mesh_io")["cases"]
 

@@ -9,6 +9,13 @@ import pytest
 from windtunnel_amd.stats import HIST, SETTLE, ForceStats, block_stats, coefficient_stats, pair_coefficients
 
 
+# every test here also runs in the GPU suite, on the shipped build of the GPU box
+@pytest.fixture(params=[pytest.param("cpu"), pytest.param("gpu", marks=pytest.mark.gpu)], autouse=True)
+def suite(request):
+    return request.param
+
+
+
 def test_pair_average_cancels_parity_oscillation():
     k = np.arange(400)
     F = np.stack([0.3 + 0.05 * (-1.0) ** k, -0.1 + 0.02 * (-1.0) ** k, 0 * k], 1)

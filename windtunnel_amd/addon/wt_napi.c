@@ -331,6 +331,81 @@ static napi_value js_step_slabs(napi_env env, napi_callback_info info)
     return NULL;
 }
 
+/* stepSlabsAsync(h, n, nslabs, forces) -> Promise: wt_step_slabs on a libuv
+ * worker thread, so the event loop keeps running while the device steps
+ * (as the reference's worker threads leave the page free, js/pool.js:97-125).
+ * The handle and the forces array are held until the batch ends; the
+ * lattice must not be used otherwise meanwhile (the pool serialises its
+ * batches). The promise rejects with the library's message on failure. */
+typedef struct {
+    napi_async_work work;
+    napi_deferred deferred;
+    napi_ref href, fref;
+    wt_lattice *h;
+    int n, ns, rc;
+    double *f;
+    char msg[512];
+} step_job;
+
+static void step_job_run(napi_env env, void *data)
+{
+    (void)env;
+    step_job *j = (step_job *)data;
+    j->rc = wt_step_slabs(j->h, j->n, j->ns, j->f);
+    if (j->rc) snprintf(j->msg, sizeof j->msg, "%s", wt_last_error());
+}
+
+static void step_job_done(napi_env env, napi_status status, void *data)
+{
+    step_job *j = (step_job *)data;
+    if (status == napi_ok && j->rc == 0) {
+        napi_value u;
+        napi_get_undefined(env, &u);
+        napi_resolve_deferred(env, j->deferred, u);
+    } else {
+        char code[16];
+        snprintf(code, sizeof code, "WT%d", j->rc);
+        napi_value c, m, e;
+        napi_create_string_utf8(env, code, NAPI_AUTO_LENGTH, &c);
+        napi_create_string_utf8(env, status == napi_ok ? j->msg : "stepSlabsAsync was cancelled", NAPI_AUTO_LENGTH, &m);
+        napi_create_error(env, c, m, &e);
+        napi_reject_deferred(env, j->deferred, e);
+    }
+    napi_delete_reference(env, j->href);
+    if (j->fref) napi_delete_reference(env, j->fref);
+    napi_delete_async_work(env, j->work);
+    free(j);
+}
+
+static napi_value js_step_slabs_async(napi_env env, napi_callback_info info)
+{
+    napi_value a[4];
+    args(env, info, 4, a);
+    wt_lattice *h = handle(env, a[0]);
+    if (!h) return NULL;
+    int ok;
+    size_t nb;
+    const int n = (int)num(env, a[1], 1), ns = (int)num(env, a[2], 1);
+    double *f = tdata(env, a[3], &nb, &ok);
+    if (!ok) return NULL;
+    if (f && nb < (size_t)n * ns * 24) { napi_throw_range_error(env, NULL, "forces array too short"); return NULL; }
+    step_job *j = (step_job *)calloc(1, sizeof *j);
+    if (!j) { napi_throw_error(env, NULL, "out of host memory"); return NULL; }
+    j->h = h; j->n = n; j->ns = ns; j->f = f;
+    napi_value promise, name;
+    if (napi_create_promise(env, &j->deferred, &promise) != napi_ok ||
+        napi_create_reference(env, a[0], 1, &j->href) != napi_ok ||
+        (f && napi_create_reference(env, a[3], 1, &j->fref) != napi_ok) ||
+        napi_create_string_utf8(env, "wt_step_slabs", NAPI_AUTO_LENGTH, &name) != napi_ok ||
+        napi_create_async_work(env, NULL, name, step_job_run, step_job_done, j, &j->work) != napi_ok ||
+        napi_queue_async_work(env, j->work) != napi_ok) {
+        free(j);
+        napi_throw_error(env, NULL, "stepSlabsAsync: N-API call failed");
+        return NULL;
+    }
+    return promise;
+}
+
 /* getWalls(h, uwx, uwy, uwz) -- Float32Arrays of n cells or null */
 static napi_value js_get_walls(napi_env env, napi_callback_info info)
 {
@@ -861,7 +936,7 @@ static napi_value init(napi_env env, napi_value exports)
     const struct { const char *name; napi_callback cb; } fns[] = {
         {"abiVersion", js_abi}, {"create", js_create}, {"setParams", js_set_params},
         {"getParams", js_get_params}, {"reset", js_reset}, {"voxelize", js_voxelize},
-        {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs},
+        {"voxelizeGrid", js_voxelize_grid}, {"step", js_step}, {"stepSlabs", js_step_slabs}, {"stepSlabsAsync", js_step_slabs_async},
         {"setParity", js_set_parity}, {"getState", js_get_state}, {"getFields", js_get_fields},
         {"getPops", js_get_pops}, {"setPops", js_set_pops}, {"getSolid", js_get_solid}, {"timeSteps", js_time_steps}, {"deviceMemory", js_device_memory}, {"setBoundaryMode", js_set_boundary_mode}, {"slice", js_slice}, {"sampleMany", js_sample_many}, {"meshParse", js_mesh_parse},
         {"voxelizeParts", js_voxelize_parts}, {"getWalls", js_get_walls}, {"stepSlab", js_step_slab}, {"boundaries", js_boundaries}, {"setGray", js_set_gray}, {"adjointForward", js_adjoint_forward}, {"adjointBackward", js_adjoint_backward},

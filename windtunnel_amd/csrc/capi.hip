@@ -676,6 +676,19 @@ static int run_steps(wt_lattice *h, int nsteps, int nslabs, double *forces)
     const int64_t fs = fault_step();
     if (fs >= 0 && h->steps <= fs && fs < h->steps + nsteps)
         return wt_fail(WT_E_FAULT, "injected fault at step %lld", (long long)fs);
+    // the lattice's device, also when the call comes from another host
+    // thread (the N-API addon steps on a worker thread); the caller's
+    // current device is restored on return
+    struct DeviceGuard {
+        int prev = -1;
+        ~DeviceGuard() { if (prev >= 0) (void)hipSetDevice(prev); }
+    } guard;
+    int cur = -1;
+    WT_HIP(hipGetDevice(&cur));
+    if (cur != h->device) {
+        WT_HIP(hipSetDevice(h->device));
+        guard.prev = cur;
+    }
     const int per = 3 * nslabs;
     const int chunk = std::max(1, h->forces_cap / nslabs);
     for (int s0 = 0; s0 < nsteps; s0 += chunk) {

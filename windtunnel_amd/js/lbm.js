@@ -283,36 +283,24 @@ class LBM {
     return o;
   }
 
-  /** Trilinear velocity sample at lattice position (js/lbm.js:515-539). */
+  /**
+   * sample() at one point (js/lbm.js:515-539): the device's batched sample
+   * (wt_sample) with a batch of one, on the fields of the last step.
+   */
   sample(x, y, z, out) {
-    const nx = this.nx, ny = this.ny, nz = this.nz;
-    if (x < 0.5 || y < 0.5 || z < 0.5 || x > nx - 1.5 || y > ny - 1.5 || z > nz - 1.5) {
-      out[0] = this.u0; out[1] = 0; out[2] = 0; return out;
-    }
-    const F = this._macro();
-    const xf = x - 0.5, yf = y - 0.5, zf = z - 0.5;
-    const x0 = xf | 0, y0 = yf | 0, z0 = zf | 0;
-    const tx = xf - x0, ty = yf - y0, tz = zf - z0;
-    let vx = 0, vy = 0, vz = 0;
-    for (let k = 0; k < 2; k++) {
-      const wz = k ? tz : 1 - tz;
-      for (let j = 0; j < 2; j++) {
-        const wy = (j ? ty : 1 - ty) * wz;
-        const ia = x0 + nx * ((y0 + j) + ny * (z0 + k)), ib = ia + 1;
-        const wa = (1 - tx) * wy, wb = tx * wy;
-        vx += F.ux[ia] * wa + F.ux[ib] * wb;
-        vy += F.uy[ia] * wa + F.uy[ib] * wb;
-        vz += F.uz[ia] * wa + F.uz[ib] * wb;
-      }
-    }
-    out[0] = vx; out[1] = vy; out[2] = vz;
+    const p = this._probe || (this._probe = { xyz: new Float64Array(3), uvw: new Float64Array(3) });
+    p.xyz[0] = x; p.xyz[1] = y; p.xyz[2] = z;
+    this._sync();  // u0 answers for points off the lattice
+    native().sampleMany(this._h, p.xyz, p.uvw);
+    out[0] = p.uvw[0]; out[1] = p.uvw[1]; out[2] = p.uvw[2];
     return out;
   }
 
+  /** isSolid() (js/lbm.js:541-545): the cell holding the point, by ToInt32 truncation. */
   isSolid(x, y, z) {
-    const xi = x | 0, yi = y | 0, zi = z | 0;
-    if (xi < 0 || yi < 0 || zi < 0 || xi >= this.nx || yi >= this.ny || zi >= this.nz) return false;
-    return this.solid[xi + this.nx * (yi + this.ny * zi)] !== 0;
+    const c = [x | 0, y | 0, z | 0], dims = [this.nx, this.ny, this.nz];
+    for (let k = 0; k < 3; k++) if (c[k] < 0 || c[k] >= dims[k]) return false;
+    return this.solid[this.idx(c[0], c[1], c[2])] !== 0;
   }
 
   /**

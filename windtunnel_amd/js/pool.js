@@ -10,8 +10,10 @@
  * exactly as the worker 'init' handler splits them (js/lbmWorker.js:43-48).
  * A step of every slab must finish before the next step starts (the
  * reference's barrier), so worker 0 launches the whole batch for all slabs
- * (wt_step_slabs) and every worker then answers its 'step' message; the pool
- * totals the per-slab forces in worker order as js/pool.js:108-118 does.
+ * (wt_step_slabs, on a libuv worker thread so the event loop stays free) and
+ * answers its 'step' message when the device is done, the other workers at
+ * once; the pool totals the per-slab forces in worker order as
+ * js/pool.js:108-118 does.
  *
  * The 'params' message carries tau, les, u0 and movingWalls as in the
  * reference; trt, lambda and csm are optional additions, and workers that do
@@ -30,13 +32,17 @@ class GpuWorker {
   }
 
   postMessage(m) {
+    const fail = (e) => { if (this.onerror) this.onerror({ message: e.message, error: e }); };
+    const reply = (r) => { if (r && !this._terminated && this.onmessage) this.onmessage({ data: r }); };
     Promise.resolve().then(() => {
       if (this._terminated) return;
       try {
-        const reply = this._handle(m);
-        if (reply && this.onmessage) this.onmessage({ data: reply });
+        const r = this._handle(m);
+        // a step batch answers when the device is done with it
+        if (r instanceof Promise) r.then(reply, fail);
+        else reply(r);
       } catch (e) {
-        if (this.onerror) this.onerror({ message: e.message, error: e });
+        fail(e);
       }
     });
   }
@@ -67,9 +73,13 @@ class GpuWorker {
         native().setParams(this.h, { tau: p.tau, csm: p.csm, lambda: p.lambda, u0: p.u0,
           les: p.les ? 1 : 0, trt: p.trt ? 1 : 0, movingWalls: p.movingWalls ? 1 : 0 });
         native().setParity(this.h, m.parity & 1);
+        // the batch runs on a libuv worker thread (stepSlabsAsync), so the
+        // event loop stays free, as with the reference's worker threads
         const f = new Float64Array(m.n * N * 3);
-        native().stepSlabs(this.h, m.n, N, f);
-        this.forces.set(f, 0);
+        return native().stepSlabsAsync(this.h, m.n, N, f).then(() => {
+          this.forces.set(f, 0);
+          return { ok: 'step', id: this.id };
+        });
       }
       return { ok: 'step', id: this.id };
     }
