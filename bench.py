@@ -649,7 +649,8 @@ def run_scale(N=3000, K=500):
     from windtunnel_amd.lbm import LBM, Voxel, device_memory
     from windtunnel_amd.synth import noisy_sphere
     dims = (640, 320, 320)
-    res = dict(grid="640x320x320", storage="f16", steps=N, tau=0.505, u0=0.08, Lambda=3 / 16, Cs=0.14)
+    res = dict(grid="640x320x320", storage="f16", adjoint_fields="binary16, scaled per step", steps=N, tau=0.505,
+               u0=0.08, Lambda=3 / 16, Cs=0.14)
     P = noisy_sphere(level=7, R=56, holes=0.005, seed=5)
     t0 = time.perf_counter()
     vox = Voxel.build(P, 128)
@@ -730,6 +731,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-drag", action="store_true")
     ap.add_argument("--no-micro", action="store_true")
+    ap.add_argument("--scale-steps", type=int, default=3000,
+                    help="recorded forward steps of the 640x320x320 adjoint leg (objective over the last sixth)")
     ap.add_argument("--fd-cells", type=int, default=64, help="central-difference cells of the adjoint leg (0: none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy three times (HBM counter calibration)")
@@ -811,7 +814,7 @@ def main():
         log(f"drag: {json.dumps(out['drag'])}")
     if not a.no_scale:
         log("scale 640x320x320 f16, level-7 leaky icosphere, 3000 steps + adjoint")
-        out["scale"] = run_scale()
+        out["scale"] = run_scale(a.scale_steps, max(1, a.scale_steps // 6))
         log(f"scale: {json.dumps(out['scale'])}")
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -18,16 +18,25 @@
 //                    copies that cell, js/lbm.js:504-511)
 //
 // It reads mu_k (19 gathered), s_{k-1} (19 gathered) and writes mu_{k-1}
-// (19): 1.5x the bytes of a forward step. States inside a checkpoint segment
-// are recomputed by the forward kernel into a replay store, so the reverse
-// sweep costs one recompute step plus one adjoint step per forward step.
+// (19): 1.5x the bytes of a forward step when mu has the states' element
+// size. States inside a checkpoint segment are recomputed by the forward
+// kernel into a replay store, so the reverse sweep costs one recompute step
+// plus one adjoint step per forward step.
 //
 // Binary16 storage: the states (checkpoints, replays) keep the forward's own
-// binary16 offsets, so every replay is bit-identical to the recorded forward;
-// the adjoint fields mu are float, and the rounding of a store is taken as
-// the identity. When the replay store of a whole segment does not fit in
-// device memory, each segment is swept in windows: the window's states are
-// recomputed from the segment's checkpoint, and only those are kept.
+// binary16 offsets, so every replay is bit-identical to the recorded forward,
+// and the rounding of a store is taken as the identity. The adjoint fields
+// are binary16 too (float with WT_ADJ_FIELD=f32 in the environment): field
+// mu_k is stored as mu_k * sigma_k, rounded to nearest, where sigma_k is the
+// power of two that puts max|mu_{k+1}| + 2 |Fbar_{k+1}|_1 (a bound of
+// lambda_{k+1} without the outlet fold) in [2^10, 2^11). One reverse step
+// grows the field by far less than the 2^5 left below binary16's largest
+// value; the kernel records each field's max |mu| (one atomic per wave that
+// raises it) for the next launch's scale, and the host checks after the
+// sweep that no stored value overflowed. When the replay store of a whole
+// segment does not fit in device memory, each segment is swept in windows:
+// the window's states are recomputed from the segment's checkpoint, and only
+// those are kept.
 
 #include "collide_adj.h"
 #include "wt_internal.h"
@@ -35,6 +44,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -73,6 +83,19 @@ template <class T> __device__ __forceinline__ T sval(const __half *p, int64_t k)
     return (T)(__half2float(p[k]) * wt::F16_INV_SCALE);
 }
 
+// Exponent of the scale of a binary16 adjoint field: 2^e * bound lies in
+// [2^10, 2^11) (e = 0 for a zero bound). The same float computation runs on
+// the host to check the stored maxima after the sweep.
+__host__ __device__ inline int mu_exp(float bound)
+{
+    if (!(bound > 0.0f)) return 0;
+    int e = 0;
+    (void)frexpf(bound, &e);  // bound in [2^(e-1), 2^e)
+    return std::min(std::max(11 - e, -120), 120);
+}
+
+__device__ __forceinline__ float pow2f(int e) { return __uint_as_float((uint32_t)(127 + e) << 23); }
+
 // S: storage type of the forward states; M: type of the adjoint fields.
 template <class S, class M> struct AdjArgs {
     const M *mu;     // mu_k
@@ -97,14 +120,58 @@ template <class S, class M> struct AdjArgs {
     // term (row >> lsh) * lstride and the lane term of the x shift, as in
     // the forward step (rows are whole storage blocks).
     const uint32_t *hot;
-    // cold cells (computed, not hot) in cell order, taken by the blocks of
-    // the first cold_planes z planes of the grid
+    // cold cells (computed, not hot) in cell order
     const uint32_t *cold;
     int64_t ncold;
-    int cold_planes;
     int64_t muo[Q], spo[Q], sto[Q];
     int lsh, lstride, lmask;
+    // binary16 fields: mx[j] = max |mu_j| as float bits, j = 0 .. N (filled
+    // by the launch that writes mu_j; the starting field mu_{N-1} is zero and
+    // mx[N - 1] = mx[N] = 0), k the reverse step, src_in / src_out =
+    // 2 |Fbar|_1 of steps k + 1 and k (0 for step N)
+    uint32_t *mx;
+    int k;
+    float src_in, src_out;
 };
+
+// Multipliers of one launch: mu_k = stored * si, stored mu_{k-1} = mu * so.
+struct MuScale {
+    float si = 1.0f, so = 1.0f;
+};
+
+template <class S, class M> __device__ __forceinline__ MuScale mu_scale(const AdjArgs<S, M> &a)
+{
+    MuScale m;
+    if constexpr (std::is_same<M, __half>::value) {
+        m.si = pow2f(-mu_exp(__uint_as_float(a.mx[a.k + 1]) + a.src_in));
+        m.so = pow2f(mu_exp(__uint_as_float(a.mx[a.k]) + a.src_out));
+    }
+    return m;
+}
+
+// Adjoint field value as T.
+template <class T> __device__ __forceinline__ T mval(const float *p, int64_t k, float) { return (T)p[k]; }
+template <class T> __device__ __forceinline__ T mval(const double *p, int64_t k, float) { return (T)p[k]; }
+template <class T> __device__ __forceinline__ T mval(const __half *p, int64_t k, float si)
+{
+    return (T)(__half2float(p[k]) * si);
+}
+template <class M, class T> __device__ __forceinline__ M mstore(T v, float so)
+{
+    if constexpr (std::is_same<M, __half>::value) return __float2half_rn((float)v * so);
+    else return (M)v;
+}
+
+// Raise the recorded max |mu| of a field to the wave's largest m (every lane
+// of the wave takes part; blocks are whole waves).
+__device__ __forceinline__ void mu_max(uint32_t *dst, float m)
+{
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+    if ((threadIdx.x & 63) == 0 &&
+        __float_as_uint(m) > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(dst, __float_as_uint(m));
+}
 
 __device__ __forceinline__ bool bitset(const uint32_t *bits, int nwx, int ny, int x, int y, int z)
 {
@@ -144,6 +211,11 @@ template <class T, class E> __device__ __forceinline__ T ld_el(const E *base, in
 {
     return sval<T>((const E *)((const char *)(base + (uoff + rb)) + lane), 0);
 }
+template <class T, class E>
+__device__ __forceinline__ T ld_mu(const E *base, int64_t uoff, int64_t rb, uint32_t lane, float si)
+{
+    return mval<T>((const E *)((const char *)(base + (uoff + rb)) + lane), 0, si);
+}
 
 // Workgroups of at most ADJ_THREADS threads: the cold path holds several
 // 19-value arrays at once, and a 256-thread workgroup lets it keep them in
@@ -163,7 +235,7 @@ inline int adj_block(int nx) { return std::min(wt::row_block(nx), ADJ_THREADS); 
 // computed read mu = 0 there (mu is only ever written at computed cells) or
 // are masked by a select.
 template <class S, class M, class T, bool LES, bool TRT>
-__device__ __forceinline__ void adj_cold(const AdjArgs<S, M> &a, int x, int y, int z)
+__device__ __forceinline__ float adj_cold(const AdjArgs<S, M> &a, const MuScale &sc, int x, int y, int z)
 {
     const int64_t i = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
     const float wi = a.wgt ? a.wgt[i] : 0.0f;
@@ -173,8 +245,8 @@ __device__ __forceinline__ void adj_cold(const AdjArgs<S, M> &a, int x, int y, i
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int64_t n1 = i + off_of(q, a.nx, a.ny);
-        mun[q] = (T)a.mu[pix(q, n1, a.lay)];
-        muh[q] = (T)a.mu[pix(q, i, a.lay)];
+        mun[q] = mval<T>(a.mu, pix(q, n1, a.lay), sc.si);
+        muh[q] = mval<T>(a.mu, pix(q, i, a.lay), sc.si);
         sj[q] = a.solid[n1];
         wj[q] = a.wgt ? a.wgt[n1] : 0.0f;
     }
@@ -219,12 +291,12 @@ __device__ __forceinline__ void adj_cold(const AdjArgs<S, M> &a, int x, int y, i
         }
         a.dJdw[i] += g;
     }
-    if (!a.Sp) return;
+    if (!a.Sp) return 0.0f;
     if (x == a.nx - 2) {
         // the outlet copies this cell: fold in lambda of x = nx-1
 #pragma unroll
         for (int q = 0; q < Q; q++)
-            if (EX[q] != 1) lam[q] += (T)a.mu[pix(q, i + 1 + off_of(q, a.nx, a.ny), a.lay)];
+            if (EX[q] != 1) lam[q] += mval<T>(a.mu, pix(q, i + 1 + off_of(q, a.nx, a.ny), a.lay), sc.si);
     }
     // pre-collision populations of step k-1 (gray-aware pull from j = i - c_q = i + c_qbar)
     T spj[Q], sph[Q];
@@ -248,58 +320,80 @@ __device__ __forceinline__ void adj_cold(const AdjArgs<S, M> &a, int x, int y, i
     }
     T mu[Q];
     adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
+    float m = 0.0f;
 #pragma unroll
-    for (int q = 0; q < Q; q++) a.mun[pix(q, i, a.lay)] = (M)mu[q];
+    for (int q = 0; q < Q; q++) {
+        a.mun[pix(q, i, a.lay)] = mstore<M>(mu[q], sc.so);
+        m = fmaxf(m, fabsf((float)mu[q]));
+    }
+    return m;
 }
 
-// One reverse step. The first cold_planes z planes of the grid are cold
-// blocks, one listed cold cell per thread, dispatched first so their long
-// dependent load chains overlap the rest; the other blocks are one per
-// lattice row (as the forward step) and take the hot cells on the
-// straight path.
+// One hot cell: no gray weight in reach and every neighbour computed, so
+// lambda_k is the plain gather of mu_k and f_{k-1} the plain pull of s_{k-1}.
+// Returns max |mu_{k-1}| of the cell.
+template <class S, class M, class T, bool LES, bool TRT>
+__device__ __forceinline__ float adj_hot(const AdjArgs<S, M> &a, const MuScale &sc, int x, int y, int z)
+{
+    const int64_t row = (int64_t)a.nx * (y + (int64_t)a.ny * z);
+    const int64_t rb = (row >> a.lsh) * a.lstride;
+    auto lane = [&](int xs) { return (uint32_t)((xs >> a.lsh) * a.lstride + (xs & a.lmask)); };
+    const uint32_t e[3] = {lane(x - 1), lane(x), lane(x + 1)};
+    // byte offsets of the lanes in the adjoint (M) and state (S) arrays
+    const uint32_t o[3] = {e[0] * (uint32_t)sizeof(M), e[1] * (uint32_t)sizeof(M), e[2] * (uint32_t)sizeof(M)};
+    const uint32_t os[3] = {e[0] * (uint32_t)sizeof(S), e[1] * (uint32_t)sizeof(S), e[2] * (uint32_t)sizeof(S)};
+    T lam[Q], f[Q], mu[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        lam[q] = ld_mu<T>(a.mu, a.muo[q], rb, o[1 + EX[q]], sc.si);
+        f[q] = ld_el<T>(a.Sp, a.spo[q], rb, os[1 - EX[q]]);
+    }
+    if (x == a.nx - 2) {
+        // the outlet copies this cell: fold in lambda of x = nx-1, whose
+        // links leaving the lattice (e_x = +1) carry nothing
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            if (EX[q] != 1) lam[q] += ld_mu<T>(a.mu, a.muo[q], rb, o[2 + EX[q]], sc.si);
+    }
+    adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
+    float m = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        *(M *)((char *)(a.mun + (a.sto[q] + rb)) + o[1]) = mstore<M>(mu[q], sc.so);
+        m = fmaxf(m, fabsf((float)mu[q]));
+    }
+    return m;
+}
+
+// One reverse step is two launches: the cold cells (one listed cell per
+// thread; few, with long dependent load chains and many live values) and
+// the hot cells (one block per lattice row, as the forward step, on the
+// straight path). Two kernels let the hot one keep its own register budget,
+// so it runs at several times the cold one's occupancy.
+template <class S, class M, class T, bool LES, bool TRT>
+__global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_cold(AdjArgs<S, M> a)
+{
+    const MuScale sc = mu_scale(a);
+    float m = 0.0f;
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < a.ncold) {
+        const int64_t i = a.cold[k];
+        const int x = (int)(i % a.nx), y = (int)((i / a.nx) % a.ny), z = (int)(i / ((int64_t)a.nx * a.ny));
+        m = adj_cold<S, M, T, LES, TRT>(a, sc, x, y, z);
+    }
+    if constexpr (std::is_same<M, __half>::value)
+        if (a.Sp) mu_max(a.mx + (a.k - 1), m);
+}
+
 template <class S, class M, class T, bool LES, bool TRT>
 __global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S, M> a)
 {
-    if ((int)blockIdx.z < a.cold_planes) {
-        const int64_t nb = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-        const int64_t k = nb * blockDim.x + threadIdx.x;
-        if (k >= a.ncold) return;
-        const int64_t i = a.cold[k];
-        const int x = (int)(i % a.nx), y = (int)((i / a.nx) % a.ny), z = (int)(i / ((int64_t)a.nx * a.ny));
-        adj_cold<S, M, T, LES, TRT>(a, x, y, z);
-        return;
-    }
+    const MuScale sc = mu_scale(a);
+    float m = 0.0f;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = (int)blockIdx.y, z = (int)blockIdx.z - a.cold_planes;
-    if (x >= a.nx || !bitset(a.hot, a.nwx, a.ny, x, y, z)) return;
-    {
-        // no gray weight in reach and every neighbour computed: lambda_k is
-        // the plain gather of mu_k, f_{k-1} the plain pull of s_{k-1}
-        if (!a.Sp) return;
-        const int64_t row = (int64_t)a.nx * (y + (int64_t)a.ny * z);
-        const int64_t rb = (row >> a.lsh) * a.lstride;
-        auto lane = [&](int xs) { return (uint32_t)((xs >> a.lsh) * a.lstride + (xs & a.lmask)); };
-        const uint32_t e[3] = {lane(x - 1), lane(x), lane(x + 1)};
-        // byte offsets of the lanes in the adjoint (M) and state (S) arrays
-        const uint32_t o[3] = {e[0] * (uint32_t)sizeof(M), e[1] * (uint32_t)sizeof(M), e[2] * (uint32_t)sizeof(M)};
-        const uint32_t os[3] = {e[0] * (uint32_t)sizeof(S), e[1] * (uint32_t)sizeof(S), e[2] * (uint32_t)sizeof(S)};
-        T lam[Q], f[Q], mu[Q];
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            lam[q] = ld_el<T>(a.mu, a.muo[q], rb, o[1 + EX[q]]);
-            f[q] = ld_el<T>(a.Sp, a.spo[q], rb, os[1 - EX[q]]);
-        }
-        if (x == a.nx - 2) {
-            // the outlet copies this cell: fold in lambda of x = nx-1, whose
-            // links leaving the lattice (e_x = +1) carry nothing
-#pragma unroll
-            for (int q = 0; q < Q; q++)
-                if (EX[q] != 1) lam[q] += ld_el<T>(a.mu, a.muo[q], rb, o[2 + EX[q]]);
-        }
-        adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
-#pragma unroll
-        for (int q = 0; q < Q; q++) *(M *)((char *)(a.mun + (a.sto[q] + rb)) + o[1]) = (M)mu[q];
-    }
+    const int y = (int)blockIdx.y, z = (int)blockIdx.z;
+    if (x < a.nx && bitset(a.hot, a.nwx, a.ny, x, y, z)) m = adj_hot<S, M, T, LES, TRT>(a, sc, x, y, z);
+    if constexpr (std::is_same<M, __half>::value) mu_max(a.mx + (a.k - 1), m);
 }
 
 // Computed cells that are not hot: interior fluid cells next to a solid or
@@ -323,16 +417,26 @@ __global__ void __launch_bounds__(256) k_adj_cold_list(const uint32_t *sbits, co
 }
 
 
+template <class S, class M, class T, bool LES, bool TRT> void launch_adj_lt(wt_lattice *h, const AdjArgs<S, M> &a)
+{
+    if (a.ncold > 0)
+        hipLaunchKernelGGL((k_adjoint_cold<S, M, T, LES, TRT>), dim3((unsigned)((a.ncold + ADJ_THREADS - 1) / ADJ_THREADS)),
+                           dim3(ADJ_THREADS), 0, h->stream, a);
+    // at k = 0 only the gradient is left (no mu_{-1}): the hot cells carry none
+    if (a.Sp) {
+        const int bx = adj_block(h->nx), gx = (h->nx + bx - 1) / bx;
+        hipLaunchKernelGGL((k_adjoint<S, M, T, LES, TRT>), dim3(gx, h->ny, h->nz), dim3(bx), 0, h->stream, a);
+    }
+}
+
 template <class S, class M, class T>
 int launch_adj(wt_lattice *h, const AdjArgs<S, M> &a)
 {
-    const int bx = adj_block(h->nx), gx = (h->nx + bx - 1) / bx;
-    const dim3 grid(gx, h->ny, h->nz + a.cold_planes);
     const bool les = h->p.les, trt = h->p.trt;
-    if (les && trt) hipLaunchKernelGGL((k_adjoint<S, M, T, true, true>), grid, dim3(bx), 0, h->stream, a);
-    else if (les) hipLaunchKernelGGL((k_adjoint<S, M, T, true, false>), grid, dim3(bx), 0, h->stream, a);
-    else if (trt) hipLaunchKernelGGL((k_adjoint<S, M, T, false, true>), grid, dim3(bx), 0, h->stream, a);
-    else hipLaunchKernelGGL((k_adjoint<S, M, T, false, false>), grid, dim3(bx), 0, h->stream, a);
+    if (les && trt) launch_adj_lt<S, M, T, true, true>(h, a);
+    else if (les) launch_adj_lt<S, M, T, true, false>(h, a);
+    else if (trt) launch_adj_lt<S, M, T, false, true>(h, a);
+    else launch_adj_lt<S, M, T, false, false>(h, a);
     WT_HIP(hipGetLastError());
     return 0;
 }
@@ -428,13 +532,22 @@ int replay_window(int K, size_t st, int *W)
 template <class S, class M, class T>
 int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJdw, double *ms)
 {
+    constexpr bool H = std::is_same<M, __half>::value;
     const int N = R.N, K = R.K, nseg = R.nseg;
     const size_t st = R.st;                                  // one stored state
     const size_t sm = 19 * (size_t)h->ns * sizeof(M);        // one adjoint field
-    Buf mu0, mu1, dg;
+    Buf mu0, mu1, dg, mx;
     WT_HIP(hipMalloc(&mu0.p, sm));
     WT_HIP(hipMalloc(&mu1.p, sm));
     WT_HIP(hipMalloc(&dg.p, sizeof(double) * (size_t)h->n));
+    // source bound 2 |Fbar_k|_1 of each step (0 for k = N)
+    std::vector<float> src((size_t)N + 1, 0.0f);
+    for (int k = 0; k < N; k++)
+        src[(size_t)k] = (float)(2.0 * (std::fabs(Fbar[3 * k]) + std::fabs(Fbar[3 * k + 1]) + std::fabs(Fbar[3 * k + 2])));
+    if (H) {
+        WT_HIP(hipMalloc(&mx.p, sizeof(uint32_t) * ((size_t)N + 1)));
+        WT_HIP(hipMemsetAsync(mx.p, 0, sizeof(uint32_t) * ((size_t)N + 1), h->stream));
+    }
     const int64_t nwords = (int64_t)h->nwx * h->ny * h->nz;
     Buf hot;
     WT_HIP(hipMalloc(&hot.p, sizeof(uint32_t) * (size_t)nwords));
@@ -477,11 +590,7 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     a.hot = (const uint32_t *)hot.p;
     a.cold = (const uint32_t *)cold.p;
     a.ncold = ncold;
-    {
-        const int bx = adj_block(h->nx);
-        const int64_t row_blocks = (int64_t)((h->nx + bx - 1) / bx) * h->ny;
-        a.cold_planes = (int)(((ncold + bx - 1) / bx + row_blocks - 1) / row_blocks);
-    }
+    a.mx = (uint32_t *)mx.p;
     a.lsh = h->bsh;
     a.lstride = h->bsh ? Q << h->bsh : 1;
     a.lmask = h->bsh ? (1 << h->bsh) - 1 : 0;
@@ -531,6 +640,9 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
                 a.Sk = (const S *)rp[(size_t)(k - s0)];
                 a.Sp = k > 0 ? (const S *)rp[(size_t)(k - 1 - s0)] : nullptr;
                 for (int c = 0; c < 3; c++) a.Fb[c] = Fbar[3 * k + c];
+                a.k = k;
+                a.src_in = src[(size_t)k + 1];
+                a.src_out = src[(size_t)k];
                 if (int rc = launch_adj<S, M, T>(h, a)) return rc;
                 std::swap(mu_k, mu_n);
             }
@@ -540,10 +652,26 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     h->parity = parity;
     WT_HIP(hipEventRecord(h->ev1, h->stream));
     WT_HIP(hipMemcpyAsync(dJdw, dg.p, sizeof(double) * (size_t)h->n, hipMemcpyDeviceToHost, h->stream));
+    std::vector<uint32_t> mxh(H ? (size_t)N + 1 : 0);
+    if (H) WT_HIP(hipMemcpyAsync(mxh.data(), mx.p, sizeof(uint32_t) * ((size_t)N + 1), hipMemcpyDeviceToHost, h->stream));
     WT_HIP(hipStreamSynchronize(h->stream));
     float t = 0;
     WT_HIP(hipEventElapsedTime(&t, h->ev0, h->ev1));
     if (ms) *ms = t;
+    if (H) {
+        // every stored field mu_j (j = N-2 .. 0) within binary16's range:
+        // values from 65520 on round to infinity
+        auto fl = [](uint32_t b) { float f; std::memcpy(&f, &b, 4); return f; };
+        for (int j = N - 2; j >= 0; j--) {
+            const float mj = fl(mxh[(size_t)j]);
+            const int e = mu_exp(fl(mxh[(size_t)j + 1]) + src[(size_t)j + 1]);
+            if (!(std::ldexp((double)mj, e) < 65520.0))
+                return wt_fail(WT_E_STATE,
+                               "binary16 adjoint field of step %d out of range (max |mu| %g, scale 2^%d): "
+                               "set WT_ADJ_FIELD=f32 for float fields",
+                               j, (double)mj, e);
+        }
+    }
     return 0;
 }
 
@@ -625,7 +753,11 @@ int wt_adjoint_backward(wt_lattice *h, int nsteps, const double *Fbar, double *d
     if (R->epoch != h->epoch || R->storage != h->storage || R->arith != h->arith)
         return wt_fail(WT_E_STATE, "the lattice configuration changed since the recorded forward pass");
     if (h->storage == WT_STORE_F64) return reverse_t<double, double, double>(h, *R, Fbar, dJdw, reverse_ms);
-    if (h->storage == WT_STORE_F16) return reverse_t<__half, float, float>(h, *R, Fbar, dJdw, reverse_ms);
+    if (h->storage == WT_STORE_F16) {
+        const char *f = std::getenv("WT_ADJ_FIELD");
+        if (f && std::strcmp(f, "f32") == 0) return reverse_t<__half, float, float>(h, *R, Fbar, dJdw, reverse_ms);
+        return reverse_t<__half, __half, float>(h, *R, Fbar, dJdw, reverse_ms);
+    }
     if (h->arith == WT_ARITH_F64) return reverse_t<float, float, double>(h, *R, Fbar, dJdw, reverse_ms);
     return reverse_t<float, float, float>(h, *R, Fbar, dJdw, reverse_ms);
 }
