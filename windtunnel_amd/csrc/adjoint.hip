@@ -42,6 +42,7 @@
 #include "wt_internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -78,9 +79,13 @@ __device__ __forceinline__ void adj_collide(const T (&f)[Q], const T (&lam)[Q], 
 
 // Stored state value (the population offset f - w) as T.
 template <class T, class E> __device__ __forceinline__ T sval(const E *p, int64_t k) { return (T)p[k]; }
+// A binary16 value times a float scale: fma(h, c, +0) is one mixed-precision
+// fma (v_fma_mix_f32), the same value as the product but for the sign of a
+// zero result.
+__device__ __forceinline__ float scaled_half(__half h, float c) { return __builtin_fmaf(__half2float(h), c, 0.0f); }
 template <class T> __device__ __forceinline__ T sval(const __half *p, int64_t k)
 {
-    return (T)(__half2float(p[k]) * wt::F16_INV_SCALE);
+    return (T)scaled_half(p[k], wt::F16_INV_SCALE);
 }
 
 // Exponent of the scale of a binary16 adjoint field: 2^e * bound lies in
@@ -123,7 +128,8 @@ template <class S, class M> struct AdjArgs {
     // cold cells (computed, not hot) in cell order
     const uint32_t *cold;
     int64_t ncold;
-    int64_t muo[Q], spo[Q], sto[Q];
+    // (32 bits, which halves the scalar registers: checked on the host)
+    int32_t muo[Q], spo[Q], sto[Q];
     int lsh, lstride, lmask;
     // binary16 fields: mx[j] = max |mu_j| as float bits, j = 0 .. N (filled
     // by the launch that writes mu_j; the starting field mu_{N-1} is zero and
@@ -154,7 +160,7 @@ template <class T> __device__ __forceinline__ T mval(const float *p, int64_t k, 
 template <class T> __device__ __forceinline__ T mval(const double *p, int64_t k, float) { return (T)p[k]; }
 template <class T> __device__ __forceinline__ T mval(const __half *p, int64_t k, float si)
 {
-    return (T)(__half2float(p[k]) * si);
+    return (T)scaled_half(p[k], si);
 }
 template <class M, class T> __device__ __forceinline__ M mstore(T v, float so)
 {
@@ -423,10 +429,9 @@ template <class S, class M, class T, bool LES, bool TRT> void launch_adj_lt(wt_l
         hipLaunchKernelGGL((k_adjoint_cold<S, M, T, LES, TRT>), dim3((unsigned)((a.ncold + ADJ_THREADS - 1) / ADJ_THREADS)),
                            dim3(ADJ_THREADS), 0, h->stream, a);
     // at k = 0 only the gradient is left (no mu_{-1}): the hot cells carry none
-    if (a.Sp) {
-        const int bx = adj_block(h->nx), gx = (h->nx + bx - 1) / bx;
-        hipLaunchKernelGGL((k_adjoint<S, M, T, LES, TRT>), dim3(gx, h->ny, h->nz), dim3(bx), 0, h->stream, a);
-    }
+    if (!a.Sp) return;
+    const int bx = adj_block(h->nx), gx = (h->nx + bx - 1) / bx;
+    hipLaunchKernelGGL((k_adjoint<S, M, T, LES, TRT>), dim3(gx, h->ny, h->nz), dim3(bx), 0, h->stream, a);
 }
 
 template <class S, class M, class T>
@@ -598,9 +603,12 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
         // slot q of the row-0 cell shifted by dyz cells (a multiple of nx, so of the block)
         const int64_t dyz = (int64_t)h->nx * (EY[q] + (int64_t)h->ny * EZ[q]);
         const int64_t slot = h->bsh ? (int64_t)wt::SLOT[q] << h->bsh : wt::SLOT[q] * h->ns;
-        a.muo[q] = slot + (dyz >> h->bsh) * a.lstride;
-        a.spo[q] = slot + ((-dyz) >> h->bsh) * a.lstride;
-        a.sto[q] = slot;
+        const int64_t mo = slot + (dyz >> h->bsh) * a.lstride, so = slot + ((-dyz) >> h->bsh) * a.lstride;
+        if (std::max({std::llabs(mo), std::llabs(so), std::llabs(slot)}) > INT32_MAX)
+            return wt_fail(WT_E_ARG, "lattice too large for the adjoint's 32-bit offsets");
+        a.muo[q] = (int32_t)mo;
+        a.spo[q] = (int32_t)so;
+        a.sto[q] = (int32_t)slot;
     }
     void *mu_k = mu0.p, *mu_n = mu1.p;
     // the replays below drive the forward kernel on explicit buffers; the

@@ -131,10 +131,14 @@ __host__ __device__ __forceinline__ float fast_sqrt(float x)
 //   pair's direction), D_p = nb_a - nb_b = -omm (lam_a - lam_b),
 //   sum over the pair of e-gradients = -w_p S_p, their difference -w_p D_p.
 // Pair directions: x, y, z, x+y, x-y, x+z, x-z, y+z, y-z (a = 1, 3, ..., 17).
+// Products and sums contract into fmas here (unlike the binary64 path,
+// which must round like the reference): the reverse step is bound by its
+// vector instruction count.
 template <bool LES, bool TRT>
 __host__ __device__ __forceinline__ void collide_adj_fast(const float (&f)[Q], const float (&lam)[Q], float tau0,
                                                           float K, float lamb, float (&mu)[Q])
 {
+#pragma clang fp contract(fast)
     float s[9], d[9];
 #pragma unroll
     for (int p = 0; p < 9; p++) { s[p] = f[2 * p + 1] + f[2 * p + 2]; d[p] = f[2 * p + 1] - f[2 * p + 2]; }
@@ -231,9 +235,10 @@ __host__ __device__ __forceinline__ void collide_adj_fast(const float (&f)[Q], c
     mu[0] = lam[0] + nb0 + drb;
 #pragma unroll
     for (int p = 0; p < 9; p++) {
-        const float nba = 0.5f * (S[p] + D[p]), nbb = 0.5f * (S[p] - D[p]);
-        mu[2 * p + 1] = lam[2 * p + 1] + nba + drb + cj[p];
-        mu[2 * p + 2] = lam[2 * p + 2] + nbb + drb - cj[p];
+        // mu_a = lam_a + (S + D)/2 + drb + cj, mu_b = lam_b + (S - D)/2 + drb - cj
+        const float h = __builtin_fmaf(0.5f, S[p], drb), e = __builtin_fmaf(0.5f, D[p], cj[p]);
+        mu[2 * p + 1] = lam[2 * p + 1] + (h + e);
+        mu[2 * p + 2] = lam[2 * p + 2] + (h - e);
     }
 }
 
