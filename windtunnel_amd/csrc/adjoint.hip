@@ -125,9 +125,11 @@ template <class S, class M> struct AdjArgs {
     // term (row >> lsh) * lstride and the lane term of the x shift, as in
     // the forward step (rows are whole storage blocks).
     const uint32_t *hot;
-    // cold cells (computed, not hot) in cell order
+    // cold cells (computed, not hot) in cell order; in the fused launch
+    // they are taken by the blocks of the first cold_planes z planes
     const uint32_t *cold;
     int64_t ncold;
+    int cold_planes;
     // (32 bits, which halves the scalar registers: checked on the host)
     int32_t muo[Q], spo[Q], sto[Q];
     int lsh, lstride, lmask;
@@ -402,6 +404,34 @@ __global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S, M> a)
     if constexpr (std::is_same<M, __half>::value) mu_max(a.mx + (a.k - 1), m);
 }
 
+// The fused form of one reverse step: one launch whose first cold_planes z
+// planes of blocks take the listed cold cells (dispatched first, so their
+// long dependent load chains overlap the rows) and whose other blocks take
+// one lattice row each, on the straight path. Every wave then carries the
+// cold path's register budget.
+template <class S, class M, class T, bool LES, bool TRT>
+__global__ void WT_ADJ_ATTR k_adjoint_fused(AdjArgs<S, M> a)
+{
+    const MuScale sc = mu_scale(a);
+    float m = 0.0f;
+    if ((int)blockIdx.z < a.cold_planes) {
+        const int64_t nb = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+        const int64_t k = nb * blockDim.x + threadIdx.x;
+        if (k < a.ncold) {
+            const int64_t i = a.cold[k];
+            const int x = (int)(i % a.nx), y = (int)((i / a.nx) % a.ny), z = (int)(i / ((int64_t)a.nx * a.ny));
+            m = adj_cold<S, M, T, LES, TRT>(a, sc, x, y, z);
+        }
+    } else {
+        // row blocks exist only when there is a step k - 1 (a.Sp set)
+        const int x = blockIdx.x * blockDim.x + threadIdx.x;
+        const int y = (int)blockIdx.y, z = (int)blockIdx.z - a.cold_planes;
+        if (x < a.nx && bitset(a.hot, a.nwx, a.ny, x, y, z)) m = adj_hot<S, M, T, LES, TRT>(a, sc, x, y, z);
+    }
+    if constexpr (std::is_same<M, __half>::value)
+        if (a.Sp) mu_max(a.mx + (a.k - 1), m);
+}
+
 // Computed cells that are not hot: interior fluid cells next to a solid or
 // gray cell, and gray cells (any order; each cell is independent).
 __global__ void __launch_bounds__(256) k_adj_cold_list(const uint32_t *sbits, const uint32_t *hot, uint32_t *cold,
@@ -425,6 +455,14 @@ __global__ void __launch_bounds__(256) k_adj_cold_list(const uint32_t *sbits, co
 
 template <class S, class M, class T, bool LES, bool TRT> void launch_adj_lt(wt_lattice *h, const AdjArgs<S, M> &a)
 {
+    if (a.cold_planes >= 0) {
+        // fused: cold planes first, then (with a step k - 1) the rows
+        const int bx = adj_block(h->nx), gx = (h->nx + bx - 1) / bx;
+        const int gz = a.cold_planes + (a.Sp ? h->nz : 0);
+        if (gz > 0)
+            hipLaunchKernelGGL((k_adjoint_fused<S, M, T, LES, TRT>), dim3(gx, h->ny, gz), dim3(bx), 0, h->stream, a);
+        return;
+    }
     if (a.ncold > 0)
         hipLaunchKernelGGL((k_adjoint_cold<S, M, T, LES, TRT>), dim3((unsigned)((a.ncold + ADJ_THREADS - 1) / ADJ_THREADS)),
                            dim3(ADJ_THREADS), 0, h->stream, a);
@@ -595,6 +633,18 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     a.hot = (const uint32_t *)hot.p;
     a.cold = (const uint32_t *)cold.p;
     a.ncold = ncold;
+    // One launch per reverse step (cold cells as the first planes of the
+    // row grid) or two (cold cells, then the rows with a register budget of
+    // their own). Measured on MI355X: one launch is faster with float
+    // fields, two with binary16 fields. WT_ADJ_FUSE=0/1 picks one.
+    bool fuse = !H;
+    if (const char *e = std::getenv("WT_ADJ_FUSE")) fuse = std::atoi(e) != 0;
+    a.cold_planes = -1;
+    if (fuse) {
+        const int bx = adj_block(h->nx), gx = (h->nx + bx - 1) / bx;
+        const int64_t per_plane = (int64_t)gx * h->ny * bx;
+        a.cold_planes = (int)((ncold + per_plane - 1) / per_plane);
+    }
     a.mx = (uint32_t *)mx.p;
     a.lsh = h->bsh;
     a.lstride = h->bsh ? Q << h->bsh : 1;
