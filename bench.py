@@ -807,7 +807,8 @@ def main():
         out["exact"] = {"what": "float32 storage, binary64 collision arithmetic in the reference's expression order "
                                 "(bit for bit the reference's populations)",
                         "value": round(rx["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(rx["ms_per_step"], 4),
-                        "kernel_ms": round(rx["kernel_ms"], 5), "roofline_frac": round(rx["achieved_gbs"] / HBM_PEAK_GBS, 4)}
+                        "kernel_ms": round(rx["kernel_ms"], 5), "roofline_frac": round(rx["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                        "traffic": traffic.get("exact")}
     if not a.no_micro:
         log("micro 512x256x256, no body, random populations")
         out["micro"] = run_micro()
