@@ -250,10 +250,13 @@ template <class S, class MAC> struct StepArgs {
     const S *lp[Q];
     S *sp[Q];
     int lsh, lstride, lmask;
-    // binary16 pair launch (k_step_pair): the load bases of the directions
-    // PAIR_DIRS, and each direction's equilibrium-plane value as a binary16
-    // bit pattern in both halves of a word
-    const S *fixp[10];
+    // pair launch (k_step_pair): byte offset of direction q's pull slot
+    // from the row's load base F + row term + lmin (lmin <= 0 is the lowest
+    // of those slots, so every offset is non-negative), and each
+    // direction's equilibrium-plane value as a binary16 bit pattern in both
+    // halves of a word
+    int32_t loff[Q];
+    int64_t lmin;
     uint32_t eqw[Q];
 };
 
@@ -882,6 +885,34 @@ constexpr int PAIR_DIRS[10] = {1, 7, 9, 11, 13, 2, 8, 10, 12, 14};  // e_x = +1 
 #define WT_PAIR_WAVES 4
 #endif
 
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
+// no swizzle, no index stride).
+constexpr int RAW_BUFFER_FLAGS = 0x00020000;
+typedef unsigned int wt_u32x2 __attribute__((ext_vector_type(2)));
+// One pair word through a buffer resource: voff is the lane's byte offset,
+// soff the uniform per-direction offset.
+template <class W> __device__ __forceinline__ W pair_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <> __device__ __forceinline__ uint32_t pair_load<uint32_t>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+template <> __device__ __forceinline__ float2 pair_load<float2>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    const wt_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ __forceinline__ void pair_store(uint32_t v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+}
+__device__ __forceinline__ void pair_store(float2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    wt_u32x2 u;
+    u.x = __float_as_uint(v.x);
+    u.y = __float_as_uint(v.y);
+    __builtin_amdgcn_raw_buffer_store_b64(u, r, voff, soff, 0);
+}
+
 // Float32 storage the same way: a pair is two 4-byte values (one 8-byte
 // access), the x shifts move one of them across lanes, and the edge values
 // are single elements.
@@ -928,28 +959,37 @@ k_step_pair(StepArgs<S, float> a)
         return __umul24((unsigned)(xs >> a.lsh), (unsigned)a.lstride) + (unsigned)(xs & a.lmask);
     };
     const uint32_t eb = (uint32_t)sizeof(S) * elem(x);  // byte offset of the pair (x, x + 1)
+    // Buffer resources over the row's slots: every access is a scalar base,
+    // a scalar per-direction offset and the lane's byte offset eb, so the
+    // 19 directions cost no vector address arithmetic and two 64-bit bases
+    // instead of 38 (raw buffers, no range check; the offsets stay below
+    // 2^31 and every address lies inside the population buffers).
+    const __amdgpu_buffer_rsrc_t rf = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)((const char *)a.F + (rb * (int64_t)sizeof(S) + a.lmin)), 0, -1, RAW_BUFFER_FLAGS);
+    const __amdgpu_buffer_rsrc_t rg =
+        __builtin_amdgcn_make_buffer_rsrc((void *)((char *)a.G + rb * (int64_t)sizeof(S)), 0, -1, RAW_BUFFER_FLAGS);
     // every lane of the row pulls (the shifted pairs need their neighbours)
     Word w[Q];
 #pragma unroll
-    for (int q = 0; q < Q; q++) w[q] = *(const Word *)((const char *)(a.lp[q] + rb) + eb);
-    uint32_t fv;
+    for (int q = 0; q < Q; q++) w[q] = pair_load<Word>(rf, (int)eb, a.loff[q]);
+    // the value beyond the wave's edge of each direction PAIR_DIRS[k]
+    // (before it for e_x = +1, after it for -1; clamped into the row at its
+    // ends, where no cell uses it): one uniform load per direction, the
+    // same word in every lane; binary16 loads the aligned word that holds it
+    uint32_t ev[10];
     {
-        // lanes 0..9: the value beyond the wave's edge of direction
-        // PAIR_DIRS[lane] (before it for e_x = +1, after it for -1; clamped
-        // into the row at its ends, where no cell uses it); binary16 loads
-        // the aligned word that holds it
-        const int li = lane < 10 ? lane : 0;
+        const int xw = __builtin_amdgcn_readfirstlane(x0);
         const int lo = HALF ? 2 : 1;
-        const int xf = li < 5 ? (x0 >= lo ? x0 - lo : 0) : (x0 + 128 <= nx - lo ? x0 + 128 : nx - lo);
-        const S *fp = a.fixp[0];  // a select chain: no dynamic index into the arguments
+        const int eb_lo = (int)(sizeof(S) * elem(xw >= lo ? xw - lo : 0));
+        const int eb_hi = (int)(sizeof(S) * elem(xw + 128 <= nx - lo ? xw + 128 : nx - lo));
 #pragma unroll
-        for (int k = 1; k < 10; k++) fp = li == k ? a.fixp[k] : fp;
-        fv = *(const uint32_t *)((const char *)(fp + rb) + sizeof(S) * elem(xf));
+        for (int k = 0; k < 10; k++)
+            ev[k] = __builtin_amdgcn_raw_buffer_load_b32(rf, 0, a.loff[PAIR_DIRS[k]] + (k < 5 ? eb_lo : eb_hi), 0);
     }
 #pragma unroll
     for (int k = 0; k < 10; k++) {
         const int q = PAIR_DIRS[k];
-        const int edge = __builtin_amdgcn_readlane((int)fv, k);
+        const int edge = (int)ev[k];
         if constexpr (HALF) {
             if (k < 5) {
                 // pull from x - 1: (high half of the lane below, low half of this lane)
@@ -999,8 +1039,11 @@ k_step_pair(StepArgs<S, float> a)
     const int64_t iA = row + x;
     // the pair to store: cell A (the equilibrium value at x = 0) and cell B
     // (at x + 1 = nx - 1, the outlet: a copy of cell A)
-    auto word = [&](auto qc) -> Word {
+    // ENDS: the wave holds the row's first or last pair (equilibrium inlet
+    // value, outlet copy); other waves store the collided pair as it is
+    auto word = [&](auto qc, auto ends) -> Word {
         constexpr int q = decltype(qc)::value;
+        constexpr bool ENDS = decltype(ends)::value;
         if constexpr (HALF) {
             const uint32_t r13A = cell_bits<S>(iA, a.seed) >> 19, r13B = cell_bits<S>(iA + 1, a.seed) >> 19;
             const uint32_t sel = bndA ? 0x07060100u : (outB ? 0x05040504u : 0x07060504u);
@@ -1016,8 +1059,10 @@ k_step_pair(StepArgs<S, float> a)
                                          : __builtin_amdgcn_cvt_pkrtz(__uint_as_float(ua), __uint_as_float(ub));
             uint32_t pw;
             __builtin_memcpy(&pw, &pk, 4);
-            return __builtin_amdgcn_perm(pw, a.eqw[q], sel);
+            if constexpr (!ENDS) return pw;
+            else return __builtin_amdgcn_perm(pw, a.eqw[q], sel);
         } else {
+            if constexpr (!ENDS) return make_float2(gA[q], gB[q]);
             const float va = bndA ? (float)a.eqv[q] : gA[q];
             return make_float2(va, outB ? va : gB[q]);
         }
@@ -1028,25 +1073,35 @@ k_step_pair(StepArgs<S, float> a)
     // from its near-wall block, a solid one copies itself into it in the
     // one-cell code below)
     const bool wA = bndA || (!solA && !nrA), wB = outB ? (!solA && !nrA) : (!solB && !nrB);
-    auto dst = [&](int q) { return (char *)(a.sp[q] + rb) + eb; };
+    // store offset of direction q (two buffers: its home slot in the block)
+    auto soff = [&](int q) { return (int)((SLOT[q] << a.lsh) * (int)sizeof(S)); };
     if (__builtin_expect(!__any(solA || solB || nrA || nrB), 1)) {
-        auto st = [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            *(Word *)dst(q) = word(qc);
-        };
-        WT_FOR_Q(st)
+        if (x0 != 0 && x0 + 128 < nx) {
+            // a wave away from both row ends: no equilibrium or outlet value
+            auto st = [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                pair_store(word(qc, std::false_type{}), rg, (int)eb, soff(q));
+            };
+            WT_FOR_Q(st)
+        } else {
+            auto st = [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                pair_store(word(qc, std::true_type{}), rg, (int)eb, soff(q));
+            };
+            WT_FOR_Q(st)
+        }
     } else {
         auto st = [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const Word v = word(qc);
+            const Word v = word(qc, std::true_type{});
             if (wA && wB) {
-                *(Word *)dst(q) = v;
+                pair_store(v, rg, (int)eb, soff(q));
             } else if constexpr (HALF) {
-                if (wA) *(uint16_t *)dst(q) = (uint16_t)v;
-                else if (wB) *(uint16_t *)(dst(q) + 2) = (uint16_t)(v >> 16);
+                if (wA) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, rg, (int)eb, soff(q), 0);
+                else if (wB) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(v >> 16), rg, (int)eb + 2, soff(q), 0);
             } else {
-                if (wA) *(float *)dst(q) = v.x;
-                else if (wB) *(float *)(dst(q) + 4) = v.y;
+                if (wA) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.x), rg, (int)eb, soff(q), 0);
+                else if (wB) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v.y), rg, (int)eb + 4, soff(q), 0);
             }
         };
         WT_FOR_Q(st)
@@ -1538,7 +1593,18 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
         // step 11 % slower on MI355X (0.989 against 0.892 ms at 512x256x256)
         const bool want = std::is_same<S, __half>::value ? h->pairs >= 1 : h->pairs >= 2;
         if (want && tpr > 0 && tpr % 64 == 0 && np * prow <= h->partial_cap) {
-            for (int k = 0; k < 10; k++) a.fixp[k] = a.lp[PAIR_DIRS[k]];
+            // pull offsets from the row's load base (two buffers: the
+            // upstream row's slot), as bytes from the lowest of them
+            int64_t lo[Q], lmin = 0;
+            for (int q = 0; q < Q; q++) {
+                lo[q] = ((int64_t)a.lp[q] - (int64_t)a.F);
+                lmin = std::min(lmin, lo[q]);
+            }
+            for (int q = 0; q < Q; q++) {
+                if (lo[q] - lmin > INT32_MAX) return wt_fail(WT_E_ARG, "lattice rows too far apart for the pair step");
+                a.loff[q] = (int32_t)(lo[q] - lmin);
+            }
+            a.lmin = lmin;
             for (int q = 0; q < Q; q++) {
                 const uint32_t b = __half_as_ushort(__float2half_rn((float)a.eqv[q] * F16_SCALE));
                 a.eqw[q] = b | (b << 16);
