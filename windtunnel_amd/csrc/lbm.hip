@@ -258,6 +258,11 @@ template <class S, class MAC> struct StepArgs {
     int32_t loff[Q];
     int64_t lmin;
     uint32_t eqw[Q];
+    // one-cell straight path through buffer resources (two buffers, when
+    // every offset fits 31 bits): store offset of direction q from G + row
+    // term, and whether loff/soff are valid
+    int32_t soff[Q];
+    int bufok;
 };
 
 // x shift of the cell whose slot holds the value a cell pulls along q (AB:
@@ -522,6 +527,44 @@ template <class S, class T> constexpr int step_waves()
                                                                                                              : WT_WAVES);
 }
 
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
+// no swizzle, no index stride).
+constexpr int RAW_BUFFER_FLAGS = 0x00020000;
+typedef unsigned int wt_u32x2 __attribute__((ext_vector_type(2)));
+// One stored population through a buffer resource (voff: the lane's byte
+// offset, soff: the uniform per-direction offset).
+template <class S> __device__ __forceinline__ S buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <> __device__ __forceinline__ float buf_ld<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <> __device__ __forceinline__ double buf_ld<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    const wt_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    return __hiloint2double((int)v.y, (int)v.x);
+}
+template <class S> __device__ __forceinline__ void buf_st(S v, __amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <> __device__ __forceinline__ void buf_st<float>(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+template <> __device__ __forceinline__ void buf_st<double>(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    wt_u32x2 u;
+    u.x = (unsigned)__double2loint(v);
+    u.y = (unsigned)__double2hiint(v);
+    __builtin_amdgcn_raw_buffer_store_b64(u, r, voff, soff, 0);
+}
+// Binary64 arithmetic (the reference's own numbers) takes the one-cell
+// straight path through buffer resources: fewer scalar instructions and
+// registers beside its longer collision (0.930 against 0.945 ms at
+// 512x256x256 on MI355X). Float arithmetic keeps the 64-bit scalar bases,
+// which measured 3 % faster there than buffer resources. Build switch
+// WT_BUF_STEP=0: 64-bit scalar bases for binary64 arithmetic too.
+#ifndef WT_BUF_STEP
+#define WT_BUF_STEP 1
+#endif
+
 // One near-wall cell (an interior fluid cell with a solid upstream
 // neighbour, a gray cell or a neighbour of one): halfway bounce-back, moving
 // wall, momentum exchange and gray links, then the collision of the
@@ -716,6 +759,36 @@ __device__ __forceinline__ void step_cell(const StepArgs<S, MAC> &a, int x, int 
             const uint32_t o[3] = {lane(x - 1), lane(x), lane(x + 1)};
             T f[Q];
             T g[Q], r, vx, vy, vz;
+#if WT_BUF_STEP
+            if constexpr (!ESO && !std::is_same<S, __half>::value && std::is_same<T, double>::value) {
+                if (a.bufok) {
+                    // the same pulls and stores through two buffer resources:
+                    // a scalar base per row and a scalar offset per direction
+                    const __amdgpu_buffer_rsrc_t rf = __builtin_amdgcn_make_buffer_rsrc(
+                        (void *)((const char *)a.F + (rb * (int64_t)sizeof(S) + a.lmin)), 0, -1, RAW_BUFFER_FLAGS);
+                    const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc(
+                        (void *)((char *)a.G + rb * (int64_t)sizeof(S)), 0, -1, RAW_BUFFER_FLAGS);
+#pragma unroll
+                    for (int q = 0; q < Q; q++) f[q] = (T)buf_ld<S>(rf, (int)o[1 - EX[q]], a.loff[q]);
+                    if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
+                        collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r, vx, vy, vz);
+                    else
+                        collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+#pragma unroll
+                    for (int q = 0; q < Q; q++) buf_st<S>((S)g[q], rg, (int)o[1], a.soff[q]);
+                    if (out) {
+                        // zero-gradient outlet: the same values again for x = nx-1
+#pragma unroll
+                        for (int q = 0; q < Q; q++) buf_st<S>((S)g[q], rg, (int)o[2], a.soff[q]);
+                    }
+                    if (MACRO) {
+                        a.rho[i] = (MAC)r; a.ux[i] = (MAC)vx; a.uy[i] = (MAC)vy; a.uz[i] = (MAC)vz;
+                        if (out) { a.rho[i + 1] = (MAC)r; a.ux[i + 1] = (MAC)vx; a.uy[i + 1] = (MAC)vy; a.uz[i + 1] = (MAC)vz; }
+                    }
+                    return;
+                }
+            }
+#endif
             if constexpr (HALF_UNITS) {
                 // binary16 storage: collide in binary16 units, no scaling,
                 // straight from the raw loads
@@ -885,10 +958,6 @@ constexpr int PAIR_DIRS[10] = {1, 7, 9, 11, 13, 2, 8, 10, 12, 14};  // e_x = +1 
 #define WT_PAIR_WAVES 4
 #endif
 
-// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
-// no swizzle, no index stride).
-constexpr int RAW_BUFFER_FLAGS = 0x00020000;
-typedef unsigned int wt_u32x2 __attribute__((ext_vector_type(2)));
 // One pair word through a buffer resource: voff is the lane's byte offset,
 // soff the uniform per-direction offset.
 template <class W> __device__ __forceinline__ W pair_load(__amdgpu_buffer_rsrc_t r, int voff, int soff);
@@ -1565,6 +1634,21 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
             a.sp[q] = (S *)((uintptr_t)a.G + (uintptr_t)(uoff(ss, sd) * (int64_t)sizeof(S)));
         }
     }
+    if (!ESO) {
+        // buffer-resource offsets of the straight path: pulls as bytes from
+        // the lowest pull slot of the row, stores from G + row term
+        int64_t lmin = 0;
+        for (int q = 0; q < Q; q++) lmin = std::min<int64_t>(lmin, (int64_t)a.lp[q] - (int64_t)a.F);
+        bool ok = true;
+        for (int q = 0; q < Q; q++) {
+            const int64_t lo = (int64_t)a.lp[q] - (int64_t)a.F - lmin, so = (int64_t)a.sp[q] - (int64_t)a.G;
+            ok = ok && lo <= INT32_MAX && so >= 0 && so <= INT32_MAX;
+            a.loff[q] = ok ? (int32_t)lo : 0;
+            a.soff[q] = ok ? (int32_t)so : 0;
+        }
+        a.lmin = lmin;
+        a.bufok = ok;
+    }
     const int bx = row_block(h->nx);
     const int gx = (h->nx + bx - 1) / bx;
     // the near-wall cells of rows z in [bz0, bz1), one per thread of the
@@ -1592,19 +1676,7 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
         // 4-byte values, twice the cells in flight per wave made the float32
         // step 11 % slower on MI355X (0.989 against 0.892 ms at 512x256x256)
         const bool want = std::is_same<S, __half>::value ? h->pairs >= 1 : h->pairs >= 2;
-        if (want && tpr > 0 && tpr % 64 == 0 && np * prow <= h->partial_cap) {
-            // pull offsets from the row's load base (two buffers: the
-            // upstream row's slot), as bytes from the lowest of them
-            int64_t lo[Q], lmin = 0;
-            for (int q = 0; q < Q; q++) {
-                lo[q] = ((int64_t)a.lp[q] - (int64_t)a.F);
-                lmin = std::min(lmin, lo[q]);
-            }
-            for (int q = 0; q < Q; q++) {
-                if (lo[q] - lmin > INT32_MAX) return wt_fail(WT_E_ARG, "lattice rows too far apart for the pair step");
-                a.loff[q] = (int32_t)(lo[q] - lmin);
-            }
-            a.lmin = lmin;
+        if (want && a.bufok && tpr > 0 && tpr % 64 == 0 && np * prow <= h->partial_cap) {
             for (int q = 0; q < Q; q++) {
                 const uint32_t b = __half_as_ushort(__float2half_rn((float)a.eqv[q] * F16_SCALE));
                 a.eqw[q] = b | (b << 16);
