@@ -46,3 +46,8 @@ def test_argument_errors_before_device_checks():
         wl.LBM(2, 8, 8)
     with pytest.raises(ValueError, match="unsupported storage"):
         wl.LBM(8, 8, 8, storage="f16", arith="f64")
+    # cell indices are 32-bit in the near-wall lists: beyond that, an error
+    with pytest.raises(ValueError, match="32-bit cell index"):
+        wl.LBM(70000, 60000, 8)
+    with pytest.raises(ValueError, match="too large"):
+        wl.LBM(1 << 24, 4, 4)

@@ -56,3 +56,6 @@ def test_sample_many_equals_oracle():
     want = np.array([O.sample(*p) for p in P])
     assert np.array_equal(got, want, equal_nan=True)
     assert np.array_equal(L.sample_many(np.zeros((0, 3))), np.zeros((0, 3)))
+    # the one-point sample() of the binding is the same device sample
+    for p in P[:8]:
+        assert np.array_equal(np.array(L.sample(*p)), np.array(O.sample(*p)), equal_nan=True)

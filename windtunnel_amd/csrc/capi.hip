@@ -156,10 +156,18 @@ int wt_create(int nx, int ny, int nz, int storage, int arith, int layout, wt_lat
     if (!out) return wt_fail(WT_E_ARG, "null output pointer");
     *out = nullptr;
     if (nx < 3 || ny < 3 || nz < 3) return wt_fail(WT_E_ARG, "lattice must be at least 3x3x3, got %dx%dx%d", nx, ny, nz);
-    if ((int64_t)nx * ny * nz > ((int64_t)1 << 31) / 19)
-        return wt_fail(WT_E_ARG, "lattice of %lld cells exceeds the 32-bit population index range",
+    // Cell indices in the near-wall and cold-cell lists are 32-bit, a lattice
+    // plane's cell count is an int, and the streaming kernels take y and z
+    // from the grid's y and z (at most 65535) and x offsets within a row from
+    // 24-bit products; population offsets are 64-bit everywhere. Beyond that
+    // the limit is device memory (about 1.5 G cells for float32 with two
+    // buffers, 2.7 G for binary16 on 288 GB), which the allocation reports.
+    if ((int64_t)nx * ny * nz > (int64_t)UINT32_MAX)
+        return wt_fail(WT_E_ARG, "lattice of %lld cells exceeds the 32-bit cell index range",
                        (long long)nx * ny * nz);
     if (ny > 65535 || nz > 65535) return wt_fail(WT_E_ARG, "ny and nz must be below 65536");
+    if (nx >= (1 << 24) || (int64_t)nx * ny > (int64_t)INT32_MAX)
+        return wt_fail(WT_E_ARG, "lattice plane of %dx%d cells is too large", nx, ny);
     const bool ok = (storage == WT_STORE_F32 && (arith == WT_ARITH_F32 || arith == WT_ARITH_F64)) ||
                     (storage == WT_STORE_F16 && arith == WT_ARITH_F32) ||
                     (storage == WT_STORE_F64 && arith == WT_ARITH_F64);

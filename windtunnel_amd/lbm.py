@@ -551,30 +551,13 @@ class LBM:
         _check(lib().wt_adjoint_release(self._h))
         self._adj_steps = 0
 
-    # ---- host accessors over the downloaded fields (js/lbm.js:515-545) ------
+    # ---- point queries (js/lbm.js:515-545) -------------------------------
     def sample(self, x, y, z, out=None):
+        """sample() at one point: the device's batched sample (wt_sample) with
+        a batch of one, on the fields of the last step."""
+        v = self.sample_many(np.array([x, y, z], dtype=np.float64))
         out = out if out is not None else [0.0, 0.0, 0.0]
-        nx, ny, nz = self.nx, self.ny, self.nz
-        if x < 0.5 or y < 0.5 or z < 0.5 or x > nx - 1.5 or y > ny - 1.5 or z > nz - 1.5:
-            out[0], out[1], out[2] = self.u0, 0.0, 0.0
-            return out
-        F = self.fields()
-        ux, uy, uz = F["ux"], F["uy"], F["uz"]
-        xf, yf, zf = x - 0.5, y - 0.5, z - 0.5
-        x0, y0, z0 = _toint32(xf), _toint32(yf), _toint32(zf)
-        tx, ty, tz = xf - x0, yf - y0, zf - z0
-        vx = vy = vz = 0.0
-        for k in range(2):
-            wz = tz if k else 1 - tz
-            for j in range(2):
-                wy = (ty if j else 1 - ty) * wz
-                base = nx * ((y0 + j) + ny * (z0 + k))
-                ia, ib = x0 + base, x0 + base + 1
-                wa, wb = (1 - tx) * wy, tx * wy
-                vx += float(ux[ia]) * wa + float(ux[ib]) * wb
-                vy += float(uy[ia]) * wa + float(uy[ib]) * wb
-                vz += float(uz[ia]) * wa + float(uz[ib]) * wb
-        out[0], out[1], out[2] = vx, vy, vz
+        out[0], out[1], out[2] = float(v[0]), float(v[1]), float(v[2])
         return out
 
     def isSolid(self, x, y, z) -> bool:
