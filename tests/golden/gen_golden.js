@@ -356,6 +356,42 @@ for (const shape of ['cube', 'sphere']) {
   console.log(`  ${name}: ${((Date.now() - t0) / 1000).toFixed(1)} s, cd=${(Fx / n / q).toFixed(4)}`);
 }
 
+// test.html procedure on the rest of its case table (test.html:37-49): the
+// shape, whether it stands on the ground and its angle of attack; the body is
+// placed, pitched and voxelized as the page does and replayed as a mask
+if (want('lbm_testpage_cases')) {
+  const t0 = Date.now();
+  const NX = 96, NY = 44, NZ = 44, WARM = 400, AVG = 300;
+  const table = [['cylinder', false, 0], ['teardrop', false, 0], ['sedan', true, 0], ['truck', true, 0],
+    ['wing', false, -8], ['wing', false, 0], ['wing', false, 8], ['wing', false, 16], ['airplane', false, 2]];
+  const cases = [];
+  for (const [shape, ground, aoa] of table) {
+    const L = new LBM(NX, NY, NZ);
+    L.tau = 0.5 + Math.pow(10, -1.1 - 2.3 * 0.62);
+    const b = new Shapes.Body(shape);
+    b.fit(NX, NY, NZ);
+    b.pos = [NX * 0.36, ground ? 1 : NY * 0.5, NZ * 0.5];
+    b.pitch = -aoa * Math.PI / 180;
+    const vox = voxelizeRecorded(L, b.makeTest(), b.box(), ground);
+    const planform = b.def.ref === 'planform';
+    const A = planform ? L.planformArea : L.frontalArea;
+    const forces = new Float64Array((WARM + AVG) * 3);
+    let Fx = 0, Fy = 0, n = 0;
+    for (let t = 0; t < WARM + AVG; t++) {
+      L.step();
+      forces.set(L.force, 3 * t);
+      if (t >= WARM) { Fx += L.force[0]; Fy += L.force[1]; n++; }
+    }
+    const q = 0.5 * L.u0 * L.u0 * A;
+    cases.push({ shape, ground, aoa, planform, area: A, voxelize: vox, forces: pack(forces, [WARM + AVG, 3]),
+      cd: Fx / n / q, cl: Fy / n / q, final: stateHashes(L) });
+    console.log(`  ${shape} aoa=${aoa}: cd=${(Fx / n / q).toFixed(4)} cl=${(Fy / n / q).toFixed(4)}`);
+  }
+  write('lbm_testpage_cases', { dims: [NX, NY, NZ], params: params(new LBM(NX, NY, NZ)), tau: 0.5 + Math.pow(10, -1.1 - 2.3 * 0.62),
+    warm: WARM, avg: AVG, cases });
+  console.log(`  lbm_testpage_cases: ${((Date.now() - t0) / 1000).toFixed(1)} s`);
+}
+
 // ------------------------------------------------------------ voxel cases
 function boxMesh(sx, sy, sz) {        // closed box soup, 12 triangles
   const h = [sx / 2, sy / 2, sz / 2], v = [];

@@ -207,3 +207,21 @@ def test_grid_lookup_meshcube():
     # frontal areas of mesh cube and analytic cube agree as testmodels.html requires (<6%)
     fa_m, fa_a = rec["result"]["frontalArea"], fx["cube"]["result"]["frontalArea"]
     assert abs(fa_m - fa_a) / fa_a < 0.06
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("k", [2, 6])
+def test_testpage_case_table(k):
+    """test.html's ground vehicle (sedan, on the road) and wing at 8 degrees
+    (planform area): every force and the final state bit for bit."""
+    fx = load("lbm_testpage_cases")
+    c = fx["cases"][k]
+    L = OracleLBM(*fx["dims"])
+    set_params(L, fx["params"])
+    L.set_params(tau=fx["tau"])
+    L.reset()
+    v = replay_voxelize(L, c["voxelize"])
+    assert (v["planformArea"] if c["planform"] else v["frontalArea"]) == c["area"]
+    f = L.step(fx["warm"] + fx["avg"])
+    assert np.array_equal(f, arr(c["forces"]))
+    check_hashes(L, c["final"])
