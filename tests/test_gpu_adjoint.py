@@ -259,3 +259,55 @@ def test_slab_count_limit_is_an_error():
     f = L.step_slabs(1, 4096)
     assert f.shape[0] == 1 and np.all(np.isfinite(f))
     L.close()
+
+
+def test_central_differences_at_the_differentiation_workload_size():
+    """The differentiation workload's lattice (192x96x96 TRT, tau 0.6,
+    u0 0.04, the seeded gray-walled ellipsoid) over 600 recorded steps,
+    J = mean C_d over the last 100: the binary64 adjoint against central
+    differences at eps = 1e-3 and their Richardson extrapolation from eps
+    and eps/2, on 4 seeded gray cells. The extrapolation removes the
+    differences' own O(eps^2) error, so it must match to 1e-5."""
+    from bench import gray_ellipsoid
+    dims, N, K, eps = (192, 96, 96), 600, 100, 1e-3
+    w = gray_ellipsoid(dims)
+    solid = (w >= 1).reshape(dims[::-1])
+    idx = np.argwhere(solid)
+    lo, hi = idx.min(0), idx.max(0)
+    box = [int(lo[2]), int(lo[1]), int(lo[0]), int(hi[2]), int(hi[1]), int(hi[0])]
+    mask = solid[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1].astype(np.uint8).reshape(-1)
+    wg = np.where(w >= 1, 0, w).astype(np.float32)
+    D = LBM(*dims, storage="f64", arith="f64")
+    D.set_params(tau=0.6, les=False, trt=True, u0=0.04, **{"lambda": 3 / 16})
+    D.reset()
+    D.voxelize_mask(box, mask)
+    q = 0.5 * 0.04 ** 2 * D.frontalArea
+    Fbar = np.zeros((N, 3))
+    Fbar[N - K:, 0] = 1.0 / (K * q)
+    D.set_gray(wg)
+    D.adjoint_forward(N)
+    g, _ = D.adjoint_backward(Fbar)
+    D.adjoint_release()
+    ok = np.flatnonzero((wg > 2 * eps) & (wg < 1 - 2 * eps))
+    cells = np.random.default_rng(5).choice(ok, 4, replace=False)
+
+    def central(c, e):
+        J, wv = [], []
+        for s in (1, -1):
+            wp = wg.copy()
+            wp[c] = np.float32(wg[c] + s * e)
+            wv.append(float(wp[c]))
+            D.reset()
+            D.set_gray(wp)
+            J.append(float(np.sum(D.step(N) * Fbar)))
+        return (J[0] - J[1]) / (wv[0] - wv[1]), 0.5 * (wv[0] - wv[1])
+
+    scale = np.abs(g[ok]).max()
+    for c in cells:
+        d1, h1 = central(c, eps)
+        d2, h2 = central(c, eps / 2)
+        rich = (h1 * h1 * d2 - h2 * h2 * d1) / (h1 * h1 - h2 * h2)
+        print(f"[adjoint fd 192] cell {c}: adjoint {g[c]:.9e} fd {d1:.9e} richardson {rich:.9e}", flush=True)
+        assert abs(g[c] - d1) <= 1e-3 * abs(d1) + 1e-4 * scale, (c, g[c], d1)
+        assert abs(g[c] - rich) <= 1e-5 * abs(rich) + 1e-7 * scale, (c, g[c], rich)
+    D.close()
