@@ -15,13 +15,13 @@ pytestmark = pytest.mark.gpu
 DIMS = (256, 24, 20)
 
 
-def run(monkeypatch, storage, pairs, les, trt, ground):
+def run(monkeypatch, storage, pairs, les, trt, ground, dims=DIMS):
     monkeypatch.setenv("WT_PAIRS", str(pairs))
-    L = LBM(*DIMS, storage=storage)
+    L = LBM(*dims, storage=storage)
     assert L.block_cells == (128 if storage == "f16" else 64)
     L.set_params(tau=0.56, les=les, trt=trt, csm=0.1, u0=0.05, **{"lambda": 3 / 16})
     L.reset()
-    nx, ny, nz = DIMS
+    nx, ny, nz = dims
     # box x 60..nx-2, y 5..16, z 4..15: a spinning sphere (wall velocity:
     # moving-wall links) and a solid column next to the outlet (x = nx - 2)
     z, y, x = np.meshgrid(np.arange(4, 16), np.arange(5, 17), np.arange(60, nx - 1), indexing="ij")
@@ -53,3 +53,18 @@ def test_pairs_equal_one_cell_step(monkeypatch, storage, les, trt, ground):
         else:
             assert np.array_equal(one[k], two[k], equal_nan=True), k
     assert np.isfinite(one["pops"]).all()
+
+
+def test_pairs_five_wave_rows(monkeypatch):
+    """Rows of 384 cells: three-wave workgroups, which take the pair step's
+    five-waves-per-SIMD variant; the same populations and fields as the
+    one-cell step bit for bit."""
+    dims = (384, 24, 20)
+    print("[pairs] f16 384-cell rows", flush=True)
+    one = run(monkeypatch, "f16", 0, True, True, True, dims)
+    two = run(monkeypatch, "f16", 1, True, True, True, dims)
+    for k in one:
+        if k.startswith("f"):
+            np.testing.assert_allclose(two[k], one[k], rtol=1e-12, atol=1e-18, err_msg=k)
+        else:
+            assert np.array_equal(one[k], two[k], equal_nan=True), k

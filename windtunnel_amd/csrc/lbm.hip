@@ -985,8 +985,8 @@ __device__ __forceinline__ void pair_store(float2 v, __amdgpu_buffer_rsrc_t r, i
 // Float32 storage the same way: a pair is two 4-byte values (one 8-byte
 // access), the x shifts move one of them across lanes, and the edge values
 // are single elements.
-template <class S, bool MACRO, bool LES, bool TRT>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WT_PAIR_WAVES)))
+template <class S, bool MACRO, bool LES, bool TRT, int WAVES>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_step_pair(StepArgs<S, float> a)
 {
     constexpr bool HALF = std::is_same<S, __half>::value;
@@ -1682,8 +1682,18 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
                 a.eqw[q] = b | (b << 16);
             }
             a.near_planes = np;
-            hipLaunchKernelGGL((k_step_pair<S, MACRO, LES, TRT>), dim3(tpr / bxp, h->ny, bz1 - bz0 + a.near_planes),
-                               dim3(bxp), 0, h->stream, a);
+            // Register budget: 4 waves per SIMD, or 5 when a workgroup's
+            // wave count does not divide the CU's 16 four-wave slots (a
+            // 640-cell row is 5 waves: 15 of 16 slots at 4, all 20 at 5;
+            // 640x320x320 steps 4.5 % faster, while 512-cell rows lose 1-3 %
+            // at 5)
+            const int wpw = bxp / 64;
+            if (16 % wpw)
+                hipLaunchKernelGGL((k_step_pair<S, MACRO, LES, TRT, WT_PAIR_WAVES + 1>),
+                                   dim3(tpr / bxp, h->ny, bz1 - bz0 + a.near_planes), dim3(bxp), 0, h->stream, a);
+            else
+                hipLaunchKernelGGL((k_step_pair<S, MACRO, LES, TRT, WT_PAIR_WAVES>),
+                                   dim3(tpr / bxp, h->ny, bz1 - bz0 + a.near_planes), dim3(bxp), 0, h->stream, a);
             paired = true;
             near_blocks = a.near_planes * prow;
         }
