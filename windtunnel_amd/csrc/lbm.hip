@@ -682,11 +682,13 @@ __device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, 
     }
 }
 
-// Where the near-wall cells are stepped: binary16 storage keeps the
-// streaming kernel within 64 VGPRs (8 waves per SIMD), where the near-wall
+// Where the near-wall cells are stepped by the one-cell kernel: binary16
+// storage keeps it within 64 VGPRs (8 waves per SIMD), where the near-wall
 // path would spill, so it steps them in a launch of its own (k_near) after
 // the streaming kernel; wider storage runs them as the first blocks of the
-// streaming launch (no second launch, no gap between launches).
+// streaming launch (no second launch, no gap between launches). The
+// binary16 pair kernel, the default binary16 step, runs them as its own
+// first blocks as well.
 // Build switch WT_EXACT_FUSED_NEAR=0: float32 storage with binary64
 // arithmetic steps its near-wall cells in a launch of their own.
 #ifndef WT_EXACT_FUSED_NEAR
@@ -945,8 +947,8 @@ __global__ void __launch_bounds__(NEAR_THREADS) k_near(StepArgs<S, MAC> a)
 // the one-cell binary16 step). The pulls along +x and -x are the word of
 // the same cells shifted across lanes (DPP wavefront shift by one lane);
 // the lane at the block's edge takes the missing half from the neighbouring
-// block, loaded once per wave by ten lanes (one per shifted direction) and
-// broadcast with readlane. Each cell's collision, rounding and results are
+// block's word, one uniform load per shifted direction and wave. Each
+// cell's collision, rounding and results are
 // those of the one-cell path bit for bit: the same code, reading the low or
 // the high half of the shared registers.
 constexpr int PAIR_DIRS[10] = {1, 7, 9, 11, 13, 2, 8, 10, 12, 14};  // e_x = +1 (5), then -1 (5)
