@@ -142,13 +142,19 @@ W_B = gray_field(DIMS_B, (40, 8, 8), 2.5, 4.5)
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_windowed_reverse_sweep_is_bit_identical(storage, monkeypatch):
     """A replay store shorter than the checkpoint segment (WT_ADJ_WINDOW)
-    recomputes more but must give the same gradient bit for bit."""
+    recomputes more but must give the same gradient bit for bit; so must the
+    sweep that replays the next segment on a second stream (the default
+    with room for two stores) against one stream (WT_ADJ_OVERLAP=0)."""
     N = 47
     Fb = fbar(N)
     print(f"[adjoint window] storage={storage}", flush=True)
     L = setup(LBM(*DIMS_B, storage=storage), DIMS_B, W_B, True, True)
     L.adjoint_forward(N, interval=10)
     g_full, _ = L.adjoint_backward(Fb)
+    monkeypatch.setenv("WT_ADJ_OVERLAP", "0")
+    g_one, _ = L.adjoint_backward(Fb)
+    monkeypatch.delenv("WT_ADJ_OVERLAP")
+    assert np.array_equal(g_one, g_full)
     for cap in (3, 1):
         monkeypatch.setenv("WT_ADJ_WINDOW", str(cap))
         g_win, _ = L.adjoint_backward(Fb)

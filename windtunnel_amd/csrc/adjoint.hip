@@ -676,7 +676,90 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
         h->seed_step = -1;
         return rc;
     };
-    for (int seg = nseg - 1; seg >= 0; seg--) {
+    // one reverse step of the sweep: step k from the replayed states
+    // store[k - s0] (s_k) and store[k - 1 - s0] (s_{k-1})
+    auto adjoint_step = [&](const std::vector<void *> &store, int s0, int k) {
+        a.mu = (const M *)mu_k;
+        a.mun = (M *)mu_n;
+        a.Sk = (const S *)store[(size_t)(k - s0)];
+        a.Sp = k > 0 ? (const S *)store[(size_t)(k - 1 - s0)] : nullptr;
+        for (int c = 0; c < 3; c++) a.Fb[c] = Fbar[3 * k + c];
+        a.k = k;
+        a.src_in = src[(size_t)k + 1];
+        a.src_out = src[(size_t)k];
+        const int rc = launch_adj<S, M, T>(h, a);
+        std::swap(mu_k, mu_n);
+        return rc;
+    };
+    // Overlap: when two replay stores of a whole segment fit, segment
+    // seg - 1 is replayed on a second stream while the adjoint steps of
+    // segment seg run on the lattice's stream (the two touch different
+    // buffers), so the small launches of each fill the other's gaps and
+    // tails. WT_ADJ_OVERLAP=0 keeps one stream; the gradient is the same bit
+    // for bit either way.
+    bool overlap = W >= K && nseg > 1;
+    if (const char *e = std::getenv("WT_ADJ_OVERLAP")) overlap = overlap && std::atoi(e) != 0;
+    if (overlap) {
+        size_t freeb = 0, total = 0;
+        WT_HIP(hipMemGetInfo(&freeb, &total));
+        const size_t reserve = std::max<size_t>((size_t)1 << 30, total / 64);
+        overlap = freeb > reserve && (freeb - reserve) / st >= (size_t)K + 1;
+    }
+    std::vector<Buf> rep2(overlap ? (size_t)K + 1 : 0);
+    // the side stream and its events live until the sweep has drained
+    struct Side {
+        hipStream_t s = nullptr;
+        std::vector<hipEvent_t> ev;
+        ~Side()
+        {
+            if (s) (void)hipStreamSynchronize(s);
+            for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } side;
+    if (overlap) {
+        for (auto &b : rep2) WT_HIP(hipMalloc(&b.p, st));
+        std::vector<void *> stores[2] = {rp, std::vector<void *>((size_t)K + 1)};
+        for (int r = 0; r <= K; r++) stores[1][(size_t)r] = rep2[(size_t)r].p;
+        WT_HIP(hipStreamCreate(&side.s));
+        side.ev.assign(2 * (size_t)nseg, nullptr);
+        for (hipEvent_t &e : side.ev) WT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        auto evB = [&](int seg) { return side.ev[(size_t)seg]; };            // replays of seg done
+        auto evA = [&](int seg) { return side.ev[(size_t)(nseg + seg)]; };   // adjoint of seg done
+        WT_HIP(hipStreamWaitEvent(side.s, h->ev0, 0));
+        // replay segment seg into a store on the side stream: s_{s0} .. s_{k1-1}
+        auto prep = [&](int seg) {
+            const int k0 = seg * K, k1 = std::min(N, k0 + K), s0 = std::max(k0 - 1, 0);
+            const std::vector<void *> &store = stores[seg & 1];
+            WT_HIP(hipMemcpyAsync(store[0], (char *)R.ck + (size_t)seg * st, st, hipMemcpyDeviceToDevice, side.s));
+            hipStream_t own = h->stream;
+            h->stream = side.s;
+            for (int r = 0; s0 + r < k1 - 1; r++) {
+                if (int rc = replay(store[(size_t)r], store[(size_t)r + 1], s0 + r)) {
+                    h->stream = own;
+                    return rc;
+                }
+            }
+            h->stream = own;
+            WT_HIP(hipEventRecord(evB(seg), side.s));
+            return 0;
+        };
+        if (int rc = prep(nseg - 1)) return rc;
+        for (int seg = nseg - 1; seg >= 0; seg--) {
+            if (seg >= 1) {
+                // the store of segment seg - 1 last served segment seg + 1
+                if (seg + 1 < nseg) WT_HIP(hipStreamWaitEvent(side.s, evA(seg + 1), 0));
+                if (int rc = prep(seg - 1)) return rc;
+            }
+            WT_HIP(hipStreamWaitEvent(h->stream, evB(seg), 0));
+            const int k0 = seg * K, k1 = std::min(N, k0 + K), s0 = std::max(k0 - 1, 0);
+            for (int k = k1 - 1; k >= k0; k--)
+                if (int rc = adjoint_step(stores[seg & 1], s0, k)) return rc;
+            WT_HIP(hipEventRecord(evA(seg), h->stream));
+        }
+        // every side-stream launch precedes evB(0), which the last segment waited for
+    }
+    for (int seg = overlap ? -1 : nseg - 1; seg >= 0; seg--) {
         // checkpoint of the segment: s_base; its reverse steps are [k0, k1)
         const int base = seg == 0 ? 0 : seg * K - 1;
         const int k0 = seg * K, k1 = std::min(N, k0 + K);
@@ -692,18 +775,8 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
             }
             for (int r = 0; s0 + r < b - 1; r++)
                 if (int rc = replay(rp[(size_t)r], rp[(size_t)r + 1], s0 + r)) return rc;
-            for (int k = b - 1; k >= a0; k--) {
-                a.mu = (const M *)mu_k;
-                a.mun = (M *)mu_n;
-                a.Sk = (const S *)rp[(size_t)(k - s0)];
-                a.Sp = k > 0 ? (const S *)rp[(size_t)(k - 1 - s0)] : nullptr;
-                for (int c = 0; c < 3; c++) a.Fb[c] = Fbar[3 * k + c];
-                a.k = k;
-                a.src_in = src[(size_t)k + 1];
-                a.src_out = src[(size_t)k];
-                if (int rc = launch_adj<S, M, T>(h, a)) return rc;
-                std::swap(mu_k, mu_n);
-            }
+            for (int k = b - 1; k >= a0; k--)
+                if (int rc = adjoint_step(rp, s0, k)) return rc;
             b = a0;
         }
     }
