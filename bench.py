@@ -22,9 +22,12 @@ Roofline: HBM bound. Algorithmic bytes per cell update = 19 populations read
 + 19 written = 152 B (float32) or 76 B (binary16); MLUPS counts all nx*ny*nz
 cells as the reference's benchmark does (js/main.js:621). `achieved` divides
 the algorithmic bytes of one launch by the average step-kernel duration,
-measured with HIP events on the lattice's stream over a second run of as
-many steps right after the timed ones (`value` is the wall clock of the
-timed steps, which carry no per-step events).
+measured with HIP events on the lattice's stream right after the timed
+steps: the device time of K + 1 back-to-back launches less that of one
+launch, over K (each region ends with the launch that writes the fields).
+`kernel_ms_per_launch_events` is the same average with an event pair around
+every launch, which adds the records' own waits. `value` is the wall clock
+of the timed steps, which carry no events.
 
 Precision leg: the same body at half scale on 128x64x64 in binary16, float32
 and binary64 storage (the binary64 GPU path equals the oracle's binary64 run
@@ -123,16 +126,24 @@ def run_gpu(wl, storage, steps, warmup, arith="f32"):
     t0 = time.perf_counter()
     forces = L.step(steps)          # returns after the stream has drained
     wall = time.perf_counter() - t0
-    # then the same number of steps with HIP events around every step launch
+    # then the step kernel's device time back to back: HIP events on the
+    # lattice's stream around K + 1 launches and around one launch (median of
+    # three); each region ends with the field-writing launch, so the
+    # difference is K launches of the timed steps' kernel
+    t_long = L.time_steps(steps + 1)
+    t_one = float(np.median([L.time_steps(1) for _ in range(3)]))
+    kavg = (t_long - t_one) / steps
+    # and with an event pair around every launch (each pair adds its own
+    # records and waits to the launch it brackets)
     L.set_timing(True)
     forces2 = L.step(steps)
     kms, launches = L.timing()
     L.set_timing(False)
     n = L.n
     bpc = 76 if storage == "f16" else 152
-    kavg = kms / launches
     res = dict(
         mlups=n * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps, kernel_ms=kavg,
+        kernel_ms_per_launch_events=kms / launches,
         achieved_gbs=n * bpc / (kavg * 1e-3) / 1e9, bytes_per_cell=bpc, block_cells=L.block_cells,
         solid_cells=L.solidCells, frontal_area=L.frontalArea, body=vinfo,
         finite=bool(np.all(np.isfinite(forces)) and np.all(np.isfinite(forces2))),
@@ -794,6 +805,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("f32"), "kernel_ms": round(r32["kernel_ms"], 5),
+                     "kernel_ms_per_launch_events": round(r32["kernel_ms_per_launch_events"], 5),
                      "bytes_per_cell": 152, "bytes_per_launch": nx * ny * nz * 152},
     }
     if not a.no_exact:
@@ -804,6 +816,7 @@ def main():
                                 "(bit for bit the reference's populations)",
                         "value": round(rx["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(rx["ms_per_step"], 4),
                         "kernel_ms": round(rx["kernel_ms"], 5), "roofline_frac": round(rx["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                        "kernel_ms_per_launch_events": round(rx["kernel_ms_per_launch_events"], 5),
                         "traffic": traffic.get("exact")}
     if not a.no_f16:
         log("f16")
@@ -814,6 +827,7 @@ def main():
                        "roofline": {"bound": "hbm", "achieved": round(r16["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                     "traffic": traffic.get("f16"), "kernel_ms": round(r16["kernel_ms"], 5),
+                                    "kernel_ms_per_launch_events": round(r16["kernel_ms_per_launch_events"], 5),
                                     "bytes_per_cell": 76, "bytes_per_launch": nx * ny * nz * 76}}
     if not a.no_micro:
         log("micro 512x256x256, no body, random populations")
