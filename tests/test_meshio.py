@@ -17,7 +17,7 @@ from windtunnel_amd import synth
 from windtunnel_amd.lbm import mesh_parse
 
 
-# every test here also runs in the GPU suite, on the shipped build of the GPU box
+# every test here also runs in the GPU suite (-m gpu), against the shipped build on the GPU machine
 @pytest.fixture(params=[pytest.param("cpu"), pytest.param("gpu", marks=pytest.mark.gpu)], autouse=True)
 def suite(request):
     return request.param

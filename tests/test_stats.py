@@ -9,7 +9,7 @@ import pytest
 from windtunnel_amd.stats import HIST, SETTLE, ForceStats, block_stats, coefficient_stats, pair_coefficients
 
 
-# every test here also runs in the GPU suite, on the shipped build of the GPU box
+# every test here also runs in the GPU suite (-m gpu), against the shipped build on the GPU machine
 @pytest.fixture(params=[pytest.param("cpu"), pytest.param("gpu", marks=pytest.mark.gpu)], autouse=True)
 def suite(request):
     return request.param
