@@ -133,6 +133,13 @@ template <class S, class M> struct AdjArgs {
     // (32 bits, which halves the scalar registers: checked on the host)
     int32_t muo[Q], spo[Q], sto[Q];
     int lsh, lstride, lmask;
+    // the same offsets for buffer resources: byte offsets from the lowest
+    // gathered element of the row (mu at muo + row + mumin, s_{k-1} at
+    // spo + row + spmin) and from the row's home element (mu_{k-1}); bufok
+    // when all of them fit 31 bits
+    int64_t mumin, spmin;
+    int32_t bmu[Q], bsp[Q], bst[Q];
+    int bufok;
     // binary16 fields: mx[j] = max |mu_j| as float bits, j = 0 .. N (filled
     // by the launch that writes mu_j; the starting field mu_{N-1} is zero and
     // mx[N - 1] = mx[N] = 0), k the reverse step, src_in / src_out =
@@ -337,10 +344,65 @@ __device__ __forceinline__ float adj_cold(const AdjArgs<S, M> &a, const MuScale 
     return m;
 }
 
+// Buffer-resource access (raw buffers: no range check, 32-bit offsets; word
+// 3 is the gfx9-family 32-bit data format). One scalar base per array and
+// row, one scalar byte offset per direction, the lane's byte offset in a
+// VGPR: no 64-bit address arithmetic per access.
+constexpr int ADJ_RAW_FLAGS = 0x00020000;
+typedef unsigned int adj_u32x2 __attribute__((ext_vector_type(2)));
+template <class E> __device__ __forceinline__ __amdgpu_buffer_rsrc_t adj_rsrc(const E *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, 0, -1, ADJ_RAW_FLAGS);
+}
+template <class E> __device__ __forceinline__ E buf_get(__amdgpu_buffer_rsrc_t r, uint32_t voff, int32_t soff)
+{
+    if constexpr (std::is_same<E, __half>::value) {
+        return __ushort_as_half(__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, soff, 0));
+    } else if constexpr (std::is_same<E, float>::value) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
+    } else {
+        const adj_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, soff, 0);
+        return __hiloint2double((int)v.y, (int)v.x);
+    }
+}
+template <class E> __device__ __forceinline__ void buf_put(E v, __amdgpu_buffer_rsrc_t r, uint32_t voff, int32_t soff)
+{
+    if constexpr (std::is_same<E, __half>::value) {
+        __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(v), r, (int)voff, soff, 0);
+    } else if constexpr (std::is_same<E, float>::value) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, soff, 0);
+    } else {
+        adj_u32x2 u;
+        u.x = (unsigned)__double2loint(v);
+        u.y = (unsigned)__double2hiint(v);
+        __builtin_amdgcn_raw_buffer_store_b64(u, r, (int)voff, soff, 0);
+    }
+}
+// Adjoint field value as T from a stored element (binary16 fields scaled).
+template <class T, class M> __device__ __forceinline__ T mget(M v, float si)
+{
+    if constexpr (std::is_same<M, __half>::value) return (T)scaled_half(v, si);
+    else return (T)v;
+}
+// State value as T from a stored element, as sval (binary16: the stored
+// offset times F16_INV_SCALE).
+template <class T, class S> __device__ __forceinline__ T sget(S v)
+{
+    if constexpr (std::is_same<S, __half>::value) return (T)scaled_half(v, wt::F16_INV_SCALE);
+    else return (T)v;
+}
+// Build switch WT_ADJ_BUF=0: 64-bit scalar bases for every access.
+#ifndef WT_ADJ_BUF
+#define WT_ADJ_BUF 1
+#endif
+
 // One hot cell: no gray weight in reach and every neighbour computed, so
 // lambda_k is the plain gather of mu_k and f_{k-1} the plain pull of s_{k-1}.
 // Returns max |mu_{k-1}| of the cell.
-template <class S, class M, class T, bool LES, bool TRT>
+// BUF: take the buffer-resource form when the offsets fit (the row kernel;
+// the fused kernel, which also carries the cold path, keeps one form so its
+// registers do not spill).
+template <class S, class M, class T, bool LES, bool TRT, bool BUF>
 __device__ __forceinline__ float adj_hot(const AdjArgs<S, M> &a, const MuScale &sc, int x, int y, int z)
 {
     const int64_t row = (int64_t)a.nx * (y + (int64_t)a.ny * z);
@@ -351,6 +413,30 @@ __device__ __forceinline__ float adj_hot(const AdjArgs<S, M> &a, const MuScale &
     const uint32_t o[3] = {e[0] * (uint32_t)sizeof(M), e[1] * (uint32_t)sizeof(M), e[2] * (uint32_t)sizeof(M)};
     const uint32_t os[3] = {e[0] * (uint32_t)sizeof(S), e[1] * (uint32_t)sizeof(S), e[2] * (uint32_t)sizeof(S)};
     T lam[Q], f[Q], mu[Q];
+#if WT_ADJ_BUF
+    if (BUF && a.bufok) {
+        const __amdgpu_buffer_rsrc_t rmu = adj_rsrc(a.mu + (rb + a.mumin)), rsp = adj_rsrc(a.Sp + (rb + a.spmin)),
+                                     rst = adj_rsrc(a.mun + rb);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            lam[q] = mget<T>(buf_get<M>(rmu, o[1 + EX[q]], a.bmu[q]), sc.si);
+            f[q] = sget<T>(buf_get<S>(rsp, os[1 - EX[q]], a.bsp[q]));
+        }
+        if (x == a.nx - 2) {
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+                if (EX[q] != 1) lam[q] += mget<T>(buf_get<M>(rmu, o[2 + EX[q]], a.bmu[q]), sc.si);
+        }
+        adj_collide<T, LES, TRT>(f, lam, a.tau, a.K, a.lambda, mu);
+        float m = 0.0f;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            buf_put<M>(mstore<M>(mu[q], sc.so), rst, o[1], a.bst[q]);
+            m = fmaxf(m, fabsf((float)mu[q]));
+        }
+        return m;
+    }
+#endif
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         lam[q] = ld_mu<T>(a.mu, a.muo[q], rb, o[1 + EX[q]], sc.si);
@@ -400,7 +486,7 @@ __global__ void WT_ADJ_ATTR k_adjoint(AdjArgs<S, M> a)
     float m = 0.0f;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = (int)blockIdx.y, z = (int)blockIdx.z;
-    if (x < a.nx && bitset(a.hot, a.nwx, a.ny, x, y, z)) m = adj_hot<S, M, T, LES, TRT>(a, sc, x, y, z);
+    if (x < a.nx && bitset(a.hot, a.nwx, a.ny, x, y, z)) m = adj_hot<S, M, T, LES, TRT, true>(a, sc, x, y, z);
     if constexpr (std::is_same<M, __half>::value) mu_max(a.mx + (a.k - 1), m);
 }
 
@@ -426,7 +512,7 @@ __global__ void WT_ADJ_ATTR k_adjoint_fused(AdjArgs<S, M> a)
         // row blocks exist only when there is a step k - 1 (a.Sp set)
         const int x = blockIdx.x * blockDim.x + threadIdx.x;
         const int y = (int)blockIdx.y, z = (int)blockIdx.z - a.cold_planes;
-        if (x < a.nx && bitset(a.hot, a.nwx, a.ny, x, y, z)) m = adj_hot<S, M, T, LES, TRT>(a, sc, x, y, z);
+        if (x < a.nx && bitset(a.hot, a.nwx, a.ny, x, y, z)) m = adj_hot<S, M, T, LES, TRT, false>(a, sc, x, y, z);
     }
     if constexpr (std::is_same<M, __half>::value)
         if (a.Sp) mu_max(a.mx + (a.k - 1), m);
@@ -624,6 +710,39 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
     WT_HIP(hipMemsetAsync(mu0.p, 0, sm, h->stream));
     WT_HIP(hipMemsetAsync(mu1.p, 0, sm, h->stream));
     WT_HIP(hipMemsetAsync(dg.p, 0, sizeof(double) * (size_t)h->n, h->stream));
+    // Overlap: when two replay stores of a whole segment fit, segment
+    // seg - 1 is replayed on a second stream while the adjoint steps of
+    // segment seg run on the lattice's stream (the two touch different
+    // buffers), so the small launches of each fill the other's gaps and
+    // tails. WT_ADJ_OVERLAP=0 keeps one stream; the gradient is the same bit
+    // for bit either way.
+    bool overlap = W >= K && nseg > 1;
+    if (const char *e = std::getenv("WT_ADJ_OVERLAP")) overlap = overlap && std::atoi(e) != 0;
+    if (overlap) {
+        size_t freeb = 0, total = 0;
+        WT_HIP(hipMemGetInfo(&freeb, &total));
+        const size_t reserve = std::max<size_t>((size_t)1 << 30, total / 64);
+        overlap = freeb > reserve && (freeb - reserve) / st >= (size_t)K + 1;
+    }
+    std::vector<Buf> rep2(overlap ? (size_t)K + 1 : 0);
+    // the side stream and its events live until the sweep has drained
+    struct Side {
+        hipStream_t s = nullptr;
+        std::vector<hipEvent_t> ev;
+        ~Side()
+        {
+            if (s) (void)hipStreamSynchronize(s);
+            for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } side;
+    if (overlap) {
+        // allocated before the timed region starts, like the first store
+        for (auto &b : rep2) WT_HIP(hipMalloc(&b.p, st));
+        WT_HIP(hipStreamCreate(&side.s));
+        side.ev.assign(2 * (size_t)nseg, nullptr);
+        for (hipEvent_t &e : side.ev) WT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
 
     WT_HIP(hipEventRecord(h->ev0, h->stream));
     AdjArgs<S, M> a{};
@@ -660,6 +779,18 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
         a.spo[q] = (int32_t)so;
         a.sto[q] = (int32_t)slot;
     }
+    a.mumin = *std::min_element(a.muo, a.muo + Q);
+    a.spmin = *std::min_element(a.spo, a.spo + Q);
+    a.bufok = 1;
+    for (int q = 0; q < Q; q++) {
+        const int64_t bm = (a.muo[q] - a.mumin) * (int64_t)sizeof(M), bs = (a.spo[q] - a.spmin) * (int64_t)sizeof(S),
+                      bt = a.sto[q] * (int64_t)sizeof(M);
+        const bool fit = bm <= INT32_MAX && bs <= INT32_MAX && bt >= 0 && bt <= INT32_MAX;
+        a.bufok = a.bufok && fit;
+        a.bmu[q] = fit ? (int32_t)bm : 0;
+        a.bsp[q] = fit ? (int32_t)bs : 0;
+        a.bst[q] = fit ? (int32_t)bt : 0;
+    }
     void *mu_k = mu0.p, *mu_n = mu1.p;
     // the replays below drive the forward kernel on explicit buffers; the
     // lattice's own state and counters are left untouched
@@ -691,39 +822,9 @@ int reverse_t(wt_lattice *h, const AdjRecord &R, const double *Fbar, double *dJd
         std::swap(mu_k, mu_n);
         return rc;
     };
-    // Overlap: when two replay stores of a whole segment fit, segment
-    // seg - 1 is replayed on a second stream while the adjoint steps of
-    // segment seg run on the lattice's stream (the two touch different
-    // buffers), so the small launches of each fill the other's gaps and
-    // tails. WT_ADJ_OVERLAP=0 keeps one stream; the gradient is the same bit
-    // for bit either way.
-    bool overlap = W >= K && nseg > 1;
-    if (const char *e = std::getenv("WT_ADJ_OVERLAP")) overlap = overlap && std::atoi(e) != 0;
     if (overlap) {
-        size_t freeb = 0, total = 0;
-        WT_HIP(hipMemGetInfo(&freeb, &total));
-        const size_t reserve = std::max<size_t>((size_t)1 << 30, total / 64);
-        overlap = freeb > reserve && (freeb - reserve) / st >= (size_t)K + 1;
-    }
-    std::vector<Buf> rep2(overlap ? (size_t)K + 1 : 0);
-    // the side stream and its events live until the sweep has drained
-    struct Side {
-        hipStream_t s = nullptr;
-        std::vector<hipEvent_t> ev;
-        ~Side()
-        {
-            if (s) (void)hipStreamSynchronize(s);
-            for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
-            if (s) (void)hipStreamDestroy(s);
-        }
-    } side;
-    if (overlap) {
-        for (auto &b : rep2) WT_HIP(hipMalloc(&b.p, st));
         std::vector<void *> stores[2] = {rp, std::vector<void *>((size_t)K + 1)};
         for (int r = 0; r <= K; r++) stores[1][(size_t)r] = rep2[(size_t)r].p;
-        WT_HIP(hipStreamCreate(&side.s));
-        side.ev.assign(2 * (size_t)nseg, nullptr);
-        for (hipEvent_t &e : side.ev) WT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         auto evB = [&](int seg) { return side.ev[(size_t)seg]; };            // replays of seg done
         auto evA = [&](int seg) { return side.ev[(size_t)(nseg + seg)]; };   // adjoint of seg done
         WT_HIP(hipStreamWaitEvent(side.s, h->ev0, 0));
