@@ -235,6 +235,7 @@ template <class S, class MAC> struct StepArgs {
     int revy, revz;  // sweep the rows with y / z descending
     uint32_t seed;   // this step's seed of the binary16 rounding bits
     double tau, sq18cs2, lambda, u0;
+    double om;  // 1/tau, divided on the host (the same IEEE quotient as on the device)
     float tauf, omf, sq18f, lamf;  // the same in float: tau, 1/tau, 18 sqrt(2) Cs^2, Lambda
     double eqv[Q];  // boundary equilibrium deviation at (rho=1, u0, 0, 0)
     // Interior-cell addressing. Rows are whole storage blocks, so the
@@ -282,6 +283,55 @@ __device__ __forceinline__ bool bit(const uint32_t *bits, int nwx, int ny, int x
 #define WT_FAST_COLLIDE 1
 #endif
 
+// IEEE quotient and square root of binary64 without the range scaling and
+// special-case fix-ups of the general sequences. The refinement is the one
+// a binary64 '/' and sqrt() compile to on gfx950 (hardware reciprocal or
+// reciprocal square root, two Newton-Raphson steps, a remainder-corrected
+// final FMA), so the results are the correctly rounded ones, bit for bit,
+// whenever no scaling would apply: divisors and quotients far inside the
+// normal range, and square-root arguments of at least 2^-767. Every
+// operand of the collision satisfies that (densities and relaxation times
+// near 1, sums of squares of deviations that are 0 or above 1e-200);
+// NaN stays non-finite, and sqrt_nz maps 0 to 0.
+template <class T> __device__ __forceinline__ T rdiv(T a, T b) { return a / b; }
+template <class T> __device__ __forceinline__ T rrcp(T b) { return T(1) / b; }
+template <class T> __device__ __forceinline__ T sqrt_pos(T x) { return sqrt(x); }
+template <class T> __device__ __forceinline__ T sqrt_nz(T x) { return sqrt(x); }
+__device__ __forceinline__ double rcp_refined(double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    y = __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+    return __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+}
+template <> __device__ __forceinline__ double rdiv<double>(double a, double b)
+{
+    const double y = rcp_refined(b), q = a * y;
+    return __builtin_fma(__builtin_fma(-b, q, a), y, q);
+}
+// a = 1: q = y exactly, so the final step needs no product
+template <> __device__ __forceinline__ double rrcp<double>(double b)
+{
+    const double y = rcp_refined(b);
+    return __builtin_fma(__builtin_fma(-b, y, 1.0), y, y);
+}
+template <> __device__ __forceinline__ double sqrt_pos<double>(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, h = y * 0.5;
+    const double e = __builtin_fma(-h, s, 0.5);
+    s = __builtin_fma(s, e, s);
+    h = __builtin_fma(h, e, h);
+    s = __builtin_fma(__builtin_fma(-s, s, x), h, s);
+    return __builtin_fma(__builtin_fma(-s, s, x), h, s);
+}
+template <> __device__ __forceinline__ double sqrt_nz<double>(double x) { return x > 0.0 ? sqrt_pos(x) : x; }
+
+template <class T, class A> __device__ __forceinline__ T omega0_of(const A &a)
+{
+    if constexpr (std::is_same<T, double>::value) return a.om;
+    else return T(1) / (T)a.tau;
+}
+
 // Moments, deviation equilibrium, LES and TRT/BGK relaxation of one cell
 // (js/lbm.js:377-460) in the reference's expression order (up to exact
 // factors of two, below).
@@ -295,7 +345,7 @@ __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq1
     const T dr = f0 + f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9 + f10 + f11 + f12 + f13 + f14 +
                  f15 + f16 + f17 + f18;
     r = T(1) + dr;
-    const T inv = T(1) / r;
+    const T inv = rrcp(r);
     vx = (f1 - f2 + f7 - f8 + f9 - f10 + f11 - f12 + f13 - f14) * inv;
     vy = (f3 - f4 + f7 - f8 - f9 + f10 + f15 - f16 + f17 - f18) * inv;
     vz = (f5 - f6 + f11 - f12 - f13 + f14 + f15 - f16 - f17 + f18) * inv;
@@ -330,11 +380,11 @@ __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq1
         const T pxy = n7 + n8 - n9 - n10;
         const T pxz = n11 + n12 - n13 - n14;
         const T pyz = n15 + n16 - n17 - n18;
-        const T mag = sqrt(T(2) * (pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz)));
-        tauEff = T(0.5) * (tau0 + sqrt(tau0 * tau0 + sq18cs2 * mag * inv));
-        om = T(1) / tauEff;
+        const T mag = sqrt_nz(T(2) * (pxx * pxx + pyy * pyy + pzz * pzz + T(2) * (pxy * pxy + pxz * pxz + pyz * pyz)));
+        tauEff = T(0.5) * (tau0 + sqrt_pos(tau0 * tau0 + sq18cs2 * mag * inv));
+        om = rrcp(tauEff);
     }
-    const T omm = TRT ? T(1) / (T(0.5) + lam / (tauEff - T(0.5))) : om;
+    const T omm = TRT ? rrcp(T(0.5) + rdiv(lam, tauEff - T(0.5))) : om;
 
     g[0] = f0 - om * (f0 - e0);
     // om * (0.5 (n_a + n_b)) rounds to exactly (0.5 om) * (n_a + n_b): a
@@ -668,7 +718,7 @@ __device__ __forceinline__ void near_cell(const StepArgs<S, MAC> &a, int64_t i, 
         collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
                                vx, vy, vz);
     else
-        collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+        collide<T, LES, TRT>(f, (T)a.tau, omega0_of<T>(a), (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         put(q, i, x, y, z, g[q]);
@@ -775,7 +825,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<S, MAC> &a, int x, int 
                     if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
                         collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r, vx, vy, vz);
                     else
-                        collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+                        collide<T, LES, TRT>(f, (T)a.tau, omega0_of<T>(a), (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
 #pragma unroll
                     for (int q = 0; q < Q; q++) buf_st<S>((S)g[q], rg, (int)o[1], a.soff[q]);
                     if (out) {
@@ -806,7 +856,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<S, MAC> &a, int x, int 
                     collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r,
                                            vx, vy, vz);
                 else
-                    collide<T, LES, TRT>(f, (T)a.tau, T(1) / (T)a.tau, (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+                    collide<T, LES, TRT>(f, (T)a.tau, omega0_of<T>(a), (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
             }
             const uint32_t rc = cell_bits<S>(i, a.seed);
             // the post-collision values of direction q go to lane offset os(q, xo)
@@ -1610,6 +1660,7 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     a.revz = (h->sweep & 1) ? odd : 0;
     a.revy = (h->sweep & 2) ? odd : 0;
     a.tau = h->p.tau; a.sq18cs2 = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
+    a.om = 1 / a.tau;
     a.tauf = (float)a.tau; a.omf = (float)(1 / a.tau); a.sq18f = (float)a.sq18cs2; a.lamf = (float)a.lambda;
     a.u0 = h->p.u0;
     eq_deviation(1, h->p.u0, 0, 0, a.eqv);
