@@ -611,6 +611,13 @@ template <> __device__ __forceinline__ void buf_st<double>(double v, __amdgpu_bu
 // 512x256x256 on MI355X). Float arithmetic keeps the 64-bit scalar bases,
 // which measured 3 % faster there than buffer resources. Build switch
 // WT_BUF_STEP=0: 64-bit scalar bases for binary64 arithmetic too.
+// Wave priority (0-3) while a binary64-arithmetic cell issues its 19 pulls:
+// waves about to load go ahead of waves in their long binary64 collision,
+// so more loads are in flight (+0.7 % at 512x256x256; on the float path the
+// same raise costs 1.2 %, and it is not used there).
+#ifndef WT_LOAD_PRIO
+#define WT_LOAD_PRIO 2
+#endif
 #ifndef WT_BUF_STEP
 #define WT_BUF_STEP 1
 #endif
@@ -820,8 +827,10 @@ __device__ __forceinline__ void step_cell(const StepArgs<S, MAC> &a, int x, int 
                         (void *)((const char *)a.F + (rb * (int64_t)sizeof(S) + a.lmin)), 0, -1, RAW_BUFFER_FLAGS);
                     const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc(
                         (void *)((char *)a.G + rb * (int64_t)sizeof(S)), 0, -1, RAW_BUFFER_FLAGS);
+                    if (WT_LOAD_PRIO) __builtin_amdgcn_s_setprio(WT_LOAD_PRIO);
 #pragma unroll
                     for (int q = 0; q < Q; q++) f[q] = (T)buf_ld<S>(rf, (int)o[1 - EX[q]], a.loff[q]);
+                    if (WT_LOAD_PRIO) __builtin_amdgcn_s_setprio(0);
                     if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
                         collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r, vx, vy, vz);
                     else
@@ -1006,6 +1015,11 @@ constexpr int PAIR_DIRS[10] = {1, 7, 9, 11, 13, 2, 8, 10, 12, 14};  // e_x = +1 
 // with as many cells as eight one-cell waves, and the near-wall path fits
 // beside the streaming path without spilling, so the near-wall cells run
 // as the first blocks of the same launch.
+// Wave priority (0-3) while a pair of cells issues its pulls and edge loads
+// (binary16: 1.8 % faster at 512x256x256).
+#ifndef WT_PAIR_PRIO
+#define WT_PAIR_PRIO 2
+#endif
 #ifndef WT_PAIR_WAVES
 #define WT_PAIR_WAVES 4
 #endif
@@ -1091,6 +1105,7 @@ k_step_pair(StepArgs<S, float> a)
         __builtin_amdgcn_make_buffer_rsrc((void *)((char *)a.G + rb * (int64_t)sizeof(S)), 0, -1, RAW_BUFFER_FLAGS);
     // every lane of the row pulls (the shifted pairs need their neighbours)
     Word w[Q];
+    if (WT_PAIR_PRIO) __builtin_amdgcn_s_setprio(WT_PAIR_PRIO);
 #pragma unroll
     for (int q = 0; q < Q; q++) w[q] = pair_load<Word>(rf, (int)eb, a.loff[q]);
     // the value beyond the wave's edge of each direction PAIR_DIRS[k]
@@ -1107,6 +1122,7 @@ k_step_pair(StepArgs<S, float> a)
         for (int k = 0; k < 10; k++)
             ev[k] = __builtin_amdgcn_raw_buffer_load_b32(rf, 0, a.loff[PAIR_DIRS[k]] + (k < 5 ? eb_lo : eb_hi), 0);
     }
+    if (WT_PAIR_PRIO) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int k = 0; k < 10; k++) {
         const int q = PAIR_DIRS[k];
