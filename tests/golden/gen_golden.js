@@ -493,7 +493,7 @@ if (want('vox_meshcube_lookup')) {
 }
 
 if (want('mesh_io')) {
-  // MeshIO.parse (js/meshio.js:322-340) on the files of gen_meshio.py, under
+  // MeshIO.parse (js/meshio.js:322-340) and MeshIO.meta on the files of gen_meshio.py, under
   // their own names and, for the ones with a mesh extension, under a name
   // whose extension the dispatch does not know (so the content is sniffed)
   const dir = path.join(OUT, 'meshio');
@@ -504,14 +504,17 @@ if (want('mesh_io')) {
     for (const name of names) {
       const b = fs.readFileSync(path.join(dir, file));
       const ab = b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength);
+      // MeshIO.meta (js/meshio.js:277-314): the credits the file carries
+      let meta;
+      try { meta = { meta: MeshIO.meta(name, ab) }; } catch (e) { meta = { meta_error: e.message }; }
       try {
         const P = MeshIO.parse(name, ab);
         let nan = 0;
         for (let i = 0; i < P.length; i++) if (P[i] !== P[i]) nan++;
-        cases.push({ file, name, ok: true, count: P.length, sha256: sha(P), nan,
-          head: Array.from(P.subarray(0, 9)).map(v => (v !== v ? 'NaN' : v)) });
+        cases.push(Object.assign({ file, name, ok: true, count: P.length, sha256: sha(P), nan,
+          head: Array.from(P.subarray(0, 9)).map(v => (v !== v ? 'NaN' : v)) }, meta));
       } catch (e) {
-        cases.push({ file, name, ok: false, error: e.message });
+        cases.push(Object.assign({ file, name, ok: false, error: e.message }, meta));
       }
     }
   }

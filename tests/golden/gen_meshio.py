@@ -183,6 +183,35 @@ def main():
     write("model.gltf", json.dumps(g).encode())
     write("unknown.txt", b"hello world\nthis is not a mesh\n")
 
+    # attribution text for MeshIO.meta: OBJ comment headers (field names in
+    # several spellings, an empty value, control characters, more lines and
+    # longer lines than are kept, a comment after the first statement), a
+    # binary STL header with fields, glTF asset credits (extras with a
+    # non-string value, a copyright string that carries the fields itself)
+    # and a plain .gltf, which meta reads but parse refuses
+    tri = "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"
+    write("credits.obj", ("# Title: Seeded Car\n#   Author = A. Person\n## License: CC-BY-4.0\n"
+                          "# source: https://example.org/car\n#\n# notes \x01\x02 with controls\n\n" + tri).encode())
+    write("credits_long.obj", ("".join(f"# line {i} {'x' * (40 * i)}\n" for i in range(16)) + "# by: later\n"
+                               + tri).encode())
+    write("credits_url.obj", ("  # name : Wing\n# creator:   \n# artist: B. Other\n# url = http://e.org/w\n"
+                              "v 0 0 0\n# author: after\n" + tri).encode())
+    write("credits_binary.stl", stl_binary(tris[:2], b"name=Binary Wing\tlicence: MIT"))
+    g5 = dict(g, asset={"version": "2.0", "generator": "seeded exporter 1.0", "copyright": "(c) 2024 Someone",
+                        "extras": {"title": "Extras Title", "author": "Extras Author", "license": "CC0",
+                                   "source": "https://example.org/m", "year": 2024}})
+    write("credits.glb", glb(g5, binb))
+    g6 = dict(g, asset={"version": "2.0", "copyright": "Title: Packed\nAuthor: C. Third\nLicense = CC-BY\nurl: x",
+                        "extras": {"title": "lost"}})
+    write("credits_copyright.glb", glb(g6, binb))
+    write("credits_many.obj", ("".join(f"#c{i}\n" for i in range(12)) + "# author: thirteenth\n# url: fourteenth\n"
+                               + tri).encode())
+    g7 = dict(g, asset={"version": "2.0", "copyright": "Author: Z\nTitle: dropped",
+                        "extras": {"title": "kept", "license": "MIT"}})
+    write("credits_license.glb", glb(g7, binb))
+    write("credits.gltf", json.dumps(dict(g, asset={"version": "2.0", "generator": "gltf \u00e9 writer \u2028 sep",
+                                                    "extras": {"author": 7}})).encode())
+
 
 if __name__ == "__main__":
     main()

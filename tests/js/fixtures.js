@@ -40,4 +40,20 @@ function replay(L, rec) {
   L.voxelizeMask(rec.box, arr(rec.mask), arr(rec.wv), rec.ground, rec.roadVel);
 }
 
-module.exports = { load, arr, sha, assert, close, applyParams, replay };
+// MeshIO.meta against the reference's credits for every mesh_io file, under
+// each name it was run under (meta needs no device); returns the case count
+function checkMeshMeta(MeshIO) {
+  const canon = (v) => JSON.stringify(v && typeof v === 'object'
+    ? Object.keys(v).sort().reduce((o, k) => { o[k] = v[k]; return o; }, {}) : v);
+  let n = 0;
+  for (const c of load('mesh_io').cases) {
+    const b = fs.readFileSync(path.join(DIR, 'meshio', c.file));
+    const got = MeshIO.meta(c.name, b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength));
+    assert(!('meta_error' in c), c.name + ': the reference threw');
+    assert(canon(got) === canon(c.meta), `${c.name} meta: ${canon(got)} vs ${canon(c.meta)}`);
+    n++;
+  }
+  return n;
+}
+
+module.exports = { load, arr, sha, assert, close, applyParams, replay, checkMeshMeta };

@@ -4,7 +4,7 @@
  * rejects so the page can fall back (js/main.js:1414-1418). */
 'use strict';
 const path = require('path');
-const { load, assert } = require('./fixtures.js');
+const { load, assert, checkMeshMeta } = require('./fixtures.js');
 const JS = path.join(__dirname, '..', '..', 'windtunnel_amd', 'js');
 const { LBM } = require(path.join(JS, 'lbm.js'));
 const { Pool, unavailableReason, suggestedWorkers } = require(path.join(JS, 'pool.js'));
@@ -50,6 +50,7 @@ const Voxel = require(path.join(JS, 'voxel.js'));
     const h = crypto.createHash('sha256').update(Buffer.from(P.buffer, P.byteOffset, P.byteLength)).digest('hex');
     assert(h === c.sha256, c.name + ' soup');
   }
+  checkMeshMeta(MeshIO);
 
   // a worker that fails rejects the pool's promise, leaving the caller to fall back
   const sim = { nx: 8, ny: 8, nz: 8, tau: 0.6, les: true, u0: 0.05, movingWalls: false, parity: 0, steps: 0,

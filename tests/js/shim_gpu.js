@@ -3,7 +3,7 @@
  * through windtunnel_amd/js against golden vectors from the reference. */
 'use strict';
 const path = require('path');
-const { load, arr, sha, assert, close, applyParams, replay } = require('./fixtures.js');
+const { load, arr, sha, assert, close, applyParams, replay, checkMeshMeta } = require('./fixtures.js');
 const JS = path.join(__dirname, '..', '..', 'windtunnel_amd', 'js');
 const { LBM, native } = require(path.join(JS, 'lbm.js'));
 const { Pool } = require(path.join(JS, 'pool.js'));
@@ -259,6 +259,11 @@ check('Voxel.build is bit-exact (vox_box, vox_hitcap, vox_sphere)', () => {
   let msg = null;
   try { Voxel.build(new Float32Array(0)); } catch (e) { msg = e.message; }
   assert(msg === load('vox_errors').empty, 'error message');
+});
+
+check('MeshIO.meta gives the reference\'s credits for every mesh_io file', () => {
+  const n = checkMeshMeta(require(path.join(JS, 'meshio.js')));
+  assert(n >= 45, 'mesh_io cases: ' + n);
 });
 
 check('ForceStats folds the drop-in\'s step forces into the app\'s run report', () => {
