@@ -23,8 +23,9 @@ Roofline: HBM bound. Algorithmic bytes per cell update = 19 populations read
 cells as the reference's benchmark does (js/main.js:621). `achieved` divides
 the algorithmic bytes of one launch by the average step-kernel duration,
 measured with HIP events on the lattice's stream right after the timed
-steps: the device time of K + 1 back-to-back launches less that of one
-launch, over K (each region ends with the launch that writes the fields).
+steps: the device time of N + 1 back-to-back launches less that of one
+launch, over N = max(K, 100) (each region ends with the launch that writes
+the fields; `kernel_launches` is N).
 `kernel_ms_per_launch_events` is the same average with an event pair around
 every launch, which adds the records' own waits. `value` is the wall clock
 of the timed steps, which carry no events.
@@ -127,12 +128,17 @@ def run_gpu(wl, storage, steps, warmup, arith="f32"):
     forces = L.step(steps)          # returns after the stream has drained
     wall = time.perf_counter() - t0
     # then the step kernel's device time back to back: HIP events on the
-    # lattice's stream around K + 1 launches and around one launch (median of
+    # lattice's stream around N + 1 launches and around one launch (median of
     # three); each region ends with the field-writing launch, so the
-    # difference is K launches of the timed steps' kernel
-    t_long = L.time_steps(steps + 1)
+    # difference is N launches of the timed steps' kernel. N = max(K, 100):
+    # the binary64-arithmetic and binary16 steps run several % slower for the
+    # first ~20 ms of a lattice's run (the trace shows it in every leg),
+    # and 100 launches keep that transient from setting the figure when K
+    # is small
+    nk = max(steps, 100)
+    t_long = L.time_steps(nk + 1)
     t_one = float(np.median([L.time_steps(1) for _ in range(3)]))
-    kavg = (t_long - t_one) / steps
+    kavg = (t_long - t_one) / nk
     # and with an event pair around every launch (each pair adds its own
     # records and waits to the launch it brackets)
     L.set_timing(True)
@@ -142,7 +148,7 @@ def run_gpu(wl, storage, steps, warmup, arith="f32"):
     n = L.n
     bpc = 76 if storage == "f16" else 152
     res = dict(
-        mlups=n * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps, kernel_ms=kavg,
+        mlups=n * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps, kernel_ms=kavg, kernel_launches=nk,
         kernel_ms_per_launch_events=kms / launches,
         achieved_gbs=n * bpc / (kavg * 1e-3) / 1e9, bytes_per_cell=bpc, block_cells=L.block_cells,
         solid_cells=L.solidCells, frontal_area=L.frontalArea, body=vinfo,
@@ -805,6 +811,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("f32"), "kernel_ms": round(r32["kernel_ms"], 5),
+                     "kernel_launches": r32["kernel_launches"],
                      "kernel_ms_per_launch_events": round(r32["kernel_ms_per_launch_events"], 5),
                      "bytes_per_cell": 152, "bytes_per_launch": nx * ny * nz * 152},
     }
@@ -815,7 +822,8 @@ def main():
         out["exact"] = {"what": "float32 storage, binary64 collision arithmetic in the reference's expression order "
                                 "(bit for bit the reference's populations)",
                         "value": round(rx["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(rx["ms_per_step"], 4),
-                        "kernel_ms": round(rx["kernel_ms"], 5), "roofline_frac": round(rx["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                        "kernel_ms": round(rx["kernel_ms"], 5), "kernel_launches": rx["kernel_launches"],
+                        "roofline_frac": round(rx["achieved_gbs"] / HBM_PEAK_GBS, 4),
                         "kernel_ms_per_launch_events": round(rx["kernel_ms_per_launch_events"], 5),
                         "traffic": traffic.get("exact")}
     if not a.no_f16:
@@ -827,6 +835,7 @@ def main():
                        "roofline": {"bound": "hbm", "achieved": round(r16["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
                                     "traffic": traffic.get("f16"), "kernel_ms": round(r16["kernel_ms"], 5),
+                                    "kernel_launches": r16["kernel_launches"],
                                     "kernel_ms_per_launch_events": round(r16["kernel_ms_per_launch_events"], 5),
                                     "bytes_per_cell": 76, "bytes_per_launch": nx * ny * nz * 76}}
     if not a.no_micro:
