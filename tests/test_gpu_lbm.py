@@ -196,6 +196,31 @@ def test_fp64_storage_matches_fp64_oracle():
         assert np.array_equal(L.fields()[k], O.array(k))
 
 
+@pytest.mark.parametrize("les", [False, True])
+def test_exact_mode_tau_one_half(les):
+    """tau = 0.5 exactly: TRT's Lambda / (tau_eff - 1/2) divides by zero in
+    every cell at rest, where IEEE division gives an infinity and the
+    antisymmetric rate 1 / (1/2 + inf) = 0. Binary64 arithmetic must still
+    equal the oracle (IEEE division in the reference's order) bit for bit."""
+    fx = load("lbm_trt_les")
+    p = dict(fx["params"], tau=0.5, les=les, trt=True)
+    L = LBM(*fx["dims"], arith="f64")
+    set_params(L, p)
+    L.reset()
+    replay_voxelize(L, fx["voxelize"])
+    O = OracleLBM(*fx["dims"], prec=4)
+    set_params(O, p)
+    O.reset()
+    replay_voxelize(O, fx["voxelize"])
+    L.step(20)
+    O.step(20)
+    want = O.array("f")
+    assert np.isfinite(want).any()
+    assert np.array_equal(L.f, want, equal_nan=True)
+    for k in FIELDS:
+        assert np.array_equal(L.fields()[k], O.array(k), equal_nan=True)
+
+
 @pytest.mark.parametrize("block_shift", LAYOUTS, indirect=True)
 def test_pool_slabs_exact(block_shift):
     """Worker-pool form: 3 slabs, per-slab forces, fields equal to the fixture."""

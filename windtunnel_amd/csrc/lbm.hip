@@ -384,7 +384,10 @@ __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq1
         tauEff = T(0.5) * (tau0 + sqrt_pos(tau0 * tau0 + sq18cs2 * mag * inv));
         om = rrcp(tauEff);
     }
-    const T omm = TRT ? rrcp(T(0.5) + rdiv(lam, tauEff - T(0.5))) : om;
+    // tauEff = 0.5 exactly (tau = 0.5 in a cell at rest) divides by zero:
+    // IEEE gives lam/0 = +-inf and 1/(0.5 +- inf) = +-0 (NaN for lam = 0)
+    const T dt = tauEff - T(0.5);
+    const T omm = TRT ? (dt != T(0) ? rrcp(T(0.5) + rdiv(lam, dt)) : T(1) / (T(0.5) + lam / dt)) : om;
 
     g[0] = f0 - om * (f0 - e0);
     // om * (0.5 (n_a + n_b)) rounds to exactly (0.5 om) * (n_a + n_b): a
