@@ -196,12 +196,14 @@ def test_fp64_storage_matches_fp64_oracle():
         assert np.array_equal(L.fields()[k], O.array(k))
 
 
+@pytest.mark.parametrize("block_shift", LAYOUTS, indirect=True)
 @pytest.mark.parametrize("les", [False, True])
-def test_exact_mode_tau_one_half(les):
+def test_exact_mode_tau_one_half(les, block_shift):
     """tau = 0.5 exactly: TRT's Lambda / (tau_eff - 1/2) divides by zero in
     every cell at rest, where IEEE division gives an infinity and the
     antisymmetric rate 1 / (1/2 + inf) = 0. Binary64 arithmetic must still
-    equal the oracle (IEEE division in the reference's order) bit for bit."""
+    equal the oracle (IEEE division in the reference's order) bit for bit,
+    in the planar layout and in blocks (the interior cells' straight path)."""
     fx = load("lbm_trt_les")
     p = dict(fx["params"], tau=0.5, les=les, trt=True)
     L = LBM(*fx["dims"], arith="f64")

@@ -236,6 +236,7 @@ template <class S, class MAC> struct StepArgs {
     uint32_t seed;   // this step's seed of the binary16 rounding bits
     double tau, sq18cs2, lambda, u0;
     double om;  // 1/tau, divided on the host (the same IEEE quotient as on the device)
+    int tau_le_half;  // tau <= 1/2: the straight path takes the guarded collision
     float tauf, omf, sq18f, lamf;  // the same in float: tau, 1/tau, 18 sqrt(2) Cs^2, Lambda
     double eqv[Q];  // boundary equilibrium deviation at (rho=1, u0, 0, 0)
     // Interior-cell addressing. Rows are whole storage blocks, so the
@@ -335,7 +336,9 @@ template <class T, class A> __device__ __forceinline__ T omega0_of(const A &a)
 // Moments, deviation equilibrium, LES and TRT/BGK relaxation of one cell
 // (js/lbm.js:377-460) in the reference's expression order (up to exact
 // factors of two, below).
-template <class T, bool LES, bool TRT>
+// GUARD: tau_eff = 1/2 exactly takes the IEEE operators (needed only when
+// tau <= 1/2: for tau > 1/2, tau_eff >= tau > 1/2 in every cell).
+template <class T, bool LES, bool TRT, bool GUARD = true>
 __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq18cs2, T lam, T (&g)[Q],
                                         T &r, T &vx, T &vy, T &vz)
 {
@@ -387,7 +390,11 @@ __device__ __forceinline__ void collide(const T (&f)[Q], T tau0, T omega0, T sq1
     // tauEff = 0.5 exactly (tau = 0.5 in a cell at rest) divides by zero:
     // IEEE gives lam/0 = +-inf and 1/(0.5 +- inf) = +-0 (NaN for lam = 0)
     const T dt = tauEff - T(0.5);
-    const T omm = TRT ? (dt != T(0) ? rrcp(T(0.5) + rdiv(lam, dt)) : T(1) / (T(0.5) + lam / dt)) : om;
+    T omm = om;
+    if (TRT) {
+        if constexpr (GUARD) omm = dt != T(0) ? rrcp(T(0.5) + rdiv(lam, dt)) : T(1) / (T(0.5) + lam / dt);
+        else omm = rrcp(T(0.5) + rdiv(lam, dt));
+    }
 
     g[0] = f0 - om * (f0 - e0);
     // om * (0.5 (n_a + n_b)) rounds to exactly (0.5 om) * (n_a + n_b): a
@@ -836,8 +843,10 @@ __device__ __forceinline__ void step_cell(const StepArgs<S, MAC> &a, int x, int 
                     if (WT_LOAD_PRIO) __builtin_amdgcn_s_setprio(0);
                     if constexpr (std::is_same<T, float>::value && WT_FAST_COLLIDE)
                         collide_fast<LES, TRT>(f, a.tauf, a.omf, a.sq18f, a.lamf, g, r, vx, vy, vz);
+                    else if (a.tau_le_half)
+                        collide<T, LES, TRT, true>(f, (T)a.tau, omega0_of<T>(a), (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
                     else
-                        collide<T, LES, TRT>(f, (T)a.tau, omega0_of<T>(a), (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
+                        collide<T, LES, TRT, false>(f, (T)a.tau, omega0_of<T>(a), (T)a.sq18cs2, (T)a.lambda, g, r, vx, vy, vz);
 #pragma unroll
                     for (int q = 0; q < Q; q++) buf_st<S>((S)g[q], rg, (int)o[1], a.soff[q]);
                     if (out) {
@@ -1680,6 +1689,7 @@ int step_t(wt_lattice *h, int bz0, int bz1, double *d_force)
     a.revy = (h->sweep & 2) ? odd : 0;
     a.tau = h->p.tau; a.sq18cs2 = (18 * std::sqrt(2.0)) * (h->p.csm * h->p.csm); a.lambda = h->p.lambda;
     a.om = 1 / a.tau;
+    a.tau_le_half = !(a.tau > 0.5);
     a.tauf = (float)a.tau; a.omf = (float)(1 / a.tau); a.sq18f = (float)a.sq18cs2; a.lamf = (float)a.lambda;
     a.u0 = h->p.u0;
     eq_deviation(1, h->p.u0, 0, 0, a.eqv);
